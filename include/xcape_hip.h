@@ -1,0 +1,182 @@
+/*
+ * xcape-hip: C-ABI of the gfx950 column parcel integrator.
+ *
+ * Plain pointers and sizes; no C++ or torch types.  Every entry point returns
+ * 0 on success or a negative XCAPE_E* code; xcape_last_error() then gives a
+ * message (thread-local).  Calls are reentrant: concurrent calls from several
+ * host threads run on per-thread HIP streams unless a stream is passed.
+ *
+ * Array layout is the reference's: a profile array of nk levels for ncol
+ * columns is the Fortran-order (nk, ncol) array of the reference's f2py
+ * routines, i.e. C-order (ncol, nk) with the levels of a column contiguous.
+ * Units are the reference's: hPa and degC in, J/kg and m out.
+ *
+ * Pointers are host pointers unless XCAPE_DEVICE_PTRS is set, in which case
+ * every array argument is a device pointer and the call is asynchronous on
+ * `stream` (a hipStream_t, or NULL for the calling thread's stream) unless
+ * `stats` is non-NULL (the counters are then read back synchronously).
+ */
+#ifndef XCAPE_HIP_H
+#define XCAPE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define XCAPE_OK 0
+#define XCAPE_EINVAL (-1)   /* bad argument (message says which)            */
+#define XCAPE_EHIP (-2)     /* HIP runtime error                            */
+#define XCAPE_ENOMEM (-3)   /* device allocation failed                     */
+#define XCAPE_ENODEV (-4)   /* no usable gfx950 device                      */
+#define XCAPE_ESTATE (-5)   /* entry point needs state that is not set up   */
+
+/* ---- flags ---------------------------------------------------------------- */
+#define XCAPE_DEVICE_PTRS 0x1u  /* array arguments are device pointers        */
+#define XCAPE_FAST_MATH 0x2u    /* hardware exp2/log2/rcp variant (a few ulp)  */
+#define XCAPE_WARM_START 0x4u   /* learned warm start of the moist-adiabat
+                                   solve (weights from xcape_surrogate_load)   */
+
+/* enums as the reference's Python layer maps them (core.py:302-311) */
+#define XCAPE_SOURCE_SURFACE 1
+#define XCAPE_SOURCE_MOST_UNSTABLE 2
+#define XCAPE_SOURCE_MIXED_LAYER 3
+#define XCAPE_ADIABAT_PSEUDO_LIQUID 1
+#define XCAPE_ADIABAT_REVERSIBLE_LIQUID 2
+#define XCAPE_ADIABAT_PSEUDO_ICE 3
+#define XCAPE_ADIABAT_REVERSIBLE_ICE 4
+
+/* Counters accumulated over one call. */
+typedef struct xcape_stats {
+    int64_t columns;       /* columns that ran the ascent (ts > 0 degC)        */
+    int64_t levels;        /* ascent levels                                     */
+    int64_t substeps;      /* pinc sub-steps                                    */
+    int64_t iterations;    /* moist-adiabat fixed-point iterations              */
+    int64_t nonconverged;  /* columns that hit the 100-iteration limit          */
+    int64_t fallbacks;     /* warm starts rejected for the cold start           */
+    int64_t reserved[2];
+} xcape_stats;
+
+const char *xcape_version(void);
+const char *xcape_last_error(void);
+
+/* Number of visible devices (0 when none); never fails. */
+int xcape_device_count(void);
+
+/*
+ * Column CAPE/CIN, the most-unstable level and the height of the last ascent
+ * level.  Replaces the f2py routines loopcape_ml
+ * (reference src/xcape/fortran/CAPE_CODE_model_lev.pyf:6-24 /
+ *  CAPE_CODE_model_lev.f90:4-91) and loopcape_pl1d
+ * (CAPE_CODE_pressure_lev.pyf:26-45 / CAPE_CODE_pressure_lev.f90:88-169),
+ * as called by cape_fortran.cape (src/xcape/cape_fortran.py:49-67).
+ *
+ *   p_shared == 0: p is (ncol, nk) model-level pressure (loopcape_ml);
+ *   p_shared == 1: p is (nk,) pressure levels shared by all columns
+ *                  (loopcape_pl1d); start_lev (ncol,) gives each column's
+ *                  1-based first level (the reference's pres_lev_pos); when
+ *                  NULL it is computed on the device with the same rule
+ *                  (core.py:286-289).
+ * t, td: (ncol, nk); ps, ts, tds: (ncol,).
+ * Outputs (ncol,): cape, cin [J/kg], mulev (1-based, surface = 1, -999999 if
+ * not most-unstable), zmulev [m].  Columns with ts <= 0 degC give zeros.
+ */
+int xcape_cape_f32(const float *p, int p_shared, const int32_t *start_lev,
+                   const float *t, const float *td,
+                   const float *ps, const float *ts, const float *tds,
+                   int64_t nk, int64_t ncol, float pinc, int source, float ml_depth,
+                   int adiabat, float *cape, float *cin, int32_t *mulev, float *zmulev,
+                   uint32_t flags, xcape_stats *stats, void *stream);
+
+/* Same computation evaluated in FP64 on the device from the same FP32 inputs
+ * (the accuracy yardstick for the FP32 kernel).  No reference counterpart:
+ * the reference computes CAPE in FP32 only. */
+int xcape_cape_f64(const float *p, int p_shared, const int32_t *start_lev,
+                   const float *t, const float *td,
+                   const float *ps, const float *ts, const float *tds,
+                   int64_t nk, int64_t ncol, double pinc, int source, double ml_depth,
+                   int adiabat, double *cape, double *cin, int32_t *mulev, double *zmulev,
+                   uint32_t flags, xcape_stats *stats, void *stream);
+
+/*
+ * Storm-relative helicity 0..depth for the Bunkers right and left movers,
+ * plus the storm motions and the 0-6 km mean wind.  Replaces the chain
+ * calc_srh runs (src/xcape/core.py:516-541): loop_stdheight_ml / _pl1d
+ * (stdheight_2D_model_lev.pyf, stdheight_2D_pressure_lev.pyf) with a 2 m
+ * surface height, bunkers_loop_ml / _pl (Bunkers_model_lev.pyf:6-21,
+ * Bunkers_pressure_lev.pyf) and loop_sreh_ml / _pl (SREH_model_lev.pyf,
+ * SREH_pressure_lev.pyf), in the reference's precisions (FP64 heights and
+ * helicity, FP32 storm motion).
+ *
+ * Arrays as for xcape_cape_f32, plus u, v (ncol, nk) and us, vs (ncol,) in
+ * m/s.  Outputs: srh_rm, srh_lm (ncol,) FP64; rm, lm, mean6 (ncol, 2) as
+ * (u, v) pairs (the reference's (2, n2) Fortran arrays), all three NULL to
+ * skip them.
+ */
+int xcape_srh_f32(const float *p, int p_shared, const int32_t *start_lev,
+                  const float *t, const float *td, const float *u, const float *v,
+                  const float *ps, const float *ts, const float *tds,
+                  const float *us, const float *vs, int64_t nk, int64_t ncol, double depth,
+                  double *srh_rm, double *srh_lm, float *rm, float *lm, float *mean6,
+                  uint32_t flags, void *stream);
+
+/*
+ * CAPE/CIN (as xcape_cape_f32) and the helicity chain (as xcape_srh_f32) of
+ * the same columns in one pass over the profiles: one kernel, each lane runs
+ * the ascent and then the helicity of its column.  No single reference
+ * counterpart: it fuses the two calls above for callers that need both.
+ */
+int xcape_cape_srh_f32(const float *p, int p_shared, const int32_t *start_lev,
+                       const float *t, const float *td, const float *u, const float *v,
+                       const float *ps, const float *ts, const float *tds,
+                       const float *us, const float *vs, int64_t nk, int64_t ncol,
+                       float pinc, int source, float ml_depth, int adiabat, double depth,
+                       float *cape, float *cin, int32_t *mulev, float *zmulev,
+                       double *srh_rm, double *srh_lm, float *rm, float *lm, float *mean6,
+                       uint32_t flags, xcape_stats *stats, void *stream);
+
+/*
+ * Ragged batches: columns with their own level count, parcel source and
+ * adiabat in one persistent launch, replayed from a captured hipGraph.
+ * No single reference counterpart: the reference handles one (nk, source,
+ * adiabat) per loopcape_ml call (CAPE_CODE_model_lev.f90:4-91), so a mixed
+ * batch is a group of such calls; per column the results are those of
+ * xcape_cape_f32 on that column.
+ *
+ * Layout (CSR): column i's levels are offsets[i] .. offsets[i+1]-1 of the
+ * concatenated p, t, td; ps, ts, tds, source, adiabat are (ncol,).  The *_host
+ * arrays are host copies of offsets/source/adiabat used to validate the batch
+ * and build the work order (sorted by adiabat, source and descending level
+ * count); every other array is a device pointer the plan keeps using.
+ *
+ * chunk: columns a wavefront takes from the work queue per grab (64, 128 or
+ * 256); tile: levels per column staged in LDS (0..64); waves_per_block 1..4.
+ * flags: XCAPE_FAST_MATH or 0.
+ */
+typedef struct xcape_ragged_plan xcape_ragged_plan;
+
+int xcape_ragged_plan_create(const int64_t *offsets_host, const int32_t *source_host,
+                             const int32_t *adiabat_host, int64_t ncol,
+                             const int64_t *offsets, const int32_t *source,
+                             const int32_t *adiabat, const float *p, const float *t,
+                             const float *td, const float *ps, const float *ts,
+                             const float *tds, float pinc, float ml_depth, float *cape,
+                             float *cin, int32_t *mulev, float *zmulev, int chunk, int tile,
+                             int waves_per_block, uint32_t flags, xcape_ragged_plan **out);
+
+/* Run the plan on `stream`: the first call captures the launch into a
+ * hipGraph, later calls replay it.  With stats != NULL the run is a direct
+ * launch with counters and returns synchronously. */
+int xcape_ragged_plan_run(xcape_ragged_plan *plan, xcape_stats *stats, void *stream);
+
+/* Launch geometry the plan chose (grid blocks, threads per block). */
+int xcape_ragged_plan_info(const xcape_ragged_plan *plan, int *grid, int *block);
+
+int xcape_ragged_plan_destroy(xcape_ragged_plan *plan);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* XCAPE_HIP_H */

@@ -220,3 +220,20 @@ def sreh_pl(u, v, aglh, us, vs, aglhs, cu_rm, cv_rm, cu_lm, cv_lm, top, start_3d
         _p(cu_rm), _p(cv_rm), _p(cu_lm), _p(cv_lm), _rd(top), _p(st),
         _ri(nk), _ri(n2), _p(srm), _p(slm))
     return srm, slm
+
+
+def bunkers_ml_timing(u, v, aglh, us, vs, aglhs):
+    """``bunkers_loop_ml`` from the default build in one call, as f2py would
+    run it: same work, but the accumulators the reference reads before
+    setting are whatever the stack holds, so the values are not meaningful.
+    Used only to time the reference's CPU path."""
+    u, v, aglh = _f32(u), _f32(v), _f32(aglh)
+    us, vs, aglhs = _f32(us), _f32(vs), _f32(aglhs)
+    nk, n2 = u.shape
+    rm = np.zeros((2, n2), np.float32, order='F')
+    lm = np.zeros((2, n2), np.float32, order='F')
+    mw = np.zeros((2, n2), np.float32, order='F')
+    _lib('Bunkers_model_lev').bunkers_loop_ml_(
+        _p(u), _p(v), _p(aglh), _p(us), _p(vs), _p(aglhs),
+        _ri(nk), _ri(n2), _p(rm), _p(lm), _p(mw))
+    return rm, lm, mw

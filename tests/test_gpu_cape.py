@@ -1,0 +1,176 @@
+"""GPU parity of the CAPE/CIN kernels against the oracle and the reference's
+goldens.  Calls go through the C-ABI (xcape_amd._hip -> libxcape_hip.so)."""
+import numpy as np
+import pytest
+
+from tests.tolerances import cape_ok, cape_tol
+from xcape_amd import synthetic
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope='module')
+def hip():
+    from xcape_amd import _hip
+    assert _hip.device_count() >= 1, 'no HIP device visible'
+    return _hip
+
+
+def _report(name, got, ref):
+    got = np.asarray(got, np.float64)
+    ref = np.asarray(ref, np.float64)
+    d = np.abs(got - ref)
+    bad = ~cape_ok(got, ref)
+    print(f'{name}: max|d|={d.max():.4g} exact={np.mean(got == ref):.3f} '
+          f'outside_tol={bad.sum()}/{bad.size}', flush=True)
+    return bad
+
+
+def _mu_tie(o, p, t, td, ps, ts, tds, lv_a, lv_b):
+    """True when the theta-e of two candidate MU levels agree to 4 ulp."""
+    pa = np.concatenate([[ps], p])
+    ta = np.concatenate([[ts], t])
+    tda = np.concatenate([[tds], td])
+
+    def the(k):
+        pk, tk, tdk = pa[k - 1] * 100.0, ta[k - 1] + 273.15, tda[k - 1] + 273.15
+        pk, tk, tdk = np.float32(pk), np.float32(tk), np.float32(tdk)
+        q = o.lib().xo_getqvs_f32
+        import ctypes
+        q.restype = ctypes.c_float
+        q.argtypes = [ctypes.c_float, ctypes.c_float]
+        qq = q(pk, tdk)
+        f = o.lib().xo_getthe_f32
+        f.restype = ctypes.c_float
+        f.argtypes = [ctypes.c_float] * 4
+        return f(pk, tk, tdk, qq)
+    a, b = the(lv_a), the(lv_b)
+    return abs(a - b) <= 4 * np.spacing(np.float32(max(a, b)))
+
+
+def test_soundings_goldens(hip, soundings, oracle_lib):
+    """Reference test_calc_cape_sigma (test/test_core.py:68-116): 13 soundings,
+    SB pinc 100, ML 500 m pinc 1000, MU pinc 100; |d| < 1.5 like the
+    reference's decimal=0, plus the 0.5 J/kg / 0.1 % bar against the oracle."""
+    s = soundings
+    P, T, TD = s['pressure'], s['temperature'], s['dewpoint']
+    args = (P[:, 1:], T[:, 1:], TD[:, 1:], P[:, 0], T[:, 0], TD[:, 0])
+    cases = [(1, 100., 'SB_CAPE_pinc100', 'SB_CIN_pinc100'),
+             (3, 1000., 'ML_CAPE_pinc1000_mldepth500', 'ML_CIN_pinc1000_mldepth500'),
+             (2, 100., 'MU_CAPE_pinc100', 'MU_CIN_pinc100')]
+    for src, pinc, kc, ki in cases:
+        c, ci, mu, z = hip.cape(*args, pinc=pinc, source=src, ml_depth=500., adiabat=1)
+        oc, oci, omu, oz = oracle_lib.cape(*args, pinc=pinc, source=src, ml_depth=500.,
+                                           adiabat=1)
+        assert np.all(np.abs(c - s[kc]) < 1.5), (kc, c, s[kc])
+        assert np.all(np.abs(ci - s[ki]) < 1.5), (ki, ci, s[ki])
+        assert not _report(f'snd src{src} cape', c, oc).any()
+        assert not _report(f'snd src{src} cin', ci, oci).any()
+        if src == 2:
+            assert np.array_equal(mu, s['MU_lv_pinc100'].astype(np.int32))
+            assert np.all(np.abs(z - s['MU_z_pinc100']) < 1.5)
+            np.testing.assert_array_equal(mu, omu)
+
+
+def test_era5_goldens(hip, era5, oracle_lib):
+    """Reference test_calc_cape_pressure (test/test_core.py:120-170):
+    17 ERA5 columns, SB / ML 300 m / MU at pinc 500."""
+    e = era5
+    lev = e['level']
+    st = oracle_lib.start_levels(lev, e['sfc_p'])
+    for src, key in ((1, 'sp500'), (3, 'ml300p500'), (2, 'mup500')):
+        for start in (st, None):     # host-computed and device-computed start levels
+            c, ci, mu, z = hip.cape(lev, e['t'], e['td'], e['sfc_p'], e['sfc_t'], e['sfc_td'],
+                                    pinc=500., source=src, ml_depth=300., adiabat=1,
+                                    p_shared=True, start_lev=start)
+            assert np.all(np.abs(c - e['sfc_cape' + key]) < 1.5)
+            assert np.all(np.abs(ci - e['sfc_cin' + key]) < 1.5)
+            oc, oci, _, _ = oracle_lib.cape(None, e['t'], e['td'], e['sfc_p'], e['sfc_t'],
+                                            e['sfc_td'], 500., src, 300., 1, p1d=lev,
+                                            start_lev=st)
+            assert not _report(f'era5 src{src} cape', c, oc).any()
+            assert not _report(f'era5 src{src} cin', ci, oci).any()
+
+
+@pytest.mark.parametrize('adiabat', [1, 2, 3, 4])
+@pytest.mark.parametrize('source', [1, 2, 3])
+def test_sigma_synthetic(hip, oracle_lib, source, adiabat):
+    """64x64 sigma columns (40 levels) from the seeded generator, every
+    source and adiabat, against the FP32 oracle."""
+    c = synthetic.sigma_columns(4096, 40, 1000)
+    args = (c['p'], c['t'], c['td'], c['ps'], c['ts'], c['tds'])
+    g = hip.cape(*args, pinc=500., source=source, ml_depth=500., adiabat=adiabat, stats=True)
+    o = oracle_lib.cape(*args, pinc=500., source=source, ml_depth=500., adiabat=adiabat,
+                        stats=True)
+    print('gpu stats', g[4], 'oracle stats', o[4])
+    bad_c = _report(f'sigma s{source} a{adiabat} cape', g[0], o[0])
+    bad_i = _report(f'sigma s{source} a{adiabat} cin', g[1], o[1])
+    assert not bad_c.any() and not bad_i.any()
+    mism = np.nonzero(g[2] != o[2])[0]
+    for i in mism:
+        assert _mu_tie(oracle_lib, c['p'][i], c['t'][i], c['td'][i], c['ps'][i], c['ts'][i],
+                       c['tds'][i], g[2][i], o[2][i]), (i, g[2][i], o[2][i])
+    assert g[4]['columns'] == o[4]['columns']
+
+
+@pytest.mark.parametrize('adiabat', [1, 3])
+@pytest.mark.parametrize('source', [1, 2, 3])
+def test_pressure_synthetic(hip, oracle_lib, source, adiabat):
+    """37 fixed pressure levels with NaN below ground, device start levels."""
+    c = synthetic.pressure_columns(4096, 1001)
+    st = oracle_lib.start_levels(c['p'], c['ps'])
+    g = hip.cape(c['p'], c['t'], c['td'], c['ps'], c['ts'], c['tds'], pinc=500.,
+                 source=source, ml_depth=500., adiabat=adiabat, p_shared=True)
+    o = oracle_lib.cape(None, c['t'], c['td'], c['ps'], c['ts'], c['tds'], 500., source, 500.,
+                        adiabat, p1d=c['p'], start_lev=st)
+    assert np.all(np.isfinite(g[0])) and np.all(np.isfinite(g[1]))
+    assert not _report(f'pres s{source} a{adiabat} cape', g[0], o[0]).any()
+    assert not _report(f'pres s{source} a{adiabat} cin', g[1], o[1]).any()
+
+
+def test_fp64_matches_fp64_oracle(hip, oracle_lib):
+    c = synthetic.sigma_columns(2048, 40, 1000)
+    args = (c['p'], c['t'], c['td'], c['ps'], c['ts'], c['tds'])
+    for src in (1, 2, 3):
+        g = hip.cape(*args, pinc=500., source=src, adiabat=3, fp64=True)
+        o = oracle_lib.cape(*[a.astype(np.float64) for a in args], pinc=500., source=src,
+                            adiabat=3, fp64=True)
+        d = np.abs(g[0] - o[0])
+        print('fp64 max|d|', d.max(), flush=True)
+        assert np.all(d <= 1e-6 * np.maximum(1.0, np.abs(o[0])))
+        assert np.all(np.abs(g[1] - o[1]) <= 1e-6 * np.maximum(1.0, np.abs(o[1])))
+
+
+def test_fast_math_within_tolerance(hip, oracle_lib):
+    c = synthetic.sigma_columns(4096, 40, 1000)
+    args = (c['p'], c['t'], c['td'], c['ps'], c['ts'], c['tds'])
+    for src, adi in ((1, 1), (2, 3), (3, 1)):
+        g = hip.cape(*args, pinc=500., source=src, adiabat=adi, fast_math=True)
+        o = oracle_lib.cape(*args, pinc=500., source=src, adiabat=adi)
+        bad = _report(f'fast s{src} a{adi} cape', g[0], o[0])
+        bad |= _report(f'fast s{src} a{adi} cin', g[1], o[1])
+        assert not bad.any()
+
+
+def test_cold_surface_gate(hip):
+    """ts <= 0 degC columns return zeros (CAPE_CODE_model_lev.f90:77,83-88)."""
+    c = synthetic.sigma_columns(256, 40, 7)
+    ts = c['ts'].copy()
+    ts[::3] = -1.0
+    ts[1] = 0.0
+    g = hip.cape(c['p'], c['t'], c['td'], c['ps'], ts, c['tds'], source=2)
+    gated = ~(ts > 0)
+    assert np.all(g[0][gated] == 0) and np.all(g[1][gated] == 0)
+    assert np.all(g[2][gated] == 0) and np.all(g[3][gated] == 0)
+
+
+def test_device_pointer_path(hip):
+    import torch
+    c = synthetic.sigma_columns(1024, 40, 11)
+    args = (c['p'], c['t'], c['td'], c['ps'], c['ts'], c['tds'])
+    h = hip.cape(*args, source=1)
+    dargs = [torch.from_numpy(a).cuda() for a in args]
+    d = hip.cape(*dargs, source=1)
+    torch.cuda.synchronize()
+    for x, y in zip(h, d):
+        np.testing.assert_array_equal(x, y.cpu().numpy())

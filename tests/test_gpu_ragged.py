@@ -1,0 +1,87 @@
+"""Ragged persistent kernel: parity with the oracle per column, and hipGraph
+replays that keep matching the direct launch."""
+import numpy as np
+import pytest
+
+from tests.tolerances import cape_ok
+from xcape_amd import synthetic
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope='module')
+def batch_np():
+    return synthetic.ragged_columns(12000, 2002)
+
+
+def _oracle_ragged(o, d):
+    """The reference's way to do a mixed batch: one loop call per group."""
+    n = len(d['nlev'])
+    out = [np.zeros(n, np.float32), np.zeros(n, np.float32), np.zeros(n, np.int32),
+           np.zeros(n, np.float32)]
+    keys = np.stack([d['nlev'], d['source'], d['adiabat']], 1)
+    for key in np.unique(keys, axis=0):
+        idx = np.nonzero((keys == key).all(1))[0]
+        nk = int(key[0])
+        rows = d['offsets'][idx][:, None] + np.arange(nk)[None, :]
+        r = o.cape(d['p'][rows], d['t'][rows], d['td'][rows], d['ps'][idx], d['ts'][idx],
+                   d['tds'][idx], pinc=500., source=int(key[1]), ml_depth=500.,
+                   adiabat=int(key[2]))
+        for j in range(4):
+            out[j][idx] = r[j]
+    return out
+
+
+@pytest.mark.parametrize('cfg', [dict(chunk=64, tile=16, waves=4),
+                                 dict(chunk=128, tile=0, waves=2),
+                                 dict(chunk=256, tile=32, waves=1)])
+def test_ragged_matches_oracle(batch_np, oracle_lib, cfg):
+    import torch
+
+    from xcape_amd.ragged import RaggedBatch, RaggedPlan
+    b = RaggedBatch.from_dict(batch_np)
+    plan = RaggedPlan(b, pinc=500., ml_depth=500., **cfg)
+    st = plan.run(stats=True)
+    torch.cuda.synchronize()
+    got = [x.cpu().numpy() for x in (b.cape, b.cin, b.mulev, b.zmulev)]
+    ref = _oracle_ragged(oracle_lib, batch_np)
+    print(cfg, plan.info(), st, flush=True)
+    assert st['columns'] == int((batch_np['ts'] > 0).sum())
+    assert np.all(cape_ok(got[0], ref[0])), np.abs(got[0] - ref[0]).max()
+    assert np.all(cape_ok(got[1], ref[1])), np.abs(got[1] - ref[1]).max()
+    mu = batch_np['source'] == 2
+    assert np.mean(got[2][mu] == ref[2][mu]) > 0.999
+    plan.close()
+
+
+def test_graph_replays_match_direct_launch(batch_np):
+    import torch
+
+    from xcape_amd.ragged import RaggedBatch, RaggedPlan
+    b = RaggedBatch.from_dict(batch_np)
+    plan = RaggedPlan(b, chunk=64, tile=16, waves=4)
+    plan.run(stats=True)              # direct launch
+    torch.cuda.synchronize()
+    ref = [x.clone() for x in (b.cape, b.cin, b.mulev, b.zmulev)]
+    for rep in range(4):
+        for x in (b.cape, b.cin, b.zmulev):
+            x.fill_(-1.0)             # eager work between replays
+        b.mulev.fill_(-7)
+        torch.cuda.synchronize()
+        plan.run()                    # graph capture (first time) / replay
+        torch.cuda.synchronize()
+        for x, y in zip((b.cape, b.cin, b.mulev, b.zmulev), ref):
+            assert torch.equal(x, y), rep
+    plan.close()
+
+
+def test_autotuner_picks_a_timed_config(batch_np, tmp_path, monkeypatch):
+    from xcape_amd import ragged
+    monkeypatch.setattr(ragged, 'CACHE', str(tmp_path / 'tune.json'))
+    b = ragged.RaggedBatch.from_dict(batch_np)
+    best, times = ragged.autotune(b, chunks=(64, 128), tiles=(0, 16), waves=(2, 4), reps=1)
+    assert len(times) == 8 and best is not None
+    assert min(times.values()) == times[f"chunk={best['chunk']},tile={best['tile']},"
+                                        f"waves={best['waves']}"]
+    best2, _ = ragged.autotune(b, chunks=(64, 128), tiles=(0, 16), waves=(2, 4), reps=1)
+    assert best2 == best              # served from the cache

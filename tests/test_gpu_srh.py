@@ -1,0 +1,107 @@
+"""GPU parity of the helicity chain (heights, Bunkers motion, SREH) and of the
+fused CAPE+SRH kernel.  Bar: Bunkers motion within 0.05 m/s, SRH within
+1 m^2/s^2 of the oracle; the reference's goldens within their own 1.5e-5."""
+import numpy as np
+import pytest
+
+from tests.tolerances import BUNKERS_ABS, SRH_ABS, cape_ok
+from xcape_amd import synthetic
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope='module')
+def hip():
+    from xcape_amd import _hip
+    assert _hip.device_count() >= 1, 'no HIP device visible'
+    return _hip
+
+
+def _cmp_srh(g, o, name):
+    srm, slm, rm, lm, mw = g
+    osrm, oslm, orm, olm, omw = o
+    d_srh = max(np.abs(srm - osrm).max(), np.abs(slm - oslm).max())
+    d_bk = max(np.abs(rm - orm).max(), np.abs(lm - olm).max(), np.abs(mw - omw).max())
+    print(f'{name}: max|dSRH|={d_srh:.3g} max|dBunkers|={d_bk:.3g} '
+          f'exactSRH={np.mean(srm == osrm):.3f} exactBk={np.mean(rm == orm):.3f}', flush=True)
+    assert d_srh <= SRH_ABS and d_bk <= BUNKERS_ABS
+
+
+def test_soundings_srh_goldens(hip, soundings, oracle_lib):
+    """Reference test_calc_srh_sigma (test/test_core.py:173-229)."""
+    s = soundings
+    P, T, TD, U, V = (s[k] for k in ('pressure', 'temperature', 'dewpoint', 'u_wind_ms',
+                                      'v_wind_ms'))
+    args = (P[:, 1:], T[:, 1:], TD[:, 1:], U[:, 1:], V[:, 1:], P[:, 0], T[:, 0], TD[:, 0],
+            U[:, 0], V[:, 0])
+    g = hip.srh(*args, depth=3000.)
+    np.testing.assert_almost_equal(g[0], s['SRH03_model_lev_rm'], 5)
+    np.testing.assert_almost_equal(g[1], s['SRH03_model_lev_lm'], 5)
+    _cmp_srh(g, oracle_lib.srh(*args, 3000.), 'soundings')
+
+
+def test_era5_srh_goldens(hip, era5, oracle_lib):
+    """Reference test_calc_srh_pressure (test/test_core.py:231-286)."""
+    e = era5
+    st = oracle_lib.start_levels(e['level'], e['sfc_p'])
+    args = (e['level'], e['t'], e['td'], e['u'], e['v'], e['sfc_p'], e['sfc_t'], e['sfc_td'],
+            e['sfc_u'], e['sfc_v'])
+    for start in (st, None):
+        g = hip.srh(*args, depth=3000., p_shared=True, start_lev=start)
+        np.testing.assert_almost_equal(g[0], e['sfc_srh_rm'], 5)
+        np.testing.assert_almost_equal(g[1], e['sfc_srh_lm'], 5)
+        o = oracle_lib.srh(None, *args[1:], 3000., p1d=e['level'], start_lev=st)
+        _cmp_srh(g, o, 'era5')
+
+
+@pytest.mark.parametrize('depth', [1000., 3000.])
+def test_sigma_synthetic_srh(hip, oracle_lib, depth):
+    c = synthetic.sigma_columns(2048, 137, 1004, winds=True)
+    args = [c[k] for k in ('p', 't', 'td', 'u', 'v', 'ps', 'ts', 'tds', 'us', 'vs')]
+    g = hip.srh(*args, depth=depth)
+    o = oracle_lib.srh(*args, depth)
+    _cmp_srh(g, o, f'sigma depth {depth}')
+
+
+def test_pressure_synthetic_srh(hip, oracle_lib):
+    c = synthetic.pressure_columns(2048, 1002, winds=True)
+    st = oracle_lib.start_levels(c['p'], c['ps'])
+    args = [c[k] for k in ('p', 't', 'td', 'u', 'v', 'ps', 'ts', 'tds', 'us', 'vs')]
+    g = hip.srh(*args, depth=3000., p_shared=True)
+    o = oracle_lib.srh(None, *args[1:], 3000., p1d=c['p'], start_lev=st)
+    assert np.all(np.isfinite(g[0]))
+    _cmp_srh(g, o, 'pressure')
+
+
+def test_shallow_column_srh_is_zero(hip, oracle_lib):
+    """No level above `depth`: the reference's sum is empty."""
+    c = synthetic.sigma_columns(64, 20, 9, winds=True)
+    args = [c[k] for k in ('p', 't', 'td', 'u', 'v', 'ps', 'ts', 'tds', 'us', 'vs')]
+    g = hip.srh(*args, depth=1.0e6)
+    assert np.all(g[0] == 0) and np.all(g[1] == 0)
+    _cmp_srh(g, oracle_lib.srh(*args, 1.0e6), 'shallow')
+
+
+@pytest.mark.parametrize('fast', [False, True])
+def test_fused_cape_srh_matches_separate_calls(hip, fast):
+    c = synthetic.sigma_columns(2048, 137, 1004, winds=True)
+    a = [c[k] for k in ('p', 't', 'td', 'u', 'v', 'ps', 'ts', 'tds', 'us', 'vs')]
+    f = hip.cape_srh(*a, pinc=500., source=2, adiabat=3, depth=3000., fast_math=fast)
+    capes = hip.cape(a[0], a[1], a[2], a[5], a[6], a[7], pinc=500., source=2, adiabat=3,
+                     fast_math=fast)
+    srh = hip.srh(*a, depth=3000.)
+    for x, y in zip(f[:4], capes):
+        np.testing.assert_array_equal(x, y)
+    for x, y in zip(f[4:], srh):
+        np.testing.assert_array_equal(x, y)
+
+
+def test_fused_c4_shape_against_oracle(hip, oracle_lib):
+    """The combined workload's recipe (137 sigma levels, most unstable,
+    pseudo-ice, SRH 0-3 km) on a slice of columns, against the oracle."""
+    c = synthetic.sigma_columns(1024, 137, 1004, winds=True)
+    a = [c[k] for k in ('p', 't', 'td', 'u', 'v', 'ps', 'ts', 'tds', 'us', 'vs')]
+    f = hip.cape_srh(*a, pinc=500., source=2, adiabat=3, depth=3000.)
+    o = oracle_lib.cape(a[0], a[1], a[2], a[5], a[6], a[7], pinc=500., source=2, adiabat=3)
+    assert np.all(cape_ok(f[0], o[0])) and np.all(cape_ok(f[1], o[1]))
+    _cmp_srh(f[4:], oracle_lib.srh(*a, 3000.), 'fused C4')

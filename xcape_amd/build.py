@@ -1,0 +1,69 @@
+"""Build libxcape_hip.so in-tree for gfx950 with hipcc.
+
+Each ``.hip`` translation unit is compiled to an object with its own
+floating-point contraction setting (the accurate kernels must not contract
+a*b+c into FMA so that they round as the reference does), then all objects are
+linked into ``xcape_amd/libxcape_hip.so``.
+
+Every HIP compile carries ``-Xclang -target-feature -Xclang -packed-fp32-ops``
+so the compiler emits no packed FP32 instructions.
+
+Usage: ``python -m xcape_amd.build`` (or ``build()``).
+"""
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, 'csrc')
+LIB = os.path.join(HERE, 'libxcape_hip.so')
+HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
+ARCH = 'gfx950'
+
+COMMON = ['--offload-arch=' + ARCH, '-O3', '-std=c++17', '-fPIC',
+          '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
+          '-Wall', '-Wno-unused-function', '-Wno-unknown-warning-option']
+
+# translation unit -> extra flags
+UNITS = {
+    'capi.hip': ['-ffp-contract=off'],
+    'cape_accurate.hip': ['-ffp-contract=off'],
+    'cape_fast.hip': ['-ffp-contract=fast'],
+    'ragged.hip': ['-ffp-contract=off'],
+}
+
+
+def _compile(unit, flags, outdir):
+    obj = os.path.join(outdir, unit.replace('.hip', '.o'))
+    cmd = [HIPCC] + COMMON + flags + ['-c', os.path.join(CSRC, unit), '-o', obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    noise = 'packed-fp32-ops'
+    err = '\n'.join(l for l in r.stderr.splitlines() if noise not in l)
+    if r.returncode != 0:
+        raise RuntimeError(f'hipcc failed for {unit}:\n{err}')
+    if err.strip():
+        print(err, file=sys.stderr)
+    return obj
+
+
+def build(verbose=True):
+    """Compile every unit and link the shared library (always rebuilds)."""
+    with tempfile.TemporaryDirectory(prefix='xcape_build_') as tmp:
+        with ThreadPoolExecutor(max_workers=min(8, len(UNITS))) as ex:
+            objs = list(ex.map(lambda kv: _compile(kv[0], kv[1], tmp), UNITS.items()))
+        out_tmp = LIB + '.tmp'
+        cmd = [HIPCC, '--offload-arch=' + ARCH, '-shared', '-fPIC', '-o', out_tmp] + objs
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError('link failed:\n' + r.stderr)
+        shutil.move(out_tmp, LIB)
+    if verbose:
+        print('built', LIB)
+    return LIB
+
+
+if __name__ == '__main__':
+    build()

@@ -1,0 +1,40 @@
+// Host-side launch descriptors shared by the C-ABI and the kernel translation
+// units.  All pointers are device pointers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace xcape {
+
+// Device counters one launch accumulates (one atomic per block per counter).
+enum { ST_COLUMNS = 0, ST_LEVELS, ST_SUBSTEPS, ST_ITERATIONS, ST_NONCONVERGED,
+       ST_FALLBACKS, ST_COUNT = 8 };
+
+struct CapeGridArgs {
+    const float *p;        // (ncol, nk_in) sigma grid, or nullptr
+    const float *p1d;      // (nk_in,) shared pressure levels, or nullptr
+    const int32_t *start;  // (ncol,) 1-based first level (pressure grid), or nullptr
+    const float *t, *td;   // (ncol, nk_in)
+    const float *ps, *ts, *tds;  // (ncol,)
+    int64_t nk_in, ncol;
+    float pinc, ml_depth;
+    int source, adiabat;
+    void *cape, *cin, *zout;     // float or double (ncol,)
+    int32_t *mulvl;              // (ncol,)
+    unsigned long long *stats;   // ST_COUNT counters or nullptr
+    // storm-relative helicity in the same pass (fused kernel) or alone
+    const float *u, *v;          // (ncol, nk_in)
+    const float *us, *vs;        // (ncol,)
+    double depth;
+    double *srh_rm, *srh_lm;     // (ncol,)
+    float *rm, *lm, *mean6;      // (ncol, 2) u, v pairs, or nullptr
+};
+
+// kind of work a grid launch does
+enum { GRID_CAPE = 1, GRID_SRH = 2, GRID_CAPE_SRH = 3 };
+
+hipError_t launch_cape_grid_accurate(const CapeGridArgs &a, bool fp64, int kind, int block,
+                                     hipStream_t s);
+hipError_t launch_cape_grid_fast(const CapeGridArgs &a, int kind, int block, hipStream_t s);
+
+}  // namespace xcape

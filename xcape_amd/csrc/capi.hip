@@ -1,0 +1,355 @@
+// C-ABI entry points of libxcape_hip.so (declared in include/xcape_hip.h).
+//
+// Host-pointer calls stage inputs through stream-ordered device allocations
+// on the calling thread's stream; device-pointer calls launch directly.  No
+// mutable global state besides the per-thread stream and error message.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+
+#include "../../include/xcape_hip.h"
+#include "capi_common.h"
+#include "cape_launch.h"
+
+using namespace xcape;
+
+namespace xcape {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what)
+{
+    if (e == hipErrorOutOfMemory)
+        return fail(XCAPE_ENOMEM, "%s: %s", what, hipGetErrorString(e));
+    return fail(XCAPE_EHIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+int ensure_device()
+{
+    static thread_local int checked = 0;
+    if (checked) return XCAPE_OK;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+        return fail(XCAPE_ENODEV, "no HIP device visible");
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return fail(XCAPE_ENODEV, "hipGetDevice failed");
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess)
+        return fail(XCAPE_ENODEV, "hipGetDeviceProperties failed");
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(XCAPE_ENODEV, "device %d is %s; this library is built for gfx950", dev,
+                    prop.gcnArchName);
+    checked = 1;
+    return XCAPE_OK;
+}
+
+hipStream_t thread_stream()
+{
+    static thread_local hipStream_t s = nullptr;
+    if (!s && hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) s = nullptr;
+    return s;
+}
+
+__global__ void zero_u64(unsigned long long *p, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+hipError_t zero_counters(unsigned long long *p, int n, hipStream_t s)
+{
+    zero_u64<<<1, 64, 0, s>>>(p, n);
+    return hipGetLastError();
+}
+
+void copy_stats(const unsigned long long *h, xcape_stats *stats)
+{
+    stats->columns = (int64_t)h[ST_COLUMNS];
+    stats->levels = (int64_t)h[ST_LEVELS];
+    stats->substeps = (int64_t)h[ST_SUBSTEPS];
+    stats->iterations = (int64_t)h[ST_ITERATIONS];
+    stats->nonconverged = (int64_t)h[ST_NONCONVERGED];
+    stats->fallbacks = (int64_t)h[ST_FALLBACKS];
+    stats->reserved[0] = stats->reserved[1] = 0;
+}
+
+}  // namespace xcape
+
+namespace {
+
+int check_common(int64_t nk, int64_t ncol)
+{
+    if (nk < 1 || nk > (1 << 20)) return fail(XCAPE_EINVAL, "nk=%lld out of range", (long long)nk);
+    if (ncol < 0) return fail(XCAPE_EINVAL, "ncol=%lld is negative", (long long)ncol);
+    return XCAPE_OK;
+}
+
+int check_cape_params(double pinc, int source, int adiabat)
+{
+    if (!(pinc >= 1.0) || !isfinite(pinc))
+        return fail(XCAPE_EINVAL, "pinc=%g Pa: must be finite and >= 1 Pa", pinc);
+    if (source < 1 || source > 3) return fail(XCAPE_EINVAL, "source=%d not in 1..3", source);
+    if (adiabat < 1 || adiabat > 4) return fail(XCAPE_EINVAL, "adiabat=%d not in 1..4", adiabat);
+    return XCAPE_OK;
+}
+
+// Everything a grid call needs; pointers as the caller passed them.
+struct GridCall {
+    int kind = GRID_CAPE;
+    bool fp64 = false;
+    const float *p = nullptr;
+    int p_shared = 0;
+    const int32_t *start = nullptr;
+    const float *t = nullptr, *td = nullptr, *ps = nullptr, *ts = nullptr, *tds = nullptr;
+    const float *u = nullptr, *v = nullptr, *us = nullptr, *vs = nullptr;
+    int64_t nk = 0, ncol = 0;
+    double pinc = 500.0, ml_depth = 500.0, depth = 3000.0;
+    int source = 1, adiabat = 1;
+    void *cape = nullptr, *cin = nullptr, *zout = nullptr;
+    int32_t *mulev = nullptr;
+    double *srh_rm = nullptr, *srh_lm = nullptr;
+    float *rm = nullptr, *lm = nullptr, *mean6 = nullptr;
+    uint32_t flags = 0;
+    xcape_stats *stats = nullptr;
+    void *stream = nullptr;
+};
+
+int grid_entry(const GridCall &c)
+{
+    int rc = check_common(c.nk, c.ncol);
+    if (rc) return rc;
+    const bool want_cape = c.kind & GRID_CAPE, want_srh = c.kind & GRID_SRH;
+    if (want_cape && (rc = check_cape_params(c.pinc, c.source, c.adiabat))) return rc;
+    if (want_srh && !isfinite(c.depth)) return fail(XCAPE_EINVAL, "depth=%g not finite", c.depth);
+    if (c.ncol > 0) {
+        if (!c.p || !c.t || !c.td || !c.ps || !c.ts || !c.tds)
+            return fail(XCAPE_EINVAL, "NULL input array");
+        if (want_cape && (!c.cape || !c.cin || !c.mulev || !c.zout))
+            return fail(XCAPE_EINVAL, "NULL CAPE output array");
+        if (want_srh && (!c.u || !c.v || !c.us || !c.vs || !c.srh_rm || !c.srh_lm))
+            return fail(XCAPE_EINVAL, "NULL helicity array");
+        if (want_srh && ((c.rm != nullptr) != (c.lm != nullptr) ||
+                         (c.rm != nullptr) != (c.mean6 != nullptr)))
+            return fail(XCAPE_EINVAL, "rm, lm and mean6 must be all NULL or all set");
+    }
+    if (c.fp64 && (c.flags & (XCAPE_FAST_MATH | XCAPE_WARM_START)))
+        return fail(XCAPE_EINVAL, "the FP64 entry point supports neither FAST_MATH nor WARM_START");
+    if (c.flags & XCAPE_WARM_START)
+        return fail(XCAPE_ESTATE, "warm start requested but no surrogate is loaded");
+    const bool dev = c.flags & XCAPE_DEVICE_PTRS;
+    if (!dev && c.p_shared && c.start) {
+        for (int64_t i = 0; i < c.ncol; ++i)
+            if (c.start[i] < 1 || c.start[i] > c.nk)
+                return fail(XCAPE_EINVAL, "start_lev[%lld]=%d outside 1..%lld", (long long)i,
+                            c.start[i], (long long)c.nk);
+    }
+    if ((rc = ensure_device())) return rc;
+    if (c.ncol == 0) return XCAPE_OK;
+    hipStream_t s = c.stream ? (hipStream_t)c.stream : thread_stream();
+    const size_t osz = c.fp64 ? sizeof(double) : sizeof(float);
+    const size_t n2 = (size_t)c.nk * (size_t)c.ncol;
+    const size_t np = c.p_shared ? (size_t)c.nk : n2;
+    const size_t ncol = (size_t)c.ncol;
+
+    DevBuf<float> dp_, dt_, dtd_, dps_, dts_, dtds_, du_, dv_, dus_, dvs_, drm_, dlm_, dmw_;
+    DevBuf<int32_t> dst_, dmul_;
+    DevBuf<unsigned char> dcape_, dcin_, dz_;
+    DevBuf<double> dsrm_, dslm_;
+    DevBuf<unsigned long long> dstats_;
+    CapeGridArgs a{};
+    a.nk_in = c.nk;
+    a.ncol = c.ncol;
+    a.pinc = (float)c.pinc;
+    a.ml_depth = (float)c.ml_depth;
+    a.source = c.source;
+    a.adiabat = c.adiabat;
+    a.depth = c.depth;
+    hipError_t e = hipSuccess;
+    if (dev) {
+        a.p = c.p_shared ? nullptr : c.p;
+        a.p1d = c.p_shared ? c.p : nullptr;
+        a.start = c.p_shared ? c.start : nullptr;
+        a.t = c.t; a.td = c.td; a.ps = c.ps; a.ts = c.ts; a.tds = c.tds;
+        a.cape = c.cape; a.cin = c.cin; a.zout = c.zout; a.mulvl = c.mulev;
+        a.u = c.u; a.v = c.v; a.us = c.us; a.vs = c.vs;
+        a.srh_rm = c.srh_rm; a.srh_lm = c.srh_lm; a.rm = c.rm; a.lm = c.lm; a.mean6 = c.mean6;
+    } else {
+#define XC_TRY(x) if ((e = (x)) != hipSuccess) return hip_fail(e, #x)
+        XC_TRY(dp_.alloc(np, s)); XC_TRY(dp_.upload(c.p, np, s));
+        XC_TRY(dt_.alloc(n2, s)); XC_TRY(dt_.upload(c.t, n2, s));
+        XC_TRY(dtd_.alloc(n2, s)); XC_TRY(dtd_.upload(c.td, n2, s));
+        XC_TRY(dps_.alloc(ncol, s)); XC_TRY(dps_.upload(c.ps, ncol, s));
+        XC_TRY(dts_.alloc(ncol, s)); XC_TRY(dts_.upload(c.ts, ncol, s));
+        XC_TRY(dtds_.alloc(ncol, s)); XC_TRY(dtds_.upload(c.tds, ncol, s));
+        if (c.p_shared && c.start) {
+            XC_TRY(dst_.alloc(ncol, s)); XC_TRY(dst_.upload(c.start, ncol, s));
+        }
+        a.p = c.p_shared ? nullptr : dp_.ptr;
+        a.p1d = c.p_shared ? dp_.ptr : nullptr;
+        a.start = (c.p_shared && c.start) ? dst_.ptr : nullptr;
+        a.t = dt_.ptr; a.td = dtd_.ptr; a.ps = dps_.ptr; a.ts = dts_.ptr; a.tds = dtds_.ptr;
+        if (want_cape) {
+            XC_TRY(dcape_.alloc(ncol * osz, s)); XC_TRY(dcin_.alloc(ncol * osz, s));
+            XC_TRY(dz_.alloc(ncol * osz, s)); XC_TRY(dmul_.alloc(ncol, s));
+            a.cape = dcape_.ptr; a.cin = dcin_.ptr; a.zout = dz_.ptr; a.mulvl = dmul_.ptr;
+        }
+        if (want_srh) {
+            XC_TRY(du_.alloc(n2, s)); XC_TRY(du_.upload(c.u, n2, s));
+            XC_TRY(dv_.alloc(n2, s)); XC_TRY(dv_.upload(c.v, n2, s));
+            XC_TRY(dus_.alloc(ncol, s)); XC_TRY(dus_.upload(c.us, ncol, s));
+            XC_TRY(dvs_.alloc(ncol, s)); XC_TRY(dvs_.upload(c.vs, ncol, s));
+            XC_TRY(dsrm_.alloc(ncol, s)); XC_TRY(dslm_.alloc(ncol, s));
+            a.u = du_.ptr; a.v = dv_.ptr; a.us = dus_.ptr; a.vs = dvs_.ptr;
+            a.srh_rm = dsrm_.ptr; a.srh_lm = dslm_.ptr;
+            if (c.rm) {
+                XC_TRY(drm_.alloc(2 * ncol, s)); XC_TRY(dlm_.alloc(2 * ncol, s));
+                XC_TRY(dmw_.alloc(2 * ncol, s));
+                a.rm = drm_.ptr; a.lm = dlm_.ptr; a.mean6 = dmw_.ptr;
+            }
+        }
+    }
+    if (c.stats && want_cape) {
+        XC_TRY(dstats_.alloc(ST_COUNT, s));
+        XC_TRY(zero_counters(dstats_.ptr, ST_COUNT, s));
+        a.stats = dstats_.ptr;
+    }
+    if (c.fp64)
+        e = launch_cape_grid_accurate(a, true, c.kind, 256, s);
+    else if (c.flags & XCAPE_FAST_MATH)
+        e = launch_cape_grid_fast(a, c.kind, 256, s);
+    else
+        e = launch_cape_grid_accurate(a, false, c.kind, 256, s);
+    if (e != hipSuccess) return hip_fail(e, "kernel launch");
+    if (!dev) {
+        if (want_cape) {
+            XC_TRY(hipMemcpyAsync(c.cape, dcape_.ptr, ncol * osz, hipMemcpyDeviceToHost, s));
+            XC_TRY(hipMemcpyAsync(c.cin, dcin_.ptr, ncol * osz, hipMemcpyDeviceToHost, s));
+            XC_TRY(hipMemcpyAsync(c.zout, dz_.ptr, ncol * osz, hipMemcpyDeviceToHost, s));
+            XC_TRY(dmul_.download(c.mulev, ncol, s));
+        }
+        if (want_srh) {
+            XC_TRY(dsrm_.download(c.srh_rm, ncol, s));
+            XC_TRY(dslm_.download(c.srh_lm, ncol, s));
+            if (c.rm) {
+                XC_TRY(drm_.download(c.rm, 2 * ncol, s));
+                XC_TRY(dlm_.download(c.lm, 2 * ncol, s));
+                XC_TRY(dmw_.download(c.mean6, 2 * ncol, s));
+            }
+        }
+    }
+    if (a.stats) {
+        unsigned long long h[ST_COUNT];
+        XC_TRY(hipMemcpyAsync(h, dstats_.ptr, sizeof h, hipMemcpyDeviceToHost, s));
+        XC_TRY(hipStreamSynchronize(s));
+        copy_stats(h, c.stats);
+    } else if (!dev) {
+        XC_TRY(hipStreamSynchronize(s));
+    }
+#undef XC_TRY
+    return XCAPE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *xcape_version(void) { return "xcape-hip 0.1.0 (gfx950)"; }
+
+const char *xcape_last_error(void) { return xcape::g_err.c_str(); }
+
+int xcape_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int xcape_cape_f32(const float *p, int p_shared, const int32_t *start_lev, const float *t,
+                   const float *td, const float *ps, const float *ts, const float *tds,
+                   int64_t nk, int64_t ncol, float pinc, int source, float ml_depth,
+                   int adiabat, float *cape, float *cin, int32_t *mulev, float *zmulev,
+                   uint32_t flags, xcape_stats *stats, void *stream)
+{
+    GridCall c;
+    c.kind = GRID_CAPE;
+    c.p = p; c.p_shared = p_shared; c.start = start_lev;
+    c.t = t; c.td = td; c.ps = ps; c.ts = ts; c.tds = tds;
+    c.nk = nk; c.ncol = ncol; c.pinc = pinc; c.source = source; c.ml_depth = ml_depth;
+    c.adiabat = adiabat;
+    c.cape = cape; c.cin = cin; c.mulev = mulev; c.zout = zmulev;
+    c.flags = flags; c.stats = stats; c.stream = stream;
+    return grid_entry(c);
+}
+
+int xcape_cape_f64(const float *p, int p_shared, const int32_t *start_lev, const float *t,
+                   const float *td, const float *ps, const float *ts, const float *tds,
+                   int64_t nk, int64_t ncol, double pinc, int source, double ml_depth,
+                   int adiabat, double *cape, double *cin, int32_t *mulev, double *zmulev,
+                   uint32_t flags, xcape_stats *stats, void *stream)
+{
+    GridCall c;
+    c.kind = GRID_CAPE;
+    c.fp64 = true;
+    c.p = p; c.p_shared = p_shared; c.start = start_lev;
+    c.t = t; c.td = td; c.ps = ps; c.ts = ts; c.tds = tds;
+    c.nk = nk; c.ncol = ncol; c.pinc = pinc; c.source = source; c.ml_depth = ml_depth;
+    c.adiabat = adiabat;
+    c.cape = cape; c.cin = cin; c.mulev = mulev; c.zout = zmulev;
+    c.flags = flags; c.stats = stats; c.stream = stream;
+    return grid_entry(c);
+}
+
+int xcape_srh_f32(const float *p, int p_shared, const int32_t *start_lev, const float *t,
+                  const float *td, const float *u, const float *v, const float *ps,
+                  const float *ts, const float *tds, const float *us, const float *vs,
+                  int64_t nk, int64_t ncol, double depth, double *srh_rm, double *srh_lm,
+                  float *rm, float *lm, float *mean6, uint32_t flags, void *stream)
+{
+    GridCall c;
+    c.kind = GRID_SRH;
+    c.p = p; c.p_shared = p_shared; c.start = start_lev;
+    c.t = t; c.td = td; c.ps = ps; c.ts = ts; c.tds = tds;
+    c.u = u; c.v = v; c.us = us; c.vs = vs;
+    c.nk = nk; c.ncol = ncol; c.depth = depth;
+    c.srh_rm = srh_rm; c.srh_lm = srh_lm; c.rm = rm; c.lm = lm; c.mean6 = mean6;
+    c.flags = flags & ~XCAPE_FAST_MATH; c.stream = stream;
+    return grid_entry(c);
+}
+
+int xcape_cape_srh_f32(const float *p, int p_shared, const int32_t *start_lev, const float *t,
+                       const float *td, const float *u, const float *v, const float *ps,
+                       const float *ts, const float *tds, const float *us, const float *vs,
+                       int64_t nk, int64_t ncol, float pinc, int source, float ml_depth,
+                       int adiabat, double depth, float *cape, float *cin, int32_t *mulev,
+                       float *zmulev, double *srh_rm, double *srh_lm, float *rm, float *lm,
+                       float *mean6, uint32_t flags, xcape_stats *stats, void *stream)
+{
+    GridCall c;
+    c.kind = GRID_CAPE_SRH;
+    c.p = p; c.p_shared = p_shared; c.start = start_lev;
+    c.t = t; c.td = td; c.ps = ps; c.ts = ts; c.tds = tds;
+    c.u = u; c.v = v; c.us = us; c.vs = vs;
+    c.nk = nk; c.ncol = ncol; c.pinc = pinc; c.source = source; c.ml_depth = ml_depth;
+    c.adiabat = adiabat; c.depth = depth;
+    c.cape = cape; c.cin = cin; c.mulev = mulev; c.zout = zmulev;
+    c.srh_rm = srh_rm; c.srh_lm = srh_lm; c.rm = rm; c.lm = lm; c.mean6 = mean6;
+    c.flags = flags; c.stats = stats; c.stream = stream;
+    return grid_entry(c);
+}
+
+}  // extern "C"

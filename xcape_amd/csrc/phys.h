@@ -1,0 +1,163 @@
+// Physical constants and scalar thermodynamics shared by the gfx950 kernels.
+//
+// Constants follow getcape (reference src/xcape/fortran/CAPE_CODE_model_lev.f90
+// :188-211).  For T = float every derived constant is folded in single
+// precision, as the reference's REAL parameters are; T = double uses the same
+// decimal values in double precision (the FP64 yardstick run).
+//
+// Math modes:
+//   MATH_ACCURATE - IEEE-rounded division and the device library's expf/logf/
+//                   powf (<= 1 ulp); translation units using it are compiled
+//                   with -ffp-contract=off so every +,-,*,/ rounds exactly as
+//                   the reference's does.
+//   MATH_FAST     - hardware exp2/log2 and reciprocal-based division (a few
+//                   ulp); FMA contraction allowed.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace xcape {
+
+enum { MATH_ACCURATE = 0, MATH_FAST = 1 };
+
+template <typename T> struct Phys;
+
+template <> struct Phys<float> {
+    static constexpr float g = 9.81f, p00 = 100000.0f, cp = 1005.7f, rd = 287.04f,
+                           rv = 461.5f, xlv = 2501000.0f, xls = 2836017.0f, t0 = 273.15f,
+                           cpv = 1875.0f, cpl = 4190.0f, cpi = 2118.636f;
+    static constexpr float lv1 = xlv + (cpl - cpv) * t0;
+    static constexpr float lv2 = cpl - cpv;
+    static constexpr float ls1 = xls + (cpi - cpv) * t0;
+    static constexpr float ls2 = cpi - cpv;
+    static constexpr float rp00 = 1.0f / p00;
+    static constexpr float eps = rd / rv;
+    static constexpr float reps = rv / rd;
+    static constexpr float rddcp = rd / cp;
+    static constexpr float cpdg = cp / g;
+    static constexpr float converge = 0.0002f;
+};
+
+template <> struct Phys<double> {
+    static constexpr double g = 9.81, p00 = 100000.0, cp = 1005.7, rd = 287.04, rv = 461.5,
+                            xlv = 2501000.0, xls = 2836017.0, t0 = 273.15, cpv = 1875.0,
+                            cpl = 4190.0, cpi = 2118.636;
+    static constexpr double lv1 = xlv + (cpl - cpv) * t0;
+    static constexpr double lv2 = cpl - cpv;
+    static constexpr double ls1 = xls + (cpi - cpv) * t0;
+    static constexpr double ls2 = cpi - cpv;
+    static constexpr double rp00 = 1.0 / p00;
+    static constexpr double eps = rd / rv;
+    static constexpr double reps = rv / rd;
+    static constexpr double rddcp = rd / cp;
+    static constexpr double cpdg = cp / g;
+    static constexpr double converge = 0.0002;
+};
+
+// ---- elementary functions per precision and mode ---------------------------
+template <typename T, int MATH> struct Fn;
+
+template <> struct Fn<float, MATH_ACCURATE> {
+    static __device__ __forceinline__ float exp(float x) { return expf(x); }
+    static __device__ __forceinline__ float log(float x) { return logf(x); }
+    static __device__ __forceinline__ float pow(float x, float y) { return powf(x, y); }
+    static __device__ __forceinline__ float div(float a, float b) { return a / b; }
+    // exp(c * a / b) with c*a formed first, as the reference does
+    static __device__ __forceinline__ float exp_ratio(float c, float a, float b)
+    {
+        return expf(c * a / b);
+    }
+};
+
+template <> struct Fn<float, MATH_FAST> {
+    static constexpr float LOG2E = 1.4426950408889634f;
+    static constexpr float LN2 = 0.6931471805599453f;
+    static __device__ __forceinline__ float rcp(float b) { return __builtin_amdgcn_rcpf(b); }
+    static __device__ __forceinline__ float exp(float x)
+    {
+        return __builtin_amdgcn_exp2f(x * LOG2E);
+    }
+    static __device__ __forceinline__ float log(float x)
+    {
+        return __builtin_amdgcn_logf(x) * LN2;
+    }
+    static __device__ __forceinline__ float pow(float x, float y)
+    {
+        return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
+    }
+    // one Newton step on the hardware reciprocal, then a residual correction:
+    // error below 1 ulp away from overflow/underflow, no special-case handling
+    static __device__ __forceinline__ float div(float a, float b)
+    {
+        float r = rcp(b);
+        float q = a * r;
+        float e = __builtin_fmaf(-b, q, a);
+        return __builtin_fmaf(e, r, q);
+    }
+    static __device__ __forceinline__ float exp_ratio(float c, float a, float b)
+    {
+        return __builtin_amdgcn_exp2f((c * LOG2E) * a * rcp(b));
+    }
+};
+
+template <int MATH> struct Fn<double, MATH> {
+    static __device__ __forceinline__ double exp(double x) { return ::exp(x); }
+    static __device__ __forceinline__ double log(double x) { return ::log(x); }
+    static __device__ __forceinline__ double pow(double x, double y) { return ::pow(x, y); }
+    static __device__ __forceinline__ double div(double a, double b) { return a / b; }
+    static __device__ __forceinline__ double exp_ratio(double c, double a, double b)
+    {
+        return ::exp(c * a / b);
+    }
+};
+
+// Fortran MAX/MIN as the reference evaluates them: the second operand wins
+// when the comparison is false (NaN included).
+template <typename T> __device__ __forceinline__ T fmx(T a, T b) { return a > b ? a : b; }
+template <typename T> __device__ __forceinline__ T fmn(T a, T b) { return a < b ? a : b; }
+
+// int(x) for the sub-step count: NaN/out-of-range map to INT32_MIN as the
+// reference build's truncating conversion does (the sub-step loop then runs
+// zero times).
+template <typename T> __device__ __forceinline__ int ftrunc(T x)
+{
+    if (!(x > T(-2147483648.0) && x < T(2147483648.0))) return INT32_MIN;
+    return (int)x;
+}
+
+// getqvs: Bolton saturation mixing ratio over liquid (CAPE_CODE_model_lev.f90
+// :570-581); es = 611.2 exp(17.67 (t-273.15)/(t-29.65)), r = eps es/(p-es)
+template <typename T, int MATH>
+__device__ __forceinline__ T qvs(T p, T t)
+{
+    using F = Fn<T, MATH>;
+    T es = T(611.2) * F::exp_ratio(T(17.67), t - T(273.15), t - T(29.65));
+    return F::div(Phys<T>::eps * es, p - es);
+}
+
+// getqvi: over ice (:587-598)
+template <typename T, int MATH>
+__device__ __forceinline__ T qvi(T p, T t)
+{
+    using F = Fn<T, MATH>;
+    T es = T(611.2) * F::exp_ratio(T(21.8745584), t - T(273.15), t - T(7.66));
+    return F::div(Phys<T>::eps * es, p - es);
+}
+
+// getthe: Bolton (1980) theta-e with the LCL temperature (:604-620)
+template <typename T, int MATH>
+__device__ __forceinline__ T thetae(T p, T t, T td, T q)
+{
+    using F = Fn<T, MATH>;
+    T tlcl;
+    if ((td - t) >= T(-0.1)) {
+        tlcl = t;
+    } else {
+        T a = F::div(T(1.0), td - T(56.0));
+        tlcl = T(56.0) + F::div(T(1.0), a + T(0.00125) * F::log(F::div(t, td)));
+    }
+    return t * F::pow(F::div(T(100000.0), p), T(0.2854) * (T(1.0) - T(0.28) * q)) *
+           F::exp((F::div(T(3376.0), tlcl) - T(2.54)) * q * (T(1.0) + T(0.81) * q));
+}
+
+}  // namespace xcape

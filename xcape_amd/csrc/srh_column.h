@@ -1,0 +1,181 @@
+// One column of the storm-relative-helicity chain on one lane:
+//   heights above ground (FP64 hypsometric equation, stdheight_2D_*_lev.f90),
+//   Bunkers right/left-mover motion and 0-6 km mean wind (FP32,
+//   Bunkers_*_lev.f90), storm-relative helicity 0..depth (FP64, SREH_*_lev.f90),
+// with the reference's precisions and operation order.
+//
+// Organisation (same results as the reference's three passes over stored
+// arrays, without any per-column scratch):
+//  * pass 1 walks the levels bottom-up, building H(k) and, for each of the 12
+//    Bunkers target heights, the interpolant of the *last* bracketing layer
+//    seen.  The reference searches top-down and stops at the first bracket
+//    (Bunkers_model_lev.f90:218-231), which is the last one bottom-up.
+//    Heights fall with k only for a column whose top is below its surface;
+//    that orientation (IP = 1 in the reference) is handled by a second walk.
+//  * pass 2 rebuilds H(k) from the surface up to the first level above
+//    `depth` and sums the helicity terms in the reference's order.
+//
+// Literal constants of the reference's DOUBLE PRECISION routines are default
+// REAL literals, so their values are those of the nearest float (SPD below).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace xcape {
+
+#define XC_SPD(x) ((double)(float)(x))
+
+struct SrhOut {
+    double srh_rm, srh_lm;
+    float rm_u, rm_v, lm_u, lm_v, mean_u, mean_v;
+};
+
+// virtual temperature of one level (stdheight_2D_model_lev.f90:102-127), FP64
+__device__ __forceinline__ double std_tv(double T, double Td, double P)
+{
+#pragma clang fp contract(off)
+    const double eps = XC_SPD(0.6219800858985514);
+    double Tin = T + XC_SPD(273.15);
+    double Tdin = Td + XC_SPD(273.15);
+    double E = XC_SPD(6.112) * exp((XC_SPD(53.49) - (6808.0 / Tdin)) - XC_SPD(5.09) * log(Tdin));
+    double w = eps * (E / (P - E));
+    return Tin * ((w + eps) / (eps * (1.0 + w)));
+}
+
+// H(k) = H(k-1) + (R ((Tv(k) + Tv(k-1))/2) / g) ln(P(k)/P(k-1)) (:147, :155)
+__device__ __forceinline__ double std_dh(double tv, double tv_prev, double P, double P_prev)
+{
+#pragma clang fp contract(off)
+    const double R = XC_SPD(287.04);
+    const double g = XC_SPD(-9.80665);
+    return ((R * ((tv + tv_prev) / 2.0)) / g) * log(P / P_prev);
+}
+
+// Column accessor C: P(k), Tc(k), Tdc(k), U(k), V(k) for k = 0..nk-1 with
+// k = 0 the surface (hPa, degC, m/s).  hs: surface height (2 m in calc_srh).
+template <class C>
+__device__ void srh_column(const C &col, int nk, double depth, double hs, SrhOut &out)
+{
+#pragma clang fp contract(off)
+    const float lv[12] = {500.f, 1000.f, 1500.f, 2000.f, 2500.f, 3000.f,
+                          3500.f, 4000.f, 4500.f, 5000.f, 5500.f, 6000.f};
+    float iu[12], iv[12];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) { iu[i] = -999999.0f; iv[i] = -999999.0f; }
+
+    // ---- pass 1: heights and Bunkers brackets (increasing orientation) ----
+    const double Ps = (double)col.P(0);
+    double tv_prev = std_tv((double)col.Tc(0), (double)col.Tdc(0), Ps);
+    double P_prev = Ps;
+    double H = hs;
+    float z_prev = (float)hs;
+    float u_prev = col.U(0), v_prev = col.V(0);
+    const float zsurf = (float)hs;
+    for (int k = 1; k < nk; ++k) {
+        const double Pk = (double)col.P(k);
+        const double tv = std_tv((double)col.Tc(k), (double)col.Tdc(k), Pk);
+        H = H + std_dh(tv, tv_prev, Pk, P_prev);
+        const float z = (float)H;
+        const float uk = col.U(k), vk = col.V(k);
+#pragma unroll
+        for (int i = 0; i < 12; ++i) {
+            if (z_prev <= lv[i] && z > lv[i]) {
+                const float w2 = (lv[i] - z_prev) / (z - z_prev);
+                const float w1 = (float)(1.0 - (double)w2);
+                iu[i] = w1 * u_prev + w2 * uk;
+                iv[i] = w1 * v_prev + w2 * vk;
+            }
+        }
+        z_prev = z; u_prev = uk; v_prev = vk;
+        tv_prev = tv; P_prev = Pk;
+    }
+    if (zsurf > z_prev) {
+        // heights decrease with k (IP = 1, IM = 0 in the reference): the
+        // bracket is z(k) <= h < z(k-1); the top-down search keeps the
+        // highest k, i.e. the last bracket of this bottom-up walk
+#pragma unroll
+        for (int i = 0; i < 12; ++i) { iu[i] = -999999.0f; iv[i] = -999999.0f; }
+        tv_prev = std_tv((double)col.Tc(0), (double)col.Tdc(0), Ps);
+        P_prev = Ps;
+        H = hs;
+        z_prev = (float)hs;
+        u_prev = col.U(0); v_prev = col.V(0);
+        for (int k = 1; k < nk; ++k) {
+            const double Pk = (double)col.P(k);
+            const double tv = std_tv((double)col.Tc(k), (double)col.Tdc(k), Pk);
+            H = H + std_dh(tv, tv_prev, Pk, P_prev);
+            const float z = (float)H;
+            const float uk = col.U(k), vk = col.V(k);
+#pragma unroll
+            for (int i = 0; i < 12; ++i) {
+                if (z <= lv[i] && z_prev > lv[i]) {
+                    const float w2 = (lv[i] - z) / (z_prev - z);
+                    const float w1 = (float)(1.0 - (double)w2);
+                    iu[i] = w1 * uk + w2 * u_prev;
+                    iv[i] = w1 * vk + w2 * v_prev;
+                }
+            }
+            z_prev = z; u_prev = uk; v_prev = vk;
+            tv_prev = tv; P_prev = Pk;
+        }
+    }
+
+    // ---- Bunkers (Bunkers_model_lev.f90:123-177), accumulators from zero ----
+    const float u0 = col.U(0), v0 = col.V(0);
+    float mu = 0.0f + u0, mv = 0.0f + v0;
+#pragma unroll
+    for (int i = 0; i < 12; ++i) { mu = mu + iu[i]; mv = mv + iv[i]; }
+    mu = mu / 13.0f;
+    mv = mv / 13.0f;
+    const float ushr_u = ((0.0f + iu[10]) + iu[11]) / 2.0f;
+    const float vshr_u = ((0.0f + iv[10]) + iv[11]) / 2.0f;
+    const float ushr_d = ((0.0f + u0) + iu[0]) / 2.0f;
+    const float vshr_d = ((0.0f + v0) + iv[0]) / 2.0f;
+    const float ushr = ushr_u - ushr_d;
+    const float vshr = vshr_u - vshr_d;
+    const float mag = sqrtf(ushr * ushr + vshr * vshr);
+    out.rm_u = mu + 7.5f * vshr / mag;
+    out.rm_v = mv - 7.5f * ushr / mag;
+    out.lm_u = mu - 7.5f * vshr / mag;
+    out.lm_v = mv + 7.5f * ushr / mag;
+    out.mean_u = mu;
+    out.mean_v = mv;
+
+    // ---- pass 2: SREH 0..depth (SREH_model_lev.f90:82-114), FP64 ----
+    const double cu_rm = (double)out.rm_u, cv_rm = (double)out.rm_v;
+    const double cu_lm = (double)out.lm_u, cv_lm = (double)out.lm_v;
+    double srm = 0.0, slm = 0.0;
+    tv_prev = std_tv((double)col.Tc(0), (double)col.Tdc(0), Ps);
+    P_prev = Ps;
+    H = hs;
+    double z1prev = hs;
+    double ut_prev = (double)col.U(0), vt_prev = (double)col.V(0);
+    for (int k = 1; k < nk; ++k) {
+        const double Pk = (double)col.P(k);
+        const double tv = std_tv((double)col.Tc(k), (double)col.Tdc(k), Pk);
+        H = H + std_dh(tv, tv_prev, Pk, P_prev);
+        double ut = (double)col.U(k), vt = (double)col.V(k);
+        const bool top = H > depth;
+        if (top) {
+            // INTERP1 (SREH_model_lev.f90:123-129): X1 = aglh(ktop) is lowered
+            // by 0.01 (a REAL literal) when it equals aglh(ktop-1)
+            double x1 = H;
+            if (z1prev == x1) x1 = x1 - XC_SPD(0.01);
+            ut = ut + ((ut_prev - ut) * ((depth - x1) / (z1prev - x1)));
+            vt = vt + ((vt_prev - vt) * ((depth - x1) / (z1prev - x1)));
+        }
+        const double xr = ((ut - cu_rm) * (vt - vt_prev)) - ((vt - cv_rm) * (ut - ut_prev));
+        const double xl = ((ut - cu_lm) * (vt - vt_prev)) - ((vt - cv_lm) * (ut - ut_prev));
+        srm = srm + xr;
+        slm = slm + xl;
+        if (top) break;
+        z1prev = H; ut_prev = ut; vt_prev = vt;
+        tv_prev = tv; P_prev = Pk;
+    }
+    // no level above depth: the reference's sum is empty (ktop = 0)
+    if (!(H > depth)) { srm = 0.0; slm = 0.0; }
+    out.srh_rm = -srm;
+    out.srh_lm = -slm;
+}
+
+}  // namespace xcape

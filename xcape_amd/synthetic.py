@@ -178,3 +178,75 @@ def ragged_columns(ncol, seed):
         ps[idx], ts[idx], tds[idx] = c['ps'], c['ts'], c['tds']
     return dict(offsets=offsets, p=ptot, t=ttot, td=tdtot, ps=ps, ts=ts, tds=tds,
                 nlev=nlev.astype(np.int32), source=source, adiabat=adiabat)
+
+
+def sigma_columns_torch(ncol, nlev, seed, device='cuda', winds=False):
+    """The sigma-grid generator above evaluated with torch on `device` (same
+    distributions and recipe, float64 arithmetic, float32 outputs), for
+    workloads too large to build quickly in numpy.  Values are not
+    bit-identical to ``sigma_columns`` (different random streams)."""
+    import torch
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    f64 = dict(dtype=torch.float64, device=device)
+
+    def U(lo, hi):
+        return lo + (hi - lo) * torch.rand(ncol, generator=g, **f64)
+    ps, ts = U(95000.0, 102000.0), U(285.0, 305.0)
+    lapse, ztrop = U(5.5, 8.0) * 1e-3, U(11000.0, 15000.0)
+    d0, gd = U(1.0, 15.0), U(1.0, 4.0) * 1e-3
+
+    def temp(z):
+        return ts - lapse * torch.minimum(z, ztrop)
+
+    def tvirt(t, td, p):
+        es = 611.2 * torch.exp(17.67 * (td - 273.15) / (td - 29.65))
+        r = 0.622 * es / torch.clamp(p - es, min=1.0)
+        return t * (1.0 + 0.608 * r)
+
+    sig = torch.linspace(0.995, 0.1, nlev, **f64)
+    out = {k: torch.empty((ncol, nlev), dtype=torch.float32, device=device)
+           for k in ('p', 't', 'td')}
+    zall = torch.empty((ncol, nlev), **f64) if winds else None
+    z_prev = torch.zeros(ncol, **f64)
+    p_prev = ps
+    t0 = ts
+    tds = t0 - d0
+    tv_prev = tvirt(t0, tds, ps)
+    for k in range(nlev):
+        pk = ps * sig[k]
+        lnr = torch.log(p_prev / pk)
+        zk = z_prev + RD * tv_prev / G * lnr
+        for _ in range(3):
+            tk = temp(zk)
+            tvk = tvirt(tk, tk - (d0 + gd * zk), pk)
+            zk = z_prev + RD * 0.5 * (tv_prev + tvk) / G * lnr
+        tk = temp(zk)
+        tdk = tk - (d0 + gd * zk)
+        out['p'][:, k] = (pk / 100.0).float()
+        out['t'][:, k] = (tk - 273.15).float()
+        out['td'][:, k] = (tdk - 273.15).float()
+        if winds:
+            zall[:, k] = zk
+        tv_prev = tvirt(tk, tdk, pk)
+        z_prev, p_prev = zk, pk
+    out['ps'] = (ps / 100.0).float()
+    out['ts'] = (ts - 273.15).float()
+    out['tds'] = (tds - 273.15).float()
+    if winds:
+        u0 = U(-5.0, 10.0)[:, None]
+        s = U(0.0, 0.01)[:, None]
+        th = U(0.0, 2 * np.pi)[:, None]
+
+        def prof(zz):
+            return torch.where(zz <= 6000.0, s * zz, s * 6000.0 + 0.5 * s * (zz - 6000.0))
+        sh = prof(zall)
+        noise = torch.randn((ncol, nlev), generator=g, **f64)
+        out['u'] = (u0 + sh * torch.cos(th) + noise).float()
+        noise = torch.randn((ncol, nlev), generator=g, **f64)
+        out['v'] = (sh * torch.sin(th) + noise).float()
+        shs = prof(torch.full((ncol, 1), 2.0, **f64))
+        out['us'] = (u0 + shs * torch.cos(th))[:, 0].float()
+        out['vs'] = (shs * torch.sin(th))[:, 0].float()
+        del zall
+    return {k: v.contiguous() for k, v in out.items()}
