@@ -13,8 +13,10 @@
  *
  * Pointers are host pointers unless XCAPE_DEVICE_PTRS is set, in which case
  * every array argument is a device pointer and the call is asynchronous on
- * `stream` (a hipStream_t, or NULL for the calling thread's stream) unless
- * `stats` is non-NULL (the counters are then read back synchronously).
+ * `stream` (a hipStream_t; NULL is HIP's default stream) unless `stats` is
+ * non-NULL (the counters are then read back synchronously).  Host-pointer
+ * calls are synchronous; with stream NULL they run on a stream of the calling
+ * thread's own.
  */
 #ifndef XCAPE_HIP_H
 #define XCAPE_HIP_H

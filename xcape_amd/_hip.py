@@ -70,6 +70,32 @@ SIGNATURES = {
 }
 
 
+ROCM_HIP = os.environ.get('XCAPE_HIP_RUNTIME', '/opt/rocm/lib/libamdhip64.so.7')
+
+
+def _hip_runtime():
+    """Make exactly one HIP runtime visible with global scope.
+
+    libxcape_hip.so carries no hard dependency on libamdhip64: its HIP calls
+    bind to the runtime already in the process.  When torch (ROCm build) is
+    importable its bundled runtime is the one: importing torch loads it, and
+    re-opening it with RTLD_GLOBAL exposes it to the library, so streams and
+    device memory are shared with torch.  Otherwise the system runtime is
+    loaded with global scope.  Returns the path used.
+    """
+    try:
+        import torch
+        if getattr(torch.version, 'hip', None):
+            path = os.path.join(os.path.dirname(torch.__file__), 'lib', 'libamdhip64.so')
+            if os.path.exists(path):
+                ctypes.CDLL(path, mode=os.RTLD_GLOBAL | os.RTLD_NOLOAD)
+                return path
+    except (ImportError, OSError):
+        pass
+    ctypes.CDLL(ROCM_HIP, mode=os.RTLD_GLOBAL)
+    return ROCM_HIP
+
+
 def lib():
     """Load the library (once) and declare the prototypes."""
     global _lib
@@ -77,6 +103,7 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(
                 f'{LIB_PATH} is not built; run `python -m xcape_amd.build` (hipcc, gfx950)')
+        _hip_runtime()
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

@@ -21,6 +21,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, 'csrc')
 LIB = os.path.join(HERE, 'libxcape_hip.so')
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
+CLANG = os.environ.get('XCAPE_LINKER', '/opt/rocm/llvm/bin/clang++')
 ARCH = 'gfx950'
 
 COMMON = ['--offload-arch=' + ARCH, '-O3', '-std=c++17', '-fPIC',
@@ -31,7 +32,7 @@ COMMON = ['--offload-arch=' + ARCH, '-O3', '-std=c++17', '-fPIC',
 UNITS = {
     'capi.hip': ['-ffp-contract=off'],
     'cape_accurate.hip': ['-ffp-contract=off'],
-    'cape_fast.hip': ['-ffp-contract=fast'],
+    'cape_fast.hip': ['-ffp-contract=fast-honor-pragmas'],
     'ragged.hip': ['-ffp-contract=off'],
 }
 
@@ -55,7 +56,12 @@ def build(verbose=True):
         with ThreadPoolExecutor(max_workers=min(8, len(UNITS))) as ex:
             objs = list(ex.map(lambda kv: _compile(kv[0], kv[1], tmp), UNITS.items()))
         out_tmp = LIB + '.tmp'
-        cmd = [HIPCC, '--offload-arch=' + ARCH, '-shared', '-fPIC', '-o', out_tmp] + objs
+        # Link without a DT_NEEDED on libamdhip64: the HIP symbols bind to the
+        # runtime already loaded with global scope (the process's only one;
+        # xcape_amd._hip arranges that, a C caller links -lamdhip64).  A hard
+        # dependency would pull /opt/rocm's runtime into a process that already
+        # runs torch's bundled copy.
+        cmd = [CLANG, '-shared', '-fPIC', '-o', out_tmp] + objs + ['-lstdc++', '-lm']
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError('link failed:\n' + r.stderr)

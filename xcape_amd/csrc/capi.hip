@@ -151,8 +151,7 @@ int grid_entry(const GridCall &c)
         return fail(XCAPE_EINVAL, "the FP64 entry point supports neither FAST_MATH nor WARM_START");
     if (c.flags & XCAPE_WARM_START)
         return fail(XCAPE_ESTATE, "warm start requested but no surrogate is loaded");
-    const bool dev = c.flags & XCAPE_DEVICE_PTRS;
-    if (!dev && c.p_shared && c.start) {
+    if (!(c.flags & XCAPE_DEVICE_PTRS) && c.p_shared && c.start) {
         for (int64_t i = 0; i < c.ncol; ++i)
             if (c.start[i] < 1 || c.start[i] > c.nk)
                 return fail(XCAPE_EINVAL, "start_lev[%lld]=%d outside 1..%lld", (long long)i,
@@ -160,7 +159,12 @@ int grid_entry(const GridCall &c)
     }
     if ((rc = ensure_device())) return rc;
     if (c.ncol == 0) return XCAPE_OK;
-    hipStream_t s = c.stream ? (hipStream_t)c.stream : thread_stream();
+    const bool dev = c.flags & XCAPE_DEVICE_PTRS;
+    // NULL stream: HIP's default stream for device-pointer calls (the
+    // caller's work on it is ordered before ours), the calling thread's own
+    // stream for host-pointer calls (synchronous anyway; keeps concurrent
+    // callers apart)
+    hipStream_t s = c.stream ? (hipStream_t)c.stream : (dev ? (hipStream_t)0 : thread_stream());
     const size_t osz = c.fp64 ? sizeof(double) : sizeof(float);
     const size_t n2 = (size_t)c.nk * (size_t)c.ncol;
     const size_t np = c.p_shared ? (size_t)c.nk : n2;

@@ -301,7 +301,7 @@ int xcape_ragged_plan_create(const int64_t *offsets_host, const int32_t *source_
 int xcape_ragged_plan_run(xcape_ragged_plan *pl, xcape_stats *stats, void *stream)
 {
     if (!pl) return fail(XCAPE_EINVAL, "plan is NULL");
-    hipStream_t s = stream ? (hipStream_t)stream : thread_stream();
+    hipStream_t s = (hipStream_t)stream;   // NULL: the default stream
     hipError_t e;
     if (stats) {
         // counted run: plain launch with the counters attached (not the graph)
