@@ -38,8 +38,14 @@ extern "C" {
 /* ---- flags ---------------------------------------------------------------- */
 #define XCAPE_DEVICE_PTRS 0x1u  /* array arguments are device pointers        */
 #define XCAPE_FAST_MATH 0x2u    /* hardware exp2/log2/rcp variant (a few ulp)  */
-#define XCAPE_WARM_START 0x4u   /* learned warm start of the moist-adiabat
-                                   solve (weights from xcape_surrogate_load)   */
+#define XCAPE_WARM_START 0x4u   /* warm-started moist-adiabat solve: each
+                                   sub-step's first iterate predicted from the
+                                   sub-steps already solved (and, with a loaded
+                                   surrogate, from the MLP); same fixed point,
+                                   reference tolerance, cold-start fallback   */
+
+#define XCAPE_NO_SURROGATE 0x8u /* with XCAPE_WARM_START: ignore a loaded
+                                   surrogate (extrapolation only)              */
 
 /* enums as the reference's Python layer maps them (core.py:302-311) */
 #define XCAPE_SOURCE_SURFACE 1
@@ -155,7 +161,7 @@ int xcape_cape_srh_f32(const float *p, int p_shared, const int32_t *start_lev,
  *
  * chunk: columns a wavefront takes from the work queue per grab (64, 128 or
  * 256); tile: levels per column staged in LDS (0..64); waves_per_block 1..4.
- * flags: XCAPE_FAST_MATH or 0.
+ * flags: XCAPE_FAST_MATH and/or XCAPE_WARM_START, or 0.
  */
 typedef struct xcape_ragged_plan xcape_ragged_plan;
 
@@ -177,6 +183,55 @@ int xcape_ragged_plan_run(xcape_ragged_plan *plan, xcape_stats *stats, void *str
 int xcape_ragged_plan_info(const xcape_ragged_plan *plan, int *grid, int *block);
 
 int xcape_ragged_plan_destroy(xcape_ragged_plan *plan);
+
+/*
+ * The warm-start surrogate: a 3 -> 64 -> 64 -> 1 MLP (tanh-form GELU) with
+ * BF16 weights and FP32 accumulation.  No reference counterpart (the
+ * reference always starts the moist-adiabat iteration cold,
+ * CAPE_CODE_model_lev.f90:431).
+ *
+ * Inputs, normalised by norm = {mean[3], scale[3], out_scale, out_offset}:
+ * ln p2 [ln Pa], the parcel's theta-e at the sub-step's start [K] (the
+ * reference's parcel formula getthe(p, t, t, qv)), total water [kg/kg].
+ * Output: the sub-step rate (theta2 - theta1) / ln(p2/p1) [K].
+ *
+ * Parameters: an FP32 master vector of xcape_mlp_nparams() floats (W1 64x3,
+ * b1, W2 64x64, b2, w3, b3).  Every pointer below is a device pointer; calls
+ * are asynchronous on `stream` (NULL: default stream) unless stated.
+ */
+int xcape_mlp_nparams(void);
+
+/* Network output for n normalised inputs x (n, 3) -> y (n,), VALU path. */
+int xcape_mlp_forward(const float *x, int64_t n, const float *theta, float *y, void *stream);
+
+/* Mean-squared-error loss over (x, t) and its gradient with respect to the
+ * parameter vector (straight-through through the BF16 rounding). */
+int xcape_mlp_loss_grad(const float *x, const float *t, int64_t n, const float *theta,
+                        float *grad, float *loss, void *stream);
+
+/* One Adam step on theta (m, v: moment vectors, step >= 1). */
+int xcape_mlp_adam(float *theta, const float *grad, float *m, float *v, int64_t step, float lr,
+                   float beta1, float beta2, float eps, float weight_decay, void *stream);
+
+/* Make theta (and norm, 8 floats) the surrogate the warm-started kernels
+ * use; synchronous.  xcape_surrogate_loaded() says whether one is loaded. */
+int xcape_surrogate_load(const float *theta, const float *norm, void *stream);
+int xcape_surrogate_loaded(void);
+
+/* The loaded surrogate on raw inputs x (n, 3) through the in-kernel path
+ * (wave-wide BF16 MFMA) -> rates y (n,). */
+int xcape_surrogate_eval(const float *x, int64_t n, float *y, void *stream);
+
+/* Training data from the cold-start ascent of a sigma grid (device arrays
+ * as xcape_cape_f32): for a hashed `keep_fraction` of the sub-steps, append
+ * (ln p2, parcel theta-e, qt, rate) to samples (capacity, 4); *count
+ * (device) is advanced atomically and may exceed capacity (extra records are
+ * dropped). */
+int xcape_surrogate_log_f32(const float *p, const float *t, const float *td,
+                            const float *ps, const float *ts, const float *tds,
+                            int64_t nk, int64_t ncol, float pinc, int source, float ml_depth,
+                            int adiabat, float keep_fraction, uint32_t seed, float *samples,
+                            int64_t capacity, unsigned long long *count, void *stream);
 
 #ifdef __cplusplus
 }

@@ -174,3 +174,32 @@ def test_device_pointer_path(hip):
     torch.cuda.synchronize()
     for x, y in zip(h, d):
         np.testing.assert_array_equal(x, y.cpu().numpy())
+
+
+@pytest.mark.parametrize('adiabat', [1, 2, 3, 4])
+@pytest.mark.parametrize('source', [1, 2, 3])
+def test_warm_start_within_tolerance(hip, oracle_lib, source, adiabat):
+    """The warm-started iteration converges to the reference's tolerance from
+    a predicted first iterate: results stay within 0.5 J/kg / 0.1 % of the
+    cold-start oracle and the iteration count drops."""
+    c = synthetic.sigma_columns(4096, 137, 1004)
+    args = (c['p'], c['t'], c['td'], c['ps'], c['ts'], c['tds'])
+    w = hip.cape(*args, pinc=500., source=source, adiabat=adiabat, warm_start=True, stats=True)
+    p = hip.cape(*args, pinc=500., source=source, adiabat=adiabat, stats=True)
+    o = oracle_lib.cape(*args, pinc=500., source=source, adiabat=adiabat)
+    print('warm', w[4], 'plain', p[4], flush=True)
+    assert not _report(f'warm s{source} a{adiabat} cape', w[0], o[0]).any()
+    assert not _report(f'warm s{source} a{adiabat} cin', w[1], o[1]).any()
+    np.testing.assert_array_equal(w[2], p[2])           # source selection untouched
+    assert w[4]['substeps'] == p[4]['substeps'] or source == 2
+    assert w[4]['iterations'] < 0.4 * p[4]['iterations']
+    assert w[4]['fallbacks'] < 0.1 * w[4]['substeps']
+
+
+def test_warm_start_fast_math(hip, oracle_lib):
+    c = synthetic.sigma_columns(4096, 137, 1004)
+    args = (c['p'], c['t'], c['td'], c['ps'], c['ts'], c['tds'])
+    w = hip.cape(*args, pinc=500., source=2, adiabat=3, warm_start=True, fast_math=True)
+    o = oracle_lib.cape(*args, pinc=500., source=2, adiabat=3)
+    assert not _report('warm fast cape', w[0], o[0]).any()
+    assert not _report('warm fast cin', w[1], o[1]).any()

@@ -85,3 +85,19 @@ def test_autotuner_picks_a_timed_config(batch_np, tmp_path, monkeypatch):
                                         f"waves={best['waves']}"]
     best2, _ = ragged.autotune(b, chunks=(64, 128), tiles=(0, 16), waves=(2, 4), reps=1)
     assert best2 == best              # served from the cache
+
+
+def test_ragged_warm_start(batch_np, oracle_lib):
+    import torch
+
+    from xcape_amd.ragged import RaggedBatch, RaggedPlan
+    b = RaggedBatch.from_dict(batch_np)
+    plan = RaggedPlan(b, chunk=64, tile=16, waves=4, warm_start=True)
+    st = plan.run(stats=True)
+    torch.cuda.synchronize()
+    got = [x.cpu().numpy() for x in (b.cape, b.cin)]
+    ref = _oracle_ragged(oracle_lib, batch_np)
+    print('ragged warm', st, flush=True)
+    assert np.all(cape_ok(got[0], ref[0])) and np.all(cape_ok(got[1], ref[1]))
+    assert st['iterations'] < 0.45 * 9.5 * st['substeps']
+    plan.close()

@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(HERE, 'libxcape_hip.so')
 XCAPE_DEVICE_PTRS = 0x1
 XCAPE_FAST_MATH = 0x2
 XCAPE_WARM_START = 0x4
+XCAPE_NO_SURROGATE = 0x8
 
 
 class XcapeError(RuntimeError):
@@ -67,6 +68,15 @@ SIGNATURES = {
     'xcape_ragged_plan_run': (_I, [_P, _P, _P]),
     'xcape_ragged_plan_info': (_I, [_P, _P, _P]),
     'xcape_ragged_plan_destroy': (_I, [_P]),
+    'xcape_mlp_nparams': (_I, []),
+    'xcape_mlp_forward': (_I, [_P, _I64, _P, _P, _P]),
+    'xcape_mlp_loss_grad': (_I, [_P, _P, _I64, _P, _P, _P, _P]),
+    'xcape_mlp_adam': (_I, [_P, _P, _P, _P, _I64, _F, _F, _F, _F, _F, _P]),
+    'xcape_surrogate_load': (_I, [_P, _P, _P]),
+    'xcape_surrogate_loaded': (_I, []),
+    'xcape_surrogate_eval': (_I, [_P, _I64, _P, _P]),
+    'xcape_surrogate_log_f32': (_I, [_P, _P, _P, _P, _P, _P, _I64, _I64, _F, _I, _F, _I, _F,
+                                     _U32, _P, _I64, _P, _P]),
 }
 
 
@@ -176,6 +186,17 @@ class _Args:
         return np.empty(n, dtype)
 
 
+def _warm_flags(warm_start):
+    """warm_start: False (cold), True (extrapolation + loaded surrogate) or
+    'extrapolation' (ignore a loaded surrogate)."""
+    if not warm_start:
+        return 0
+    f = XCAPE_WARM_START
+    if warm_start == 'extrapolation':
+        f |= XCAPE_NO_SURROGATE
+    return f
+
+
 def cape(p, t, td, ps, ts, tds, *, pinc=500., source=1, ml_depth=500., adiabat=1,
          p_shared=False, start_lev=None, fast_math=False, warm_start=False, fp64=False,
          stats=False):
@@ -189,7 +210,7 @@ def cape(p, t, td, ps, ts, tds, *, pinc=500., source=1, ml_depth=500., adiabat=1
     cape_o, cin_o, z_o = (A.empty(ncol, odt, t) for _ in range(3))
     mul_o = A.empty(ncol, np.int32, t)
     flags = (XCAPE_DEVICE_PTRS if A.device else 0) | (XCAPE_FAST_MATH if fast_math else 0) \
-        | (XCAPE_WARM_START if warm_start else 0)
+        | _warm_flags(warm_start)
     st = Stats()
     fn = lib().xcape_cape_f64 if fp64 else lib().xcape_cape_f32
     rc = fn(_ptr(p), 1 if p_shared else 0, _ptr(start_lev), _ptr(t), _ptr(td), _ptr(ps),
@@ -242,7 +263,7 @@ def cape_srh(p, t, td, u, v, ps, ts, tds, us, vs, *, pinc=500., source=1, ml_dep
     srm, slm = (A.empty(ncol, np.float64, t) for _ in range(2))
     rm, lm, mw = (A.empty(2 * ncol, np.float32, t) for _ in range(3))
     flags = (XCAPE_DEVICE_PTRS if A.device else 0) | (XCAPE_FAST_MATH if fast_math else 0) \
-        | (XCAPE_WARM_START if warm_start else 0)
+        | _warm_flags(warm_start)
     st = Stats()
     rc = lib().xcape_cape_srh_f32(
         _ptr(p), 1 if p_shared else 0, _ptr(start_lev), _ptr(t), _ptr(td), _ptr(u), _ptr(v),

@@ -34,6 +34,8 @@ UNITS = {
     'cape_accurate.hip': ['-ffp-contract=off'],
     'cape_fast.hip': ['-ffp-contract=fast-honor-pragmas'],
     'ragged.hip': ['-ffp-contract=off'],
+    'mlp.hip': ['-ffp-contract=fast'],
+    'surrogate_log.hip': ['-ffp-contract=off'],
 }
 
 

@@ -18,6 +18,13 @@
 //    reference evaluates and multiplies by zero (:445); identical unless it
 //    overflows to Inf (p equal to the ice saturation pressure).
 //
+// Two ascent drivers share the pieces below:
+//  * cape_column      - the reference's nested level / sub-step / iteration
+//                       loops with the cold start thlast = th1 (:431);
+//  * cape_column_warm - the same iteration started from a predicted theta and
+//                       driven by a flattened loop (one fixed-point iteration
+//                       per lane per trip), see its comment.
+//
 // The column accessor C provides P(k), Tc(k), Tdc(k) in hPa / degC for
 // k = 0 .. nk-1, k = 0 being the surface.
 #pragma once
@@ -33,7 +40,7 @@ template <typename T> struct ColumnOut {
 };
 
 struct LaneStats {
-    uint32_t levels = 0, substeps = 0, iterations = 0, nonconverged = 0;
+    uint32_t levels = 0, substeps = 0, iterations = 0, nonconverged = 0, fallbacks = 0;
 };
 
 template <typename T> struct Env {
@@ -42,7 +49,7 @@ template <typename T> struct Env {
 
 // :232-240 for one level
 template <typename T, int MATH, class C>
-__device__ __forceinline__ Env<T> env_at(const C& col, int k)
+XC_HD __forceinline__ Env<T> env_at(const C &col, int k)
 {
     using F = Fn<T, MATH>;
     using K = Phys<T>;
@@ -59,62 +66,69 @@ __device__ __forceinline__ Env<T> env_at(const C& col, int k)
 
 // layer thickness from the hydrostatic equation in Exner pressure (:246, :503)
 template <typename T>
-__device__ __forceinline__ T layer_dz(T thv_k, T thv_km1, T pi_k, T pi_km1)
+XC_HD __forceinline__ T layer_dz(T thv_k, T thv_km1, T pi_k, T pi_km1)
 {
     return -Phys<T>::cpdg * T(0.5) * (thv_k + thv_km1) * (pi_k - pi_km1);
 }
 
-// Moist-adiabat fixed point for one sub-step (:428-480).  Returns false when
-// the iteration limit is exceeded (the reference then zeroes CAPE/CIN and
-// returns, :465-474).  On entry th2 holds the starting guess (the cold start
-// is th1, :431); on exit th2/t2/qv2/ql2/qi2 are the converged state.
+// One fixed-point evaluation (:437-462): the parcel state at p2 implied by
+// the iterate thlast, and the theta it gives back.
 template <typename T, int MATH, bool ICE>
-__device__ __forceinline__ bool solve_substep(T p1, T p2, T pi2, T t1, T th1, T qv1, T ql1,
-                                              T qi1, T qt, T thlast, T& th2, T& t2, T& qv2,
-                                              T& ql2, T& qi2, LaneStats& st)
+XC_HD __forceinline__ void fp_eval(T p2, T pi2, T logp, T t1, T th1, T qv1, T ql1, T qi1,
+                                        T qt, T thlast, T &th2, T &t2, T &qv2, T &ql2, T &qi2)
 {
     using F = Fn<T, MATH>;
     using K = Phys<T>;
+    t2 = thlast * pi2;
+    if (ICE) {
+        const T fliq = fmx(fmn(F::div(t2 - T(233.15), T(273.15) - T(233.15)), T(1.0)), T(0.0));
+        const T fice = T(1.0) - fliq;
+        qv2 = fmn(qt, fliq * qvs<T, MATH>(p2, t2) + fice * qvi<T, MATH>(p2, t2));
+        qi2 = fmx(fice * (qt - qv2), T(0.0));
+    } else {
+        qv2 = fmn(qt, qvs<T, MATH>(p2, t2));
+        qi2 = T(0.0);
+    }
+    ql2 = fmx(qt - qv2 - qi2, T(0.0));
+    const T tbar = T(0.5) * (t1 + t2);
+    const T qvbar = T(0.5) * (qv1 + qv2);
+    const T qlbar = T(0.5) * (ql1 + ql2);
+    const T lhv = K::lv1 - K::lv2 * tbar;
+    const T rm = K::rd + K::rv * qvbar;
+    T arg;
+    if (ICE) {
+        const T qibar = T(0.5) * (qi1 + qi2);
+        const T lhs = K::ls1 - K::ls2 * tbar;
+        const T cpm = K::cp + K::cpv * qvbar + K::cpl * qlbar + K::cpi * qibar;
+        const T den = cpm * tbar;
+        arg = F::div(lhv * (ql2 - ql1), den) + F::div(lhs * (qi2 - qi1), den) +
+              (F::div(rm, cpm) - K::rddcp) * logp;
+    } else {
+        const T cpm = K::cp + K::cpv * qvbar + K::cpl * qlbar;
+        arg = F::div(lhv * (ql2 - ql1), cpm * tbar) + (F::div(rm, cpm) - K::rddcp) * logp;
+    }
+    th2 = th1 * F::exp(arg);
+}
+
+// Moist-adiabat fixed point for one sub-step from the cold start (:428-480).
+// Returns false when the iteration limit is exceeded (the reference then
+// zeroes CAPE/CIN and returns, :465-474).
+template <typename T, int MATH, bool ICE>
+XC_HD __forceinline__ bool solve_substep(T p1, T p2, T pi2, T t1, T th1, T qv1, T ql1,
+                                              T qi1, T qt, T thlast, T &th2, T &t2, T &qv2,
+                                              T &ql2, T &qi2, LaneStats &st)
+{
+    using F = Fn<T, MATH>;
     const T logp = F::log(F::div(p2, p1));
-    const T ln_coef = K::rddcp;
     int i = 0;
     while (true) {
         i = i + 1;
-        t2 = thlast * pi2;
-        T fliq, fice;
-        if (ICE) {
-            fliq = fmx(fmn(F::div(t2 - T(233.15), T(273.15) - T(233.15)), T(1.0)), T(0.0));
-            fice = T(1.0) - fliq;
-            qv2 = fmn(qt, fliq * qvs<T, MATH>(p2, t2) + fice * qvi<T, MATH>(p2, t2));
-            qi2 = fmx(fice * (qt - qv2), T(0.0));
-        } else {
-            qv2 = fmn(qt, qvs<T, MATH>(p2, t2));
-            qi2 = T(0.0);
-        }
-        ql2 = fmx(qt - qv2 - qi2, T(0.0));
-
-        const T tbar = T(0.5) * (t1 + t2);
-        const T qvbar = T(0.5) * (qv1 + qv2);
-        const T qlbar = T(0.5) * (ql1 + ql2);
-        const T lhv = K::lv1 - K::lv2 * tbar;
-        const T rm = K::rd + K::rv * qvbar;
-        T arg;
-        if (ICE) {
-            const T qibar = T(0.5) * (qi1 + qi2);
-            const T lhs = K::ls1 - K::ls2 * tbar;
-            const T cpm = K::cp + K::cpv * qvbar + K::cpl * qlbar + K::cpi * qibar;
-            const T den = cpm * tbar;
-            arg = F::div(lhv * (ql2 - ql1), den) + F::div(lhs * (qi2 - qi1), den) +
-                  (F::div(rm, cpm) - ln_coef) * logp;
-        } else {
-            const T cpm = K::cp + K::cpv * qvbar + K::cpl * qlbar;
-            arg = F::div(lhv * (ql2 - ql1), cpm * tbar) + (F::div(rm, cpm) - ln_coef) * logp;
-        }
-        th2 = th1 * F::exp(arg);
+        fp_eval<T, MATH, ICE>(p2, pi2, logp, t1, th1, qv1, ql1, qi1, qt, thlast, th2, t2, qv2,
+                              ql2, qi2);
         st.iterations++;
         if (i > 100) return false;
         const T d = th2 - thlast;
-        if ((d < T(0.0) ? -d : d) > K::converge) {
+        if ((d < T(0.0) ? -d : d) > Phys<T>::converge) {
             thlast = thlast + T(0.3) * (th2 - thlast);
         } else {
             return true;
@@ -122,28 +136,25 @@ __device__ __forceinline__ bool solve_substep(T p1, T p2, T pi2, T t1, T th1, T 
     }
 }
 
-// Warm-start policy: WarmStart::guess(...) returns the first iterate of the
-// fixed point for a sub-step; ColdStart reproduces the reference (:431).
-struct ColdStart {
-    template <typename T>
-    __device__ __forceinline__ T guess(T th1, T, T, T, T) const { return th1; }
+// Parcel source (:257-379): chooses the starting level kmax, the parcel's
+// initial state and the height of kmax.  Returns false for an unknown source
+// (unreachable through the Python layer; the reference stops there).
+template <typename T> struct Parcel {
+    int kmax, mulvl;
+    T th2, pi2, p2, t2, qv2, b2, zk;
+    Env<T> e;       // environment at kmax
 };
 
-template <typename T, int MATH, bool ICE, bool PSEUDO, class C, class WS = ColdStart>
-__device__ void cape_column(const C& col, int nk, T pinc, int source, T ml_depth,
-                            ColumnOut<T>& out, LaneStats& st, const WS& ws = WS())
+template <typename T, int MATH, class C>
+XC_HD __forceinline__ bool parcel_source(const C &col, int nk, int source, T ml_depth,
+                                              Parcel<T> &pc)
 {
     using F = Fn<T, MATH>;
     using K = Phys<T>;
-
-    int mulvl = -999999;
-    T zout = T(-999999.0);
-
-    // ---- parcel source (:257-352) --------------------------------------
+    pc.mulvl = -999999;
     int kmax = 0;
-    T avgth = T(0.0), avgqv = T(0.0);   // ML means; double in the reference
-    double avgth_d = 0.0, avgqv_d = 0.0;
-    T zk = T(0.0);                      // z at kmax
+    double avgth_d = 0.0, avgqv_d = 0.0;   // real*8 in the reference (:183)
+    T zk = T(0.0);
     if (source == SRC_MU) {
         T p0 = T(100.0) * T(col.P(0));
         if (!(p0 < T(50000.0))) {
@@ -154,13 +165,12 @@ __device__ void cape_column(const C& col, int nk, T pinc, int source, T ml_depth
                     Env<T> e = env_at<T, MATH>(col, k);
                     T the = thetae<T, MATH>(e.p, e.t, e.td, e.q);
                     if (the > maxthe) {
-                        mulvl = k + 1;
+                        pc.mulvl = k + 1;
                         maxthe = the;
                         kmax = k;
                     }
                 }
             }
-            // height of the source level
             if (kmax > 0) {
                 Env<T> ep = env_at<T, MATH>(col, 0);
                 for (int k = 1; k <= kmax; ++k) {
@@ -216,92 +226,308 @@ __device__ void cape_column(const C& col, int nk, T pinc, int source, T ml_depth
             avgqv_d = avgqv_d / (double)ml_depth;
             kmax = 0;
         }
-        avgth = (T)avgth_d;
-        avgqv = (T)avgqv_d;
     } else if (source != SRC_SURFACE) {
-        out.cape = T(0.0); out.cin = T(0.0); out.mulvl = mulvl; out.zout = zout;
+        return false;
+    }
+
+    // parcel at its initial location (:355-375)
+    pc.e = env_at<T, MATH>(col, kmax);
+    if (source != SRC_ML) {
+        pc.th2 = pc.e.th; pc.pi2 = pc.e.pi; pc.p2 = pc.e.p; pc.t2 = pc.e.t; pc.qv2 = pc.e.q;
+        pc.b2 = T(0.0);
+    } else {
+        pc.th2 = (T)avgth_d;
+        pc.qv2 = (T)avgqv_d;
+        T thv2 = F::div(pc.th2 * (T(1.0) + K::reps * pc.qv2), T(1.0) + pc.qv2);
+        pc.pi2 = pc.e.pi; pc.p2 = pc.e.p;
+        pc.t2 = pc.th2 * pc.pi2;
+        pc.b2 = F::div(K::g * (thv2 - pc.e.thv), pc.e.thv);
+    }
+    pc.kmax = kmax;
+    pc.zk = zk;
+    return true;
+}
+
+// Buoyancy of the parcel at level k and the CAPE/CIN trapezoid of the layer
+// below it (:501-547).  CIN is committed only on entering positive buoyancy.
+template <typename T, int MATH>
+XC_HD __forceinline__ void layer_update(T th2, T qv2, T ql2, T qi2, const Env<T> &e,
+                                             T thv_prev, T pi_prev, T b1, T &b2, T &dz,
+                                             T &cape, T &cin, T &narea)
+{
+    using F = Fn<T, MATH>;
+    using K = Phys<T>;
+    const T thv2 = F::div(th2 * (T(1.0) + K::reps * qv2), T(1.0) + qv2 + ql2 + qi2);
+    b2 = F::div(K::g * (thv2 - e.thv), e.thv);
+    dz = layer_dz(e.thv, thv_prev, e.pi, pi_prev);
+    T parea;
+    if (b2 >= T(0.0) && b1 < T(0.0)) {
+        const T frac = F::div(b2, b2 - b1);
+        parea = T(0.5) * b2 * dz * frac;
+        narea = narea - T(0.5) * b1 * dz * (T(1.0) - frac);
+        cin = cin + narea;
+        narea = T(0.0);
+    } else if (b2 < T(0.0) && b1 > T(0.0)) {
+        const T frac = F::div(b1, b1 - b2);
+        parea = T(0.5) * b1 * dz * frac;
+        narea = -T(0.5) * b2 * dz * (T(1.0) - frac);
+    } else if (b2 < T(0.0)) {
+        parea = T(0.0);
+        narea = narea - T(0.5) * dz * (b1 + b2);
+    } else {
+        parea = T(0.5) * dz * (b1 + b2);
+        narea = T(0.0);
+    }
+    cape = cape + fmx(T(0.0), parea);
+}
+
+// sub-step count and size for the layer p_prev -> p_k (:408-415)
+template <typename T, int MATH>
+XC_HD __forceinline__ int substep_count(T &dp, T pinc)
+{
+    if (dp < pinc) return 1;
+    const int nloop = 1 + ftrunc(Fn<T, MATH>::div(dp, pinc));
+    dp = Fn<T, MATH>::div(dp, (T)nloop);
+    return nloop;
+}
+
+// ---------------------------------------------------------------------------
+// Sub-step observer of the reference driver: called after every converged
+// sub-step with its start state (p1, t1, th1, qv1), end pressure p2, total
+// water qt and converged th2.  NoLog is the default; the training-data logger
+// records the surrogate's inputs and target.
+struct NoLog {
+    template <typename T>
+    XC_HD __forceinline__ void operator()(T, T, T, T, T, T, T) const {}
+};
+
+// Reference driver: nested loops, cold start.
+template <typename T, int MATH, bool ICE, bool PSEUDO, class C, class LOG = NoLog>
+XC_HD void cape_column(const C &col, int nk, T pinc, int source, T ml_depth,
+                       ColumnOut<T> &out, LaneStats &st, const LOG &log = LOG())
+{
+    using K = Phys<T>;
+    Parcel<T> pc;
+    if (!parcel_source<T, MATH>(col, nk, source, ml_depth, pc)) {
+        out.cape = T(0.0); out.cin = T(0.0); out.mulvl = pc.mulvl; out.zout = T(-999999.0);
         return;
     }
-
-    // ---- parcel at its initial location (:355-379) ---------------------
-    Env<T> ek = env_at<T, MATH>(col, kmax);
-    T th2, pi2, p2, t2, qv2, b2;
-    if (source != SRC_ML) {
-        th2 = ek.th; pi2 = ek.pi; p2 = ek.p; t2 = ek.t; qv2 = ek.q; b2 = T(0.0);
-    } else {
-        th2 = avgth;
-        qv2 = avgqv;
-        T thv2 = F::div(th2 * (T(1.0) + K::reps * qv2), T(1.0) + qv2);
-        pi2 = ek.pi; p2 = ek.p;
-        t2 = th2 * pi2;
-        b2 = F::div(K::g * (thv2 - ek.thv), ek.thv);
-    }
+    const int mulvl = pc.mulvl;
+    T zout = T(-999999.0);
+    T th2 = pc.th2, pi2 = pc.pi2, p2 = pc.p2, t2 = pc.t2, qv2 = pc.qv2, b2 = pc.b2, zk = pc.zk;
     T ql2 = T(0.0), qi2 = T(0.0), qt = qv2;
     T cape = T(0.0), cin = T(0.0), narea = T(0.0);
-    T p_prev = ek.p, pi_prev = ek.pi, thv_prev = ek.thv;
-    int k = kmax;
+    T p_prev = pc.e.p, pi_prev = pc.e.pi, thv_prev = pc.e.thv;
+    int k = pc.kmax;
     bool doit = true;
 
-    // ---- ascent (:403-559) -----------------------------------------------
+    // ascent (:403-559)
     while (doit && k < nk - 1) {
         k = k + 1;
         Env<T> e = env_at<T, MATH>(col, k);
         const T b1 = b2;
         T dp = p_prev - e.p;
-        int nloop;
-        if (dp < pinc) {
-            nloop = 1;
-        } else {
-            nloop = 1 + ftrunc(F::div(dp, pinc));
-            dp = F::div(dp, (T)nloop);
-        }
+        const int nloop = substep_count<T, MATH>(dp, pinc);
         st.levels++;
         for (int n = 1; n <= nloop; ++n) {
             const T p1 = p2, t1 = t2, th1 = th2, qv1 = qv2, ql1 = ql2, qi1 = qi2;
             p2 = p2 - dp;
-            pi2 = F::pow(p2 * K::rp00, K::rddcp);
+            pi2 = Fn<T, MATH>::pow(p2 * K::rp00, K::rddcp);
             st.substeps++;
-            const T guess = ws.guess(th1, p2, qt, t1, pi2);
-            if (!solve_substep<T, MATH, ICE>(p1, p2, pi2, t1, th1, qv1, ql1, qi1, qt, guess, th2,
+            if (!solve_substep<T, MATH, ICE>(p1, p2, pi2, t1, th1, qv1, ql1, qi1, qt, th1, th2,
                                              t2, qv2, ql2, qi2, st)) {
                 st.nonconverged++;
                 out.cape = T(0.0); out.cin = T(0.0); out.mulvl = mulvl; out.zout = zout;
                 return;
             }
+            log(p1, t1, th1, qv1, p2, qt, th2);
             if (PSEUDO) {
                 qt = qv2;
                 ql2 = T(0.0);
                 qi2 = T(0.0);
             }
         }
-
-        // buoyancy and the CAPE/CIN trapezoids (:501-547)
-        const T thv2 = F::div(th2 * (T(1.0) + K::reps * qv2), T(1.0) + qv2 + ql2 + qi2);
-        b2 = F::div(K::g * (thv2 - e.thv), e.thv);
-        const T dz = layer_dz(e.thv, thv_prev, e.pi, pi_prev);
-        T parea;
-        if (b2 >= T(0.0) && b1 < T(0.0)) {
-            const T frac = F::div(b2, b2 - b1);
-            parea = T(0.5) * b2 * dz * frac;
-            narea = narea - T(0.5) * b1 * dz * (T(1.0) - frac);
-            cin = cin + narea;
-            narea = T(0.0);
-        } else if (b2 < T(0.0) && b1 > T(0.0)) {
-            const T frac = F::div(b1, b1 - b2);
-            parea = T(0.5) * b1 * dz * frac;
-            narea = -T(0.5) * b2 * dz * (T(1.0) - frac);
-        } else if (b2 < T(0.0)) {
-            parea = T(0.0);
-            narea = narea - T(0.5) * dz * (b1 + b2);
-        } else {
-            parea = T(0.5) * dz * (b1 + b2);
-            narea = T(0.0);
-        }
-        cape = cape + fmx(T(0.0), parea);
+        T dz;
+        layer_update<T, MATH>(th2, qv2, ql2, qi2, e, thv_prev, pi_prev, b1, b2, dz, cape, cin,
+                              narea);
         if (e.p <= T(10000.0) && b2 < T(0.0)) doit = false;
         zk = zk + dz;
         zout = zk;
         p_prev = e.p; pi_prev = e.pi; thv_prev = e.thv;
+    }
+    out.cape = cape; out.cin = cin; out.mulvl = mulvl; out.zout = zout;
+}
+
+// ---------------------------------------------------------------------------
+// Warm-started driver.
+//
+// The fixed point of each sub-step is the reference's (same evaluation, same
+// relaxation 0.3, same 2e-4 K test, same 100-iteration limit); only the first
+// iterate differs.  It is predicted from the sub-steps already converged in
+// this column: the rate r = (x* - th1) / ln(p2/p1) of the last two sub-steps
+// extrapolated linearly in ln p to the midpoint of the new sub-step (second
+// order; first order with one sub-step of history).  x* is the secant
+// (Aitken) estimate of the sub-step's exact fixed point,
+// x* = x + (f(x) - x) / (1 - f'), with f' from the last two iterates: the
+// converged th2 itself is only within ~1e-4 K of the fixed point, and
+// differencing that noise into rates would make the guesses no better than
+// a few 1e-4 K.  x* only feeds the predictor; the state carried to the next
+// sub-step is the reference iteration's th2.  With no history (the
+// ascent's first sub-step) the caller's guess is used -- the learned surrogate
+// when one is loaded, else the cold start.  Fallback: when the first
+// iteration's residual |f(g) - g| exceeds `fallback_tol` (1e-3 K) the guess
+// is discarded and the sub-step restarts from the reference's cold start
+// th1; a failed second-order guess also restarts the history at first order.
+//
+// Loop shape: with one or two iterations per sub-step the wavefront would
+// spend most of its time waiting for the lane with the most iterations if
+// sub-steps were lock-stepped.  Here every trip does exactly one iteration
+// for every running lane, and each lane moves to its next sub-step or level
+// on its own (the set-up blocks execute under the mask of the lanes that need
+// them).  The wave runs until its longest column is done.
+// `pc` comes from parcel_source (the caller may have used it to build the
+// first sub-step's surrogate inputs).
+template <typename T, int MATH, bool ICE, bool PSEUDO, class C>
+XC_HD void cape_column_warm(const C &col, int nk, T pinc, const Parcel<T> &pc,
+                                 T first_guess, T fallback_tol, ColumnOut<T> &out,
+                                 LaneStats &st)
+{
+    using F = Fn<T, MATH>;
+    using K = Phys<T>;
+    const int mulvl = pc.mulvl;
+    T zout = T(-999999.0);
+    T th2 = pc.th2, pi2 = pc.pi2, p2 = pc.p2, t2 = pc.t2, qv2 = pc.qv2, b2 = pc.b2, zk = pc.zk;
+    T ql2 = T(0.0), qi2 = T(0.0), qt = qv2;
+    T cape = T(0.0), cin = T(0.0), narea = T(0.0);
+    T p_prev = pc.e.p, pi_prev = pc.e.pi, thv_prev = pc.e.thv;
+    int k = pc.kmax;
+    if (!(k < nk - 1)) {
+        out.cape = cape; out.cin = cin; out.mulvl = mulvl; out.zout = zout;
+        return;
+    }
+    // level state
+    Env<T> e;
+    T b1 = T(0.0), dp = T(0.0);
+    int nloop = 0, n = 0;
+    // sub-step state
+    T p1 = T(0.0), t1 = T(0.0), th1 = T(0.0), qv1 = T(0.0), ql1 = T(0.0), qi1 = T(0.0);
+    T logp = T(0.0), thlast = T(0.0);
+    int it = 0;
+    bool warm = false, fell_back = false;
+    // previous iterate (for the secant slope) and the last slope estimate
+    T x_prev = T(0.0), f_prev = T(0.0), fpr = T(0.0);
+    // extrapolation history: last two rates and their ln-p steps
+    T r1 = T(0.0), r2 = T(0.0), dl1 = T(0.0), dl2 = T(0.0);
+    int nh = 0;
+    bool new_level = true, new_sub = true;
+    for (;;) {
+        if (new_level) {
+            k = k + 1;
+            e = env_at<T, MATH>(col, k);
+            b1 = b2;
+            dp = p_prev - e.p;
+            nloop = substep_count<T, MATH>(dp, pinc);
+            n = 0;
+            st.levels++;
+            new_level = false;
+            new_sub = nloop >= 1;
+        }
+        if (new_sub) {
+            n = n + 1;
+            p1 = p2; t1 = t2; th1 = th2; qv1 = qv2; ql1 = ql2; qi1 = qi2;
+            p2 = p2 - dp;
+            pi2 = F::pow(p2 * K::rp00, K::rddcp);
+            logp = F::log(F::div(p2, p1));
+            T g = th1;
+            if (nh >= 2) {
+                const T r = r1 + (r1 - r2) * F::div(logp + dl1, dl1 + dl2);
+                g = th1 + r * logp;
+            } else if (nh == 1) {
+                g = th1 + r1 * logp;
+            } else if (first_guess == first_guess) {
+                g = first_guess;
+                first_guess = T(NAN);
+            }
+            // a guess that is not finite or absurdly far is no guess
+            const T dg = g - th1;
+            if (!((dg < T(0.0) ? -dg : dg) < T(50.0))) g = th1;
+            thlast = g;
+            warm = g != th1;
+            fell_back = false;
+            it = 0;
+            st.substeps++;
+            new_sub = false;
+        }
+        if (n > nloop) {
+            // zero sub-steps (NaN/overflow in the step count): the reference's
+            // loop body does not run; go straight to the buoyancy update
+        } else {
+            // one fixed-point iteration (:436-479)
+            it = it + 1;
+            fp_eval<T, MATH, ICE>(p2, pi2, logp, t1, th1, qv1, ql1, qi1, qt, thlast, th2, t2, qv2,
+                                  ql2, qi2);
+            st.iterations++;
+            if (it >= 2) {
+                const T dx = thlast - x_prev;
+                if (dx != T(0.0)) {
+                    const T sl = F::div(th2 - f_prev, dx);
+                    fpr = sl < T(-10.0) ? T(-10.0) : (sl > T(0.9) ? T(0.9) : sl);
+                }
+            }
+            x_prev = thlast;
+            f_prev = th2;
+            const T d = th2 - thlast;
+            const T ad = d < T(0.0) ? -d : d;
+            if (it > 100) {
+                st.nonconverged++;
+                out.cape = T(0.0); out.cin = T(0.0); out.mulvl = mulvl; out.zout = zout;
+                return;
+            }
+            if (warm && it == 1 && ad > fallback_tol) {
+                // fallback: the reference's cold start, full iteration budget
+                thlast = th1;
+                warm = false;
+                fell_back = true;
+                it = 0;
+                st.fallbacks++;
+                continue;
+            }
+            if (ad > K::converge) {
+                thlast = thlast + T(0.3) * (th2 - thlast);
+                continue;
+            }
+            // converged: record the rate, finish the sub-step (:485-497)
+            if (logp != T(0.0)) {
+                const T xs = thlast + F::div(th2 - thlast, T(1.0) - fpr);
+                r2 = r1; dl2 = dl1;
+                r1 = F::div(xs - th1, logp);
+                dl1 = logp;
+                // a second-order guess that failed marks a change of regime
+                // (saturation, ice onset): the older rate belongs to the other
+                // side, so restart at first order; otherwise keep building
+                nh = fell_back && nh >= 2 ? 1 : (nh < 2 ? nh + 1 : 2);
+            } else {
+                nh = 0;
+            }
+            if (PSEUDO) {
+                qt = qv2;
+                ql2 = T(0.0);
+                qi2 = T(0.0);
+            }
+            if (n < nloop) {
+                new_sub = true;
+                continue;
+            }
+        }
+        // level complete (:501-558)
+        T dz;
+        layer_update<T, MATH>(th2, qv2, ql2, qi2, e, thv_prev, pi_prev, b1, b2, dz, cape, cin,
+                              narea);
+        zk = zk + dz;
+        zout = zk;
+        p_prev = e.p; pi_prev = e.pi; thv_prev = e.thv;
+        if ((e.p <= T(10000.0) && b2 < T(0.0)) || !(k < nk - 1)) break;
+        new_level = true;
     }
     out.cape = cape; out.cin = cin; out.mulvl = mulvl; out.zout = zout;
 }

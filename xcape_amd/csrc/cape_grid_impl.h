@@ -9,6 +9,7 @@
 #pragma once
 #include "cape_column.h"
 #include "cape_launch.h"
+#include "mlp_device.h"
 #include "srh_column.h"
 
 namespace xcape {
@@ -43,20 +44,21 @@ __device__ __forceinline__ int start_level(const float *p1d, int nk_in, float ps
 }
 
 __device__ __forceinline__ void block_stats_add(const LaneStats &st, bool ran,
-                                                unsigned long long *stats, int nonconv)
+                                                unsigned long long *stats)
 {
-    __shared__ unsigned long long red[5][16];
-    unsigned long long v[5] = {(unsigned long long)ran, st.levels, st.substeps, st.iterations,
-                               (unsigned long long)nonconv};
+    constexpr int NC = 6;
+    __shared__ unsigned long long red[NC][16];
+    unsigned long long v[NC] = {(unsigned long long)ran, st.levels, st.substeps, st.iterations,
+                                st.nonconverged, st.fallbacks};
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
-    for (int c = 0; c < 5; ++c) {
+    for (int c = 0; c < NC; ++c) {
         unsigned long long x = v[c];
         for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
         if (lane == 0) red[c][wid] = x;
     }
     __syncthreads();
-    if (threadIdx.x < 5) {
+    if (threadIdx.x < NC) {
         unsigned long long s = 0;
         for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += red[threadIdx.x][w];
         if (s) atomicAdd(&stats[threadIdx.x], s);
@@ -65,17 +67,23 @@ __device__ __forceinline__ void block_stats_add(const LaneStats &st, bool ran,
 
 // KIND: GRID_CAPE, GRID_SRH or GRID_CAPE_SRH (both from one read of the
 // column: the helicity chain runs after the ascent on the same lane).
-template <typename T, bool ICE, bool PSEUDO, int KIND>
+// WARM: warm-started ascent; with a surrogate loaded (a.surrogate), the first
+// sub-step's iterate of every lane comes from one wave-wide MLP evaluation
+// (MFMA), done at a point every lane of the block reaches.
+template <typename T, bool ICE, bool PSEUDO, int KIND, bool WARM>
 __global__ void __launch_bounds__(256) cape_grid_kernel(CapeGridArgs a)
 {
+    extern __shared__ uint32_t dyn_lds[];
     const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     LaneStats st;
     bool ran = false;
-    if (col < a.ncol) {
-        SigmaCol c;
-        int nk;
+    const bool in_grid = col < a.ncol;
+    SigmaCol c{};
+    int nk = 0;
+    int64_t off = 0;
+    if (in_grid) {
         const float ps = a.ps[col];
-        int64_t off = col * a.nk_in;
+        off = col * a.nk_in;
         if (a.p1d) {
             int s1 = a.start ? a.start[col] : start_level(a.p1d, (int)a.nk_in, ps);
             // a start level outside 1..nk_in cannot come from the rule above;
@@ -94,10 +102,58 @@ __global__ void __launch_bounds__(256) cape_grid_kernel(CapeGridArgs a)
         c.ps = ps;
         c.ts = a.ts[col];
         c.tds = a.tds[col];
-        if (KIND & GRID_CAPE) {
-            T *cape = static_cast<T *>(a.cape), *cin = static_cast<T *>(a.cin),
-              *zout = static_cast<T *>(a.zout);
-            if (c.ts > 0.0f) {
+    }
+    if constexpr ((KIND & GRID_CAPE) != 0) {
+        T *cape = static_cast<T *>(a.cape), *cin = static_cast<T *>(a.cin),
+          *zout = static_cast<T *>(a.zout);
+        const bool lane_cape = in_grid && c.ts > 0.0f;
+        if constexpr (WARM) {
+            using F = Fn<T, XC_MATH>;
+            using K = Phys<T>;
+            Parcel<T> pc;
+            bool ok = false;
+            T guess = T(NAN), lp1 = T(0.0);
+            float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
+            if (lane_cape) {
+                ok = parcel_source<T, XC_MATH>(c, nk, a.source, T(a.ml_depth), pc);
+                if (ok && a.surrogate && pc.kmax < nk - 1) {
+                    // the first sub-step: pc.p2 -> pc.p2 - dp (:408-429)
+                    const Env<T> e1 = env_at<T, XC_MATH>(c, pc.kmax + 1);
+                    T dp = pc.e.p - e1.p;
+                    const int nl = substep_count<T, XC_MATH>(dp, T(a.pinc));
+                    const T p2 = pc.p2 - dp;
+                    if (nl >= 1 && p2 > T(0.0)) {
+                        lp1 = F::log(F::div(p2, pc.p2));
+                        f0 = (float)F::log(p2);
+                        f1 = (float)thetae<T, XC_MATH>(pc.p2, pc.t2, pc.t2, pc.qv2);
+                        f2 = (float)pc.qv2;
+                    }
+                }
+            }
+            if (a.surrogate) {
+                // uniform across the block: every lane takes part
+                const float r = surrogate_eval_wave(
+                    static_cast<const SurrogateDev *>(a.surrogate), f0, f1, f2,
+                    dyn_lds + (threadIdx.x >> 6) * MLP_LDS_WORDS_PER_WAVE);
+                if (lp1 != T(0.0)) guess = pc.th2 + T(r) * lp1;
+            }
+            (void)K::g;
+            if (lane_cape) {
+                ran = true;
+                ColumnOut<T> o;
+                if (ok) {
+                    cape_column_warm<T, XC_MATH, ICE, PSEUDO>(c, nk, T(a.pinc), pc, guess,
+                                                              T(a.fallback_tol), o, st);
+                } else {
+                    o.cape = T(0); o.cin = T(0); o.mulvl = pc.mulvl; o.zout = T(-999999.0);
+                }
+                cape[col] = o.cape;
+                cin[col] = o.cin;
+                a.mulvl[col] = o.mulvl;
+                zout[col] = o.zout;
+            }
+        } else {
+            if (lane_cape) {
                 ran = true;
                 ColumnOut<T> o;
                 cape_column<T, XC_MATH, ICE, PSEUDO>(c, nk, T(a.pinc), a.source, T(a.ml_depth),
@@ -106,14 +162,17 @@ __global__ void __launch_bounds__(256) cape_grid_kernel(CapeGridArgs a)
                 cin[col] = o.cin;
                 a.mulvl[col] = o.mulvl;
                 zout[col] = o.zout;
-            } else {
-                cape[col] = T(0);
-                cin[col] = T(0);
-                zout[col] = T(0);
-                a.mulvl[col] = 0;
             }
         }
-        if (KIND & GRID_SRH) {
+        if (in_grid && !lane_cape) {
+            cape[col] = T(0);
+            cin[col] = T(0);
+            zout[col] = T(0);
+            a.mulvl[col] = 0;
+        }
+    }
+    if constexpr ((KIND & GRID_SRH) != 0) {
+        if (in_grid) {
             c.u = a.u + off;
             c.v = a.v + off;
             c.us = a.us[col];
@@ -132,22 +191,25 @@ __global__ void __launch_bounds__(256) cape_grid_kernel(CapeGridArgs a)
             }
         }
     }
-    if ((KIND & GRID_CAPE) && a.stats) block_stats_add(st, ran, a.stats, (int)st.nonconverged);
+    if ((KIND & GRID_CAPE) && a.stats) block_stats_add(st, ran, a.stats);
 }
 
-template <typename T, int KIND>
+template <typename T, int KIND, bool WARM>
 hipError_t launch_kind(const CapeGridArgs &a, int block, hipStream_t s)
 {
     dim3 grid((unsigned)((a.ncol + block - 1) / block));
+    const size_t lds = (WARM && a.surrogate)
+                           ? sizeof(uint32_t) * MLP_LDS_WORDS_PER_WAVE * (size_t)(block / 64)
+                           : 0;
     if (KIND == GRID_SRH) {
-        cape_grid_kernel<T, false, true, GRID_SRH><<<grid, block, 0, s>>>(a);
+        cape_grid_kernel<T, false, true, GRID_SRH, false><<<grid, block, 0, s>>>(a);
         return hipGetLastError();
     }
     switch (a.adiabat) {
-    case 1: cape_grid_kernel<T, false, true, KIND><<<grid, block, 0, s>>>(a); break;
-    case 2: cape_grid_kernel<T, false, false, KIND><<<grid, block, 0, s>>>(a); break;
-    case 3: cape_grid_kernel<T, true, true, KIND><<<grid, block, 0, s>>>(a); break;
-    case 4: cape_grid_kernel<T, true, false, KIND><<<grid, block, 0, s>>>(a); break;
+    case 1: cape_grid_kernel<T, false, true, KIND, WARM><<<grid, block, lds, s>>>(a); break;
+    case 2: cape_grid_kernel<T, false, false, KIND, WARM><<<grid, block, lds, s>>>(a); break;
+    case 3: cape_grid_kernel<T, true, true, KIND, WARM><<<grid, block, lds, s>>>(a); break;
+    case 4: cape_grid_kernel<T, true, false, KIND, WARM><<<grid, block, lds, s>>>(a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -158,14 +220,20 @@ hipError_t launch_t(const CapeGridArgs &a, int kind, int block, hipStream_t s)
 {
     if (a.ncol <= 0) return hipSuccess;
     if constexpr (sizeof(T) == 8) {
-        // the FP64 yardstick covers CAPE only
-        return kind == GRID_CAPE ? launch_kind<T, GRID_CAPE>(a, block, s) : hipErrorInvalidValue;
-    }
-    switch (kind) {
-    case GRID_CAPE: return launch_kind<T, GRID_CAPE>(a, block, s);
-    case GRID_SRH: return launch_kind<T, GRID_SRH>(a, block, s);
-    case GRID_CAPE_SRH: return launch_kind<T, GRID_CAPE_SRH>(a, block, s);
-    default: return hipErrorInvalidValue;
+        // the FP64 yardstick covers CAPE with the reference's cold start only
+        if (kind != GRID_CAPE || a.warm) return hipErrorInvalidValue;
+        return launch_kind<T, GRID_CAPE, false>(a, block, s);
+    } else {
+        switch (kind) {
+        case GRID_CAPE:
+            return a.warm ? launch_kind<T, GRID_CAPE, true>(a, block, s)
+                          : launch_kind<T, GRID_CAPE, false>(a, block, s);
+        case GRID_SRH: return launch_kind<T, GRID_SRH, false>(a, block, s);
+        case GRID_CAPE_SRH:
+            return a.warm ? launch_kind<T, GRID_CAPE_SRH, true>(a, block, s)
+                          : launch_kind<T, GRID_CAPE_SRH, false>(a, block, s);
+        default: return hipErrorInvalidValue;
+        }
     }
 }
 

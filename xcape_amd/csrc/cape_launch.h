@@ -28,6 +28,10 @@ struct CapeGridArgs {
     double depth;
     double *srh_rm, *srh_lm;     // (ncol,)
     float *rm, *lm, *mean6;      // (ncol, 2) u, v pairs, or nullptr
+    // warm start of the moist-adiabat iteration
+    int warm;                    // 0: reference cold start
+    float fallback_tol;          // first-iteration residual that rejects a guess [K]
+    const void *surrogate;       // device MLP weights (or nullptr)
 };
 
 // kind of work a grid launch does

@@ -149,8 +149,6 @@ int grid_entry(const GridCall &c)
     }
     if (c.fp64 && (c.flags & (XCAPE_FAST_MATH | XCAPE_WARM_START)))
         return fail(XCAPE_EINVAL, "the FP64 entry point supports neither FAST_MATH nor WARM_START");
-    if (c.flags & XCAPE_WARM_START)
-        return fail(XCAPE_ESTATE, "warm start requested but no surrogate is loaded");
     if (!(c.flags & XCAPE_DEVICE_PTRS) && c.p_shared && c.start) {
         for (int64_t i = 0; i < c.ncol; ++i)
             if (c.start[i] < 1 || c.start[i] > c.nk)
@@ -183,6 +181,9 @@ int grid_entry(const GridCall &c)
     a.source = c.source;
     a.adiabat = c.adiabat;
     a.depth = c.depth;
+    a.warm = (c.flags & XCAPE_WARM_START) ? 1 : 0;
+    a.fallback_tol = 1.0e-3f;
+    a.surrogate = (a.warm && !(c.flags & XCAPE_NO_SURROGATE)) ? surrogate_device() : nullptr;
     hipError_t e = hipSuccess;
     if (dev) {
         a.p = c.p_shared ? nullptr : c.p;
@@ -232,12 +233,15 @@ int grid_entry(const GridCall &c)
         XC_TRY(zero_counters(dstats_.ptr, ST_COUNT, s));
         a.stats = dstats_.ptr;
     }
+    // warm launches use one-wave blocks: the flattened ascent has no
+    // cross-wave work, and the surrogate's LDS stays per wave
+    const int block = a.warm ? 64 : 256;
     if (c.fp64)
-        e = launch_cape_grid_accurate(a, true, c.kind, 256, s);
+        e = launch_cape_grid_accurate(a, true, c.kind, block, s);
     else if (c.flags & XCAPE_FAST_MATH)
-        e = launch_cape_grid_fast(a, c.kind, 256, s);
+        e = launch_cape_grid_fast(a, c.kind, block, s);
     else
-        e = launch_cape_grid_accurate(a, false, c.kind, 256, s);
+        e = launch_cape_grid_accurate(a, false, c.kind, block, s);
     if (e != hipSuccess) return hip_fail(e, "kernel launch");
     if (!dev) {
         if (want_cape) {

@@ -14,6 +14,7 @@ int ensure_device();
 hipStream_t thread_stream();
 hipError_t zero_counters(unsigned long long *p, int n, hipStream_t s);
 void copy_stats(const unsigned long long *h, xcape_stats *stats);
+const void *surrogate_device();   // loaded surrogate (device), or nullptr
 
 // Stream-ordered device allocation released on scope exit (hipFreeAsync on
 // the same stream, so it never races the work that uses it).

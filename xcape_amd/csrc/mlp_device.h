@@ -1,0 +1,158 @@
+// The warm-start surrogate: a 3 -> 64 -> 64 -> 1 multilayer perceptron with
+// tanh-form GELU activations.
+//
+// Inputs (normalised with the stored mean/scale): ln p2, the parcel's
+// equivalent potential temperature (the reference's own parcel theta-e,
+// getthe(p, t, t, qv), CAPE_CODE_model_lev.f90:505), and its total water
+// mixing ratio.  Output: the sub-step's potential-temperature rate
+// r = (theta2 - theta1) / ln(p2/p1), de-normalised with the stored scale and
+// offset, from which the first iterate is theta1 + r ln(p2/p1).
+//
+// Precision: weights are BF16 (W1, w3 as BF16-rounded floats on the VALU,
+// W2 as BF16 MFMA operands); the first hidden layer is rounded to BF16 where
+// it feeds the MFMA; all accumulation is FP32.
+//
+// Parameter vector (FP32 master copy, length MLP_NPARAMS): W1 (64x3), b1,
+// W2 (64x64, W2[i][j] maps hidden-1 unit j to hidden-2 unit i), b2, w3, b3.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace xcape {
+
+constexpr int MLP_H = 64;
+constexpr int MLP_IN = 3;
+constexpr int MLP_OFF_W1 = 0;
+constexpr int MLP_OFF_B1 = MLP_OFF_W1 + MLP_H * MLP_IN;
+constexpr int MLP_OFF_W2 = MLP_OFF_B1 + MLP_H;
+constexpr int MLP_OFF_B2 = MLP_OFF_W2 + MLP_H * MLP_H;
+constexpr int MLP_OFF_W3 = MLP_OFF_B2 + MLP_H;
+constexpr int MLP_OFF_B3 = MLP_OFF_W3 + MLP_H;
+constexpr int MLP_NPARAMS = MLP_OFF_B3 + 1;   // 4481
+
+// Device copy the ascent kernels read (built from the FP32 master vector).
+struct SurrogateDev {
+    uint16_t w2[MLP_H * MLP_H];   // BF16 bits, row-major [i][j]
+    float w1[MLP_H * MLP_IN];     // BF16-rounded
+    float b1[MLP_H];
+    float b2[MLP_H];
+    float w3[MLP_H];              // BF16-rounded
+    float b3;
+    float in_mean[MLP_IN], in_scale[MLP_IN];
+    float out_scale, out_offset;
+};
+
+// round to nearest even to BF16 (bits / value); NaN stays NaN
+__host__ __device__ __forceinline__ uint16_t bf16_bits(float f)
+{
+    uint32_t u;
+    __builtin_memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__host__ __device__ __forceinline__ float bf16_round(float f)
+{
+    uint32_t u = (uint32_t)bf16_bits(f) << 16;
+    float r;
+    __builtin_memcpy(&r, &u, 4);
+    return r;
+}
+
+// tanh-form GELU and its derivative, FP32: 0.5 z (1 + tanh(c (z + a z^3)))
+__device__ __forceinline__ float gelu_f(float z)
+{
+    const float c = 0.7978845608028654f, a = 0.044715f;
+    const float u = c * (z + a * z * z * z);
+    // tanh(u) = 1 - 2 / (1 + exp(2u)), saturating cleanly for large |u|
+    const float e = __expf(2.0f * u);
+    const float t = 1.0f - 2.0f / (1.0f + e);
+    return 0.5f * z * (1.0f + t);
+}
+__device__ __forceinline__ float gelu_grad_f(float z)
+{
+    const float c = 0.7978845608028654f, a = 0.044715f;
+    const float u = c * (z + a * z * z * z);
+    const float e = __expf(2.0f * u);
+    const float t = 1.0f - 2.0f / (1.0f + e);
+    return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * c * (1.0f + 3.0f * a * z * z);
+}
+
+typedef __bf16 xc_bf16x8 __attribute__((ext_vector_type(8)));
+typedef float xc_f32x16 __attribute__((ext_vector_type(16)));
+
+// LDS the wave needs for one evaluation: 64 samples x 64 hidden-1 values as
+// BF16, rows padded to 33 words so that 64 lanes writing their own rows hit
+// distinct banks.
+constexpr int MLP_LDS_ROW_WORDS = 33;
+constexpr int MLP_LDS_WORDS_PER_WAVE = 64 * MLP_LDS_ROW_WORDS;
+
+// Evaluate the network for the 64 samples of a wavefront (one per lane).
+// Must be called by all 64 lanes of the wave together (the MFMA and the LDS
+// exchange use every lane); `lds` points to this wave's
+// MLP_LDS_WORDS_PER_WAVE words.  `sync` is a block-wide barrier the caller
+// provides (every wave of the block calls this function once).
+//
+// Layer 2 runs as v_mfma_f32_32x32x16_bf16: Z2^T (hidden-2 x samples) =
+// W2 (hidden-2 x hidden-1) . H1^T (hidden-1 x samples), four 32x32 output
+// tiles, each accumulated over 4 K-steps of 16.  Operand layout per lane l:
+// A[m = l%32][k = 8(l/32) .. +7], B[k = 8(l/32) .. +7][n = l%32]; result
+// register r of lane l is C[8(r/4) + 4(l/32) + r%4][l%32].
+__device__ __forceinline__ float surrogate_eval_wave(const SurrogateDev *S, float x0, float x1,
+                                                     float x2, uint32_t *lds)
+{
+    const int lane = threadIdx.x & 63;
+    // ---- layer 1 on the VALU: this lane's sample --------------------------
+    const float u0 = (x0 - S->in_mean[0]) * S->in_scale[0];
+    const float u1 = (x1 - S->in_mean[1]) * S->in_scale[1];
+    const float u2 = (x2 - S->in_mean[2]) * S->in_scale[2];
+    uint32_t *row = lds + lane * MLP_LDS_ROW_WORDS;
+#pragma unroll 4
+    for (int j = 0; j < MLP_H; j += 2) {
+        const float za = S->b1[j] + S->w1[3 * j] * u0 + S->w1[3 * j + 1] * u1 +
+                         S->w1[3 * j + 2] * u2;
+        const float zb = S->b1[j + 1] + S->w1[3 * j + 3] * u0 + S->w1[3 * j + 4] * u1 +
+                         S->w1[3 * j + 5] * u2;
+        row[j >> 1] = (uint32_t)bf16_bits(gelu_f(za)) | ((uint32_t)bf16_bits(gelu_f(zb)) << 16);
+    }
+    __syncthreads();
+    // ---- layer 2 (MFMA) + layer 3 ------------------------------------------
+    const int half = lane >> 5, col = lane & 31;
+    float out[2] = {0.0f, 0.0f};
+#pragma unroll
+    for (int tn = 0; tn < 2; ++tn) {
+        float part = 0.0f;
+#pragma unroll
+        for (int tm = 0; tm < 2; ++tm) {
+            xc_f32x16 acc = {};
+#pragma unroll
+            for (int kb = 0; kb < 4; ++kb) {
+                const int k0 = 16 * kb + 8 * half;
+                // A: W2[32 tm + col][k0 .. k0+7]
+                const uint16_t *wa = S->w2 + (32 * tm + col) * MLP_H + k0;
+                xc_bf16x8 a, b;
+                // B: H1[sample 32 tn + col][k0 .. k0+7] from LDS (4 words)
+                const uint32_t *hb = lds + (32 * tn + col) * MLP_LDS_ROW_WORDS + (k0 >> 1);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    uint32_t wq = (uint32_t)wa[2 * q] | ((uint32_t)wa[2 * q + 1] << 16);
+                    uint32_t hq = hb[q];
+                    __builtin_memcpy((char *)&a + 4 * q, &wq, 4);
+                    __builtin_memcpy((char *)&b + 4 * q, &hq, 4);
+                }
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int i = 32 * tm + 8 * (r >> 2) + 4 * half + (r & 3);
+                part += S->w3[i] * gelu_f(acc[r] + S->b2[i]);
+            }
+        }
+        out[tn] = part + __shfl_xor(part, 32, 64);
+    }
+    __syncthreads();   // the LDS rows may be reused after this call
+    const float y = (half ? out[1] : out[0]) + S->b3;
+    return y * S->out_scale + S->out_offset;
+}
+
+}  // namespace xcape

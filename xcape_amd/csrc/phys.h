@@ -16,6 +16,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// Functions usable from host code too (the accurate path only needs the C
+// math library), so the column routine can be exercised on the CPU.
+#define XC_HD __host__ __device__
+
 namespace xcape {
 
 enum { MATH_ACCURATE = 0, MATH_FAST = 1 };
@@ -58,12 +62,12 @@ template <> struct Phys<double> {
 template <typename T, int MATH> struct Fn;
 
 template <> struct Fn<float, MATH_ACCURATE> {
-    static __device__ __forceinline__ float exp(float x) { return expf(x); }
-    static __device__ __forceinline__ float log(float x) { return logf(x); }
-    static __device__ __forceinline__ float pow(float x, float y) { return powf(x, y); }
-    static __device__ __forceinline__ float div(float a, float b) { return a / b; }
+    static XC_HD __forceinline__ float exp(float x) { return expf(x); }
+    static XC_HD __forceinline__ float log(float x) { return logf(x); }
+    static XC_HD __forceinline__ float pow(float x, float y) { return powf(x, y); }
+    static XC_HD __forceinline__ float div(float a, float b) { return a / b; }
     // exp(c * a / b) with c*a formed first, as the reference does
-    static __device__ __forceinline__ float exp_ratio(float c, float a, float b)
+    static XC_HD __forceinline__ float exp_ratio(float c, float a, float b)
     {
         return expf(c * a / b);
     }
@@ -101,11 +105,11 @@ template <> struct Fn<float, MATH_FAST> {
 };
 
 template <int MATH> struct Fn<double, MATH> {
-    static __device__ __forceinline__ double exp(double x) { return ::exp(x); }
-    static __device__ __forceinline__ double log(double x) { return ::log(x); }
-    static __device__ __forceinline__ double pow(double x, double y) { return ::pow(x, y); }
-    static __device__ __forceinline__ double div(double a, double b) { return a / b; }
-    static __device__ __forceinline__ double exp_ratio(double c, double a, double b)
+    static XC_HD __forceinline__ double exp(double x) { return ::exp(x); }
+    static XC_HD __forceinline__ double log(double x) { return ::log(x); }
+    static XC_HD __forceinline__ double pow(double x, double y) { return ::pow(x, y); }
+    static XC_HD __forceinline__ double div(double a, double b) { return a / b; }
+    static XC_HD __forceinline__ double exp_ratio(double c, double a, double b)
     {
         return ::exp(c * a / b);
     }
@@ -113,13 +117,13 @@ template <int MATH> struct Fn<double, MATH> {
 
 // Fortran MAX/MIN as the reference evaluates them: the second operand wins
 // when the comparison is false (NaN included).
-template <typename T> __device__ __forceinline__ T fmx(T a, T b) { return a > b ? a : b; }
-template <typename T> __device__ __forceinline__ T fmn(T a, T b) { return a < b ? a : b; }
+template <typename T> XC_HD __forceinline__ T fmx(T a, T b) { return a > b ? a : b; }
+template <typename T> XC_HD __forceinline__ T fmn(T a, T b) { return a < b ? a : b; }
 
 // int(x) for the sub-step count: NaN/out-of-range map to INT32_MIN as the
 // reference build's truncating conversion does (the sub-step loop then runs
 // zero times).
-template <typename T> __device__ __forceinline__ int ftrunc(T x)
+template <typename T> XC_HD __forceinline__ int ftrunc(T x)
 {
     if (!(x > T(-2147483648.0) && x < T(2147483648.0))) return INT32_MIN;
     return (int)x;
@@ -128,7 +132,7 @@ template <typename T> __device__ __forceinline__ int ftrunc(T x)
 // getqvs: Bolton saturation mixing ratio over liquid (CAPE_CODE_model_lev.f90
 // :570-581); es = 611.2 exp(17.67 (t-273.15)/(t-29.65)), r = eps es/(p-es)
 template <typename T, int MATH>
-__device__ __forceinline__ T qvs(T p, T t)
+XC_HD __forceinline__ T qvs(T p, T t)
 {
     using F = Fn<T, MATH>;
     T es = T(611.2) * F::exp_ratio(T(17.67), t - T(273.15), t - T(29.65));
@@ -137,7 +141,7 @@ __device__ __forceinline__ T qvs(T p, T t)
 
 // getqvi: over ice (:587-598)
 template <typename T, int MATH>
-__device__ __forceinline__ T qvi(T p, T t)
+XC_HD __forceinline__ T qvi(T p, T t)
 {
     using F = Fn<T, MATH>;
     T es = T(611.2) * F::exp_ratio(T(21.8745584), t - T(273.15), t - T(7.66));
@@ -146,7 +150,7 @@ __device__ __forceinline__ T qvi(T p, T t)
 
 // getthe: Bolton (1980) theta-e with the LCL temperature (:604-620)
 template <typename T, int MATH>
-__device__ __forceinline__ T thetae(T p, T t, T td, T q)
+XC_HD __forceinline__ T thetae(T p, T t, T td, T q)
 {
     using F = Fn<T, MATH>;
     T tlcl;

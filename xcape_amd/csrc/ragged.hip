@@ -49,7 +49,26 @@ struct RaggedArgs {
     unsigned long long *stats;
     int chunk;                 // columns a wavefront takes per grab (64 or 128)
     int tile;                  // levels staged in LDS (0 = none)
+    int warm;                  // warm-started iteration (see cape_column_warm)
+    float fallback_tol;
 };
+
+template <int MATH, bool ICE, bool PSEUDO, class C>
+__device__ __forceinline__ void run_column(const RaggedArgs &a, const C &c, int nk, int src,
+                                           ColumnOut<float> &o, LaneStats &st)
+{
+    if (a.warm) {
+        Parcel<float> pc;
+        if (parcel_source<float, MATH>(c, nk, src, a.ml_depth, pc))
+            cape_column_warm<float, MATH, ICE, PSEUDO>(c, nk, a.pinc, pc, NAN, a.fallback_tol,
+                                                       o, st);
+        else {
+            o.cape = 0.0f; o.cin = 0.0f; o.mulvl = pc.mulvl; o.zout = -999999.0f;
+        }
+    } else {
+        cape_column<float, MATH, ICE, PSEUDO>(c, nk, a.pinc, src, a.ml_depth, o, st);
+    }
+}
 
 // Column accessor: levels below `tile` from this lane's LDS slice, the rest
 // from global memory.
@@ -138,10 +157,10 @@ __global__ void __launch_bounds__(256) ragged_kernel(RaggedArgs a)
                     const int nk = nk_in + 1;
                     const int src = a.source[col];
                     switch (a.adiabat[col]) {
-                    case 1: cape_column<float, MATH, false, true>(c, nk, a.pinc, src, a.ml_depth, o, st); break;
-                    case 2: cape_column<float, MATH, false, false>(c, nk, a.pinc, src, a.ml_depth, o, st); break;
-                    case 3: cape_column<float, MATH, true, true>(c, nk, a.pinc, src, a.ml_depth, o, st); break;
-                    default: cape_column<float, MATH, true, false>(c, nk, a.pinc, src, a.ml_depth, o, st); break;
+                    case 1: run_column<MATH, false, true>(a, c, nk, src, o, st); break;
+                    case 2: run_column<MATH, false, false>(a, c, nk, src, o, st); break;
+                    case 3: run_column<MATH, true, true>(a, c, nk, src, o, st); break;
+                    default: run_column<MATH, true, false>(a, c, nk, src, o, st); break;
                     }
                     a.cape[col] = o.cape;
                     a.cin[col] = o.cin;
@@ -157,9 +176,10 @@ __global__ void __launch_bounds__(256) ragged_kernel(RaggedArgs a)
         }
     }
     if (a.stats) {
-        unsigned long long v[5] = {ran, st.levels, st.substeps, st.iterations, st.nonconverged};
+        unsigned long long v[6] = {ran, st.levels, st.substeps, st.iterations, st.nonconverged,
+                                   st.fallbacks};
 #pragma unroll
-        for (int c = 0; c < 5; ++c) {
+        for (int c = 0; c < 6; ++c) {
             unsigned long long x = v[c];
             for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
             if (lane == 0 && x) atomicAdd(&a.stats[c], x);
@@ -237,7 +257,7 @@ int xcape_ragged_plan_create(const int64_t *offsets_host, const int32_t *source_
         return fail(XCAPE_EINVAL, "tile=%d outside 0..%d", tile, MAX_TILE);
     if (waves_per_block < 1 || waves_per_block > 4)
         return fail(XCAPE_EINVAL, "waves_per_block=%d outside 1..4", waves_per_block);
-    if (flags & ~(uint32_t)(XCAPE_FAST_MATH | XCAPE_DEVICE_PTRS))
+    if (flags & ~(uint32_t)(XCAPE_FAST_MATH | XCAPE_DEVICE_PTRS | XCAPE_WARM_START))
         return fail(XCAPE_EINVAL, "unsupported flags 0x%x", flags);
     // validate the layout on the host copies and build the work order
     if (offsets_host[0] != 0) return fail(XCAPE_EINVAL, "offsets[0] must be 0");
@@ -294,6 +314,8 @@ int xcape_ragged_plan_create(const int64_t *offsets_host, const int32_t *source_
     a.ncol = ncol; a.pinc = pinc; a.ml_depth = ml_depth;
     a.cape = cape; a.cin = cin; a.zout = zmulev; a.mulvl = mulev;
     a.counter = pl->d_counter; a.stats = nullptr; a.chunk = chunk; a.tile = tile;
+    a.warm = (flags & XCAPE_WARM_START) ? 1 : 0;
+    a.fallback_tol = 1.0e-3f;
     *out = pl;
     return XCAPE_OK;
 }

@@ -63,13 +63,14 @@ class RaggedBatch:
 
 class RaggedPlan:
     def __init__(self, batch, pinc=500., ml_depth=500., chunk=64, tile=16, waves=4,
-                 fast_math=False):
+                 fast_math=False, warm_start=False):
         self.batch = batch
         self.cfg = dict(chunk=chunk, tile=tile, waves=waves)
         b = batch
         P = _hip._ptr
         h = ctypes.c_void_p()
-        flags = _hip.XCAPE_FAST_MATH if fast_math else 0
+        flags = (_hip.XCAPE_FAST_MATH if fast_math else 0) | \
+            (_hip.XCAPE_WARM_START if warm_start else 0)
         rc = _hip.lib().xcape_ragged_plan_create(
             P(b.offsets_host), P(b.source_host), P(b.adiabat_host), b.ncol,
             P(b.offsets), P(b.source), P(b.adiabat), P(b.p), P(b.t), P(b.td), P(b.ps), P(b.ts),
@@ -134,13 +135,13 @@ def _save_cache(c):
 
 
 def autotune(batch, pinc=500., ml_depth=500., fast_math=False, chunks=CHUNKS, tiles=TILES,
-             waves=WAVES_PER_BLOCK, reps=3, use_cache=True, log=None):
+             waves=WAVES_PER_BLOCK, reps=3, use_cache=True, log=None, warm_start=False):
     """Time every (chunk, tile, waves-per-block) plan on `batch` and return
     (best config, {config: ms}).  Results are cached per device, batch
     signature and method settings."""
     import torch
     dev = torch.cuda.get_device_name(0)
-    key = f'{dev}|{batch.signature()}|{pinc}|{ml_depth}|{int(fast_math)}'
+    key = f'{dev}|{batch.signature()}|{pinc}|{ml_depth}|{int(fast_math)}|{int(warm_start)}'
     cache = _load_cache() if use_cache else {}
     if key in cache:
         ent = cache[key]
@@ -150,7 +151,7 @@ def autotune(batch, pinc=500., ml_depth=500., fast_math=False, chunks=CHUNKS, ti
     for ch in chunks:
         for tl in tiles:
             for wv in waves:
-                plan = RaggedPlan(batch, pinc, ml_depth, ch, tl, wv, fast_math)
+                plan = RaggedPlan(batch, pinc, ml_depth, ch, tl, wv, fast_math, warm_start)
                 t0 = time.perf_counter()
                 ms = _time_plan(plan, reps)
                 plan.close()
