@@ -46,6 +46,10 @@ extern "C" {
 
 #define XCAPE_NO_SURROGATE 0x8u /* with XCAPE_WARM_START: ignore a loaded
                                    surrogate (extrapolation only)              */
+#define XCAPE_LEVEL_MAJOR 0x10u /* level-major profiles: the (nk, ncol) arrays
+                                   (p on a sigma grid, t, td, u, v) hold level
+                                   j of column i at [j * ncol + i] instead of
+                                   the default column-contiguous [i * nk + j] */
 
 /* enums as the reference's Python layer maps them (core.py:302-311) */
 #define XCAPE_SOURCE_SURFACE 1
@@ -64,7 +68,9 @@ typedef struct xcape_stats {
     int64_t iterations;    /* moist-adiabat fixed-point iterations              */
     int64_t nonconverged;  /* columns that hit the 100-iteration limit          */
     int64_t fallbacks;     /* warm starts rejected for the cold start           */
-    int64_t reserved[2];
+    int64_t first_substep_iterations; /* iterations of the ascents' first
+                                         sub-steps (where no history exists) */
+    int64_t reserved[1];
 } xcape_stats;
 
 const char *xcape_version(void);
@@ -223,15 +229,17 @@ int xcape_surrogate_loaded(void);
 int xcape_surrogate_eval(const float *x, int64_t n, float *y, void *stream);
 
 /* Training data from the cold-start ascent of a sigma grid (device arrays
- * as xcape_cape_f32): for a hashed `keep_fraction` of the sub-steps, append
- * (ln p2, parcel theta-e, qt, rate) to samples (capacity, 4); *count
+ * as xcape_cape_f32): for a hashed `keep_fraction` of the sub-steps (with
+ * first_only != 0: of each ascent's first sub-step only), append
+ * (ln p2, parcel theta-e, qt, rate, ln(p2/p1)) to samples (capacity, 5); *count
  * (device) is advanced atomically and may exceed capacity (extra records are
  * dropped). */
 int xcape_surrogate_log_f32(const float *p, const float *t, const float *td,
                             const float *ps, const float *ts, const float *tds,
                             int64_t nk, int64_t ncol, float pinc, int source, float ml_depth,
-                            int adiabat, float keep_fraction, uint32_t seed, float *samples,
-                            int64_t capacity, unsigned long long *count, void *stream);
+                            int adiabat, float keep_fraction, uint32_t seed, int first_only,
+                            float *samples, int64_t capacity, unsigned long long *count,
+                            void *stream);
 
 #ifdef __cplusplus
 }

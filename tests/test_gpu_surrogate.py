@@ -1,0 +1,211 @@
+"""The warm-start surrogate on the GPU: training kernels against the CPU
+oracle (oracle/mlp_oracle.py), the ascent kernels' MFMA evaluation against
+the same oracle, the sample logger against the CPU ascent's own sub-step
+records, training, and the warm-started ascent with a surrogate loaded.
+
+Tolerances: the GPU evaluates in FP32 with BF16-rounded weights and BF16
+hidden-1 activations; the oracle applies the same rounding in float64, so
+the remaining differences are FP32 accumulation order and the odd BF16
+rounding flip of a hidden value that lies within FP32 error of a rounding
+boundary (2^-8 relative on one entry): outputs within 2e-3 (1 + |y|),
+gradients within 1e-3 of each parameter block's largest entry.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.join(os.path.dirname(__file__), '..'))
+from oracle import mlp_oracle as mo  # noqa: E402
+from oracle import oracle as orc  # noqa: E402
+from tolerances import cape_ok  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+BLOCKS = {'W1': (0, 192), 'b1': (192, 256), 'W2': (256, 4352), 'b2': (4352, 4416),
+          'w3': (4416, 4480), 'b3': (4480, 4481)}
+
+
+def _theta(seed):
+    rng = np.random.default_rng(seed)
+    W1, b1, W2, b2, w3, b3 = mo.unpack(mo.init(seed))
+    return mo.pack(W1, rng.normal(0, .3, 64), W2, rng.normal(0, .3, 64), w3,
+                   0.1).astype(np.float32)
+
+
+@pytest.fixture(scope='module')
+def torch():
+    import torch
+    return torch
+
+
+def _t(torch, a):
+    return torch.as_tensor(np.ascontiguousarray(a, np.float32), device='cuda')
+
+
+def test_loss_grad_matches_oracle(torch):
+    from xcape_amd import surrogate
+    theta = _theta(1)
+    rng = np.random.default_rng(7)
+    n = 3000
+    x = rng.normal(0, 1, (n, 3)).astype(np.float32)
+    t = rng.normal(0, 1, n).astype(np.float32)
+    loss, g = surrogate.loss_grad(_t(torch, theta), _t(torch, x), _t(torch, t))
+    torch.cuda.synchronize()
+    g = g.cpu().numpy().astype(np.float64)
+    ref_loss, ref_g = mo.loss_and_grad(theta.astype(np.float64), x.astype(np.float64), t,
+                                       bf16_round=True)
+    assert abs(loss.item() - ref_loss) <= 1e-4 * ref_loss
+    for name, (lo, hi) in BLOCKS.items():
+        scale = np.abs(ref_g[lo:hi]).max()
+        err = np.abs(g[lo:hi] - ref_g[lo:hi]).max()
+        print(f'{name}: max|d|={err:.3g} scale={scale:.3g}')
+        assert err <= 1e-3 * scale + 1e-7, name
+
+
+def test_forward_matches_oracle(torch):
+    from xcape_amd import _hip
+    theta = _theta(2)
+    x = np.random.default_rng(3).normal(0, 1.5, (1000, 3)).astype(np.float32)
+    y = torch.empty(1000, dtype=torch.float32, device='cuda')
+    xd, thd = _t(torch, x), _t(torch, theta)   # kept alive across the call
+    _hip.check(_hip.lib().xcape_mlp_forward(_hip._ptr(xd), 1000, _hip._ptr(thd), _hip._ptr(y),
+                                            None))
+    torch.cuda.synchronize()
+    ref = mo.forward(theta.astype(np.float64), x.astype(np.float64), bf16_round=True)[0]
+    np.testing.assert_array_less(np.abs(y.cpu().numpy() - ref), 2e-3 * (1 + np.abs(ref)))
+
+
+@pytest.mark.parametrize('n', [1000, 64, 1])
+def test_in_kernel_mfma_eval_matches_oracle(torch, n):
+    """The ascent kernels' wave-wide evaluation (v_mfma_f32_32x32x16_bf16
+    layer 2) against the oracle, with an identity and a non-trivial
+    normalisation; n not a multiple of the wave or block size."""
+    from xcape_amd import surrogate
+    theta = _theta(4)
+    rng = np.random.default_rng(5)
+    x = rng.normal(0, 1.5, (n, 3)).astype(np.float32)
+    ident = np.array([0, 0, 0, 1, 1, 1, 1, 0], np.float32)
+    surrogate.Surrogate(theta, ident).activate()
+    y = surrogate.eval_in_kernel(_t(torch, x))
+    torch.cuda.synchronize()
+    ref = mo.forward(theta.astype(np.float64), x.astype(np.float64), bf16_round=True)[0]
+    np.testing.assert_array_less(np.abs(y.cpu().numpy() - ref), 2e-3 * (1 + np.abs(ref)))
+    # physical-unit inputs through the stored normalisation
+    norm = np.array([11.0, 330.0, 0.01, 2.0, 0.05, 150.0, 20.0, 30.0], np.float32)
+    xr = (x / norm[3:6] + norm[:3]).astype(np.float32)
+    surrogate.Surrogate(theta, norm).activate()
+    y = surrogate.eval_in_kernel(_t(torch, xr)).cpu().numpy()
+    u = (xr.astype(np.float64) - norm[:3]) * norm[3:6]
+    ref = mo.forward(theta.astype(np.float64), u.astype(np.float32).astype(np.float64),
+                     bf16_round=True)[0] * norm[6] + norm[7]
+    np.testing.assert_array_less(np.abs(y - ref), 2e-3 * norm[6] * (1 + np.abs(ref) / norm[6]))
+
+
+def test_sample_logger_matches_cpu_ascent(torch):
+    """keep_fraction 1, one column per call: the records are the CPU
+    ascent's sub-steps in order (ln p2, theta-e at the sub-step start, qt,
+    rate (th2 - th1) / ln(p2/p1), ln(p2/p1)); with first_only, its first.
+    qt and the rate follow the converged iterate, which may stop at a
+    different point within the 2e-4 K tolerance than the CPU's."""
+    from xcape_amd import _hip, synthetic
+    c = synthetic.sigma_columns(6, 60, seed=11)
+    cap = 4096
+    for adiabat, source in ((1, 1), (3, 2), (1, 3)):
+        for i in range(6):
+            sl = slice(i, i + 1)
+            arr = [c['p'][sl], c['t'][sl], c['td'][sl], c['ps'][sl], c['ts'][sl], c['tds'][sl]]
+            dev = [_t(torch, a) for a in arr]
+
+            def run(first_only):
+                out = torch.zeros((cap, 5), dtype=torch.float32, device='cuda')
+                cnt = torch.zeros(1, dtype=torch.int64, device='cuda')
+                _hip.check(_hip.lib().xcape_surrogate_log_f32(
+                    *[_hip._ptr(d) for d in dev], 60, 1, 500., source, 500., adiabat, 1.0, 0,
+                    first_only, _hip._ptr(out), cap, _hip._ptr(cnt), None))
+                torch.cuda.synchronize()
+                return out[:int(cnt.item())].cpu().numpy().astype(np.float64)
+            got = run(0)
+            first = run(1)
+            rec = orc.cape_log(*arr, pinc=500., source=source, adiabat=adiabat)
+            if len(rec) == 0:
+                assert len(got) == 0 and len(first) == 0
+                continue
+            np.testing.assert_array_equal(first, got[:1])
+            # the first record's p1 is the parcel's start pressure, not in the
+            # oracle's record; compare rates from the second record on
+            assert len(got) == len(rec), (adiabat, source, i)
+            p2 = rec[:, 0].astype(np.float64)
+            np.testing.assert_allclose(got[:, 0], np.log(p2), rtol=1e-6)
+            np.testing.assert_allclose(got[:, 1], rec[:, 1], rtol=2e-6)
+            np.testing.assert_allclose(got[:, 2], rec[:, 2], rtol=2e-4, atol=1e-9)
+            lp = np.log(p2[1:] / p2[:-1])
+            np.testing.assert_allclose(got[1:, 4], lp, rtol=1e-4)
+            rate = (rec[1:, 4].astype(np.float64) - rec[1:, 3]) / lp
+            np.testing.assert_allclose(got[1:, 3], rate, atol=0.05)
+            assert np.median(np.abs(got[1:, 3] - rate)) < 5e-3
+
+
+@pytest.fixture(scope='module')
+def trained(torch):
+    """A surrogate trained on the GPU from the first sub-steps of 200 000
+    logged ascents (validation: 50 000 columns of another seed)."""
+    from xcape_amd import surrogate
+    tr = surrogate.log_samples(200000, seed=1)
+    va = surrogate.log_samples(50000, seed=2)
+    s = surrogate.fit(tr, va, epochs=30, batch=8192, lr=3e-3, seed=0)
+    return s, tr, va
+
+
+def test_training_reduces_validation_loss(trained):
+    """The trained network's first iterates are within the reference's
+    2e-4 K convergence test of the converged theta for most validation
+    sub-steps (so one iteration usually confirms them)."""
+    from xcape_amd import surrogate
+    s, tr, va = trained
+    h = s.meta['history']
+    print('samples', tr.shape[0], va.shape[0], 'history', h[0], h[-1])
+    assert tr.shape[0] > 150000 and va.shape[0] > 35000
+    assert all(np.isfinite(r['val_mse']) for r in h)
+    assert h[-1]['val_mse'] < 0.1 * h[0]['val_mse']
+    err = surrogate.guess_error(s, va).cpu().numpy()
+    print('guess error [K]: median', np.median(err), 'p90', np.quantile(err, 0.9))
+    assert np.median(err) < 2e-4 and np.quantile(err, 0.9) < 1e-3
+
+
+def test_in_kernel_eval_of_trained_network_matches_training_forward(torch, trained):
+    s, _, va = trained
+    s.activate()
+    x = va[:5000, :3].contiguous()
+    from xcape_amd import surrogate
+    y_k = surrogate.eval_in_kernel(x).cpu().numpy()
+    y_f = s.predict(x).cpu().numpy()
+    scale = float(s.norm[6])
+    np.testing.assert_array_less(np.abs(y_k - y_f), 2e-3 * scale * (1 + np.abs(y_f) / scale))
+
+
+def test_warm_start_with_surrogate_parity_and_gain(torch, trained):
+    """With the trained surrogate loaded the warm-started ascent stays within
+    the CAPE/CIN tolerance of the CPU ascent, and the first sub-steps (the
+    ones the surrogate serves) need fewer iterations than with
+    extrapolation alone, which starts them cold."""
+    from xcape_amd import _hip, synthetic
+    s, _, _ = trained
+    s.activate()
+    c = synthetic.sigma_columns(3000, 137, seed=21)
+    arr = [c[k] for k in ('p', 't', 'td', 'ps', 'ts', 'tds')]
+    dev = [_t(torch, a) for a in arr]
+    for adiabat, source in ((3, 2), (1, 1)):
+        ref = orc.cape(*arr, pinc=500., source=source, adiabat=adiabat)
+        kw = dict(pinc=500., source=source, adiabat=adiabat, stats=True)
+        out_s = _hip.cape(*dev, warm_start=True, **kw)
+        out_e = _hip.cape(*dev, warm_start='extrapolation', **kw)
+        for out in (out_s, out_e):
+            assert cape_ok(out[0].cpu().numpy(), ref[0]).all()
+            assert cape_ok(out[1].cpu().numpy(), ref[1]).all()
+        st_s, st_e = out_s[-1], out_e[-1]
+        print(adiabat, source, 'surrogate', st_s, 'extrapolation', st_e)
+        assert st_s['substeps'] == st_e['substeps']
+        assert st_s['first_substep_iterations'] < 0.5 * st_e['first_substep_iterations']
+        assert st_s['iterations'] < st_e['iterations']

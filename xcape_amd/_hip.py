@@ -20,6 +20,7 @@ XCAPE_DEVICE_PTRS = 0x1
 XCAPE_FAST_MATH = 0x2
 XCAPE_WARM_START = 0x4
 XCAPE_NO_SURROGATE = 0x8
+XCAPE_LEVEL_MAJOR = 0x10
 
 
 class XcapeError(RuntimeError):
@@ -34,7 +35,7 @@ class Stats(ctypes.Structure):
     _fields_ = [('columns', ctypes.c_int64), ('levels', ctypes.c_int64),
                 ('substeps', ctypes.c_int64), ('iterations', ctypes.c_int64),
                 ('nonconverged', ctypes.c_int64), ('fallbacks', ctypes.c_int64),
-                ('reserved', ctypes.c_int64 * 2)]
+                ('first_substep_iterations', ctypes.c_int64), ('reserved', ctypes.c_int64 * 1)]
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_ if n != 'reserved'}
@@ -76,7 +77,7 @@ SIGNATURES = {
     'xcape_surrogate_loaded': (_I, []),
     'xcape_surrogate_eval': (_I, [_P, _I64, _P, _P]),
     'xcape_surrogate_log_f32': (_I, [_P, _P, _P, _P, _P, _P, _I64, _I64, _F, _I, _F, _I, _F,
-                                     _U32, _P, _I64, _P, _P]),
+                                     _U32, _I, _P, _I64, _P, _P]),
 }
 
 
@@ -199,18 +200,19 @@ def _warm_flags(warm_start):
 
 def cape(p, t, td, ps, ts, tds, *, pinc=500., source=1, ml_depth=500., adiabat=1,
          p_shared=False, start_lev=None, fast_math=False, warm_start=False, fp64=False,
-         stats=False):
+         stats=False, level_major=False):
     """Run xcape_cape_f32 (or _f64).  t, td: (ncol, nk); p: (ncol, nk) or, with
-    p_shared, (nk,).  Returns cape, cin, mulev, zmulev (and a stats dict)."""
+    p_shared, (nk,); with ``level_major`` the profile arrays are (nk, ncol).
+    Returns cape, cin, mulev, zmulev (and a stats dict)."""
     A = _Args([p, t, td, ps, ts, tds, start_lev])
     p, t, td, ps, ts, tds = (A.f32(a) for a in (p, t, td, ps, ts, tds))
     start_lev = A.i32(start_lev)
-    ncol, nk = t.shape
+    ncol, nk = t.shape[::-1] if level_major else t.shape
     odt = np.float64 if fp64 else np.float32
     cape_o, cin_o, z_o = (A.empty(ncol, odt, t) for _ in range(3))
     mul_o = A.empty(ncol, np.int32, t)
     flags = (XCAPE_DEVICE_PTRS if A.device else 0) | (XCAPE_FAST_MATH if fast_math else 0) \
-        | _warm_flags(warm_start)
+        | _warm_flags(warm_start) | (XCAPE_LEVEL_MAJOR if level_major else 0)
     st = Stats()
     fn = lib().xcape_cape_f64 if fp64 else lib().xcape_cape_f32
     rc = fn(_ptr(p), 1 if p_shared else 0, _ptr(start_lev), _ptr(t), _ptr(td), _ptr(ps),
@@ -224,20 +226,20 @@ def cape(p, t, td, ps, ts, tds, *, pinc=500., source=1, ml_depth=500., adiabat=1
 
 
 def srh(p, t, td, u, v, ps, ts, tds, us, vs, *, depth=3000., p_shared=False, start_lev=None,
-        motions=True):
+        motions=True, level_major=False):
     """Run xcape_srh_f32.  Returns srh_rm, srh_lm (float64) and, when
     ``motions``, rm, lm, mean6 as (ncol, 2) float32 (u, v) pairs."""
     A = _Args([p, t, td, u, v, ps, ts, tds, us, vs, start_lev])
     p, t, td, u, v, ps, ts, tds, us, vs = (A.f32(a) for a in (p, t, td, u, v, ps, ts, tds,
                                                               us, vs))
     start_lev = A.i32(start_lev)
-    ncol, nk = t.shape
+    ncol, nk = t.shape[::-1] if level_major else t.shape
     srm, slm = (A.empty(ncol, np.float64, t) for _ in range(2))
     if motions:
         rm, lm, mw = (A.empty(2 * ncol, np.float32, t) for _ in range(3))
     else:
         rm = lm = mw = None
-    flags = XCAPE_DEVICE_PTRS if A.device else 0
+    flags = (XCAPE_DEVICE_PTRS if A.device else 0) | (XCAPE_LEVEL_MAJOR if level_major else 0)
     rc = lib().xcape_srh_f32(_ptr(p), 1 if p_shared else 0, _ptr(start_lev), _ptr(t), _ptr(td),
                              _ptr(u), _ptr(v), _ptr(ps), _ptr(ts), _ptr(tds), _ptr(us),
                              _ptr(vs), nk, ncol, depth, _ptr(srm), _ptr(slm), _ptr(rm),
@@ -250,20 +252,20 @@ def srh(p, t, td, u, v, ps, ts, tds, us, vs, *, depth=3000., p_shared=False, sta
 
 def cape_srh(p, t, td, u, v, ps, ts, tds, us, vs, *, pinc=500., source=1, ml_depth=500.,
              adiabat=1, depth=3000., p_shared=False, start_lev=None, fast_math=False,
-             warm_start=False, stats=False):
+             warm_start=False, stats=False, level_major=False):
     """Run the fused xcape_cape_srh_f32: (cape, cin, mulev, zmulev, srh_rm,
     srh_lm, rm, lm, mean6[, stats])."""
     A = _Args([p, t, td, u, v, ps, ts, tds, us, vs, start_lev])
     p, t, td, u, v, ps, ts, tds, us, vs = (A.f32(a) for a in (p, t, td, u, v, ps, ts, tds,
                                                               us, vs))
     start_lev = A.i32(start_lev)
-    ncol, nk = t.shape
+    ncol, nk = t.shape[::-1] if level_major else t.shape
     cape_o, cin_o, z_o = (A.empty(ncol, np.float32, t) for _ in range(3))
     mul_o = A.empty(ncol, np.int32, t)
     srm, slm = (A.empty(ncol, np.float64, t) for _ in range(2))
     rm, lm, mw = (A.empty(2 * ncol, np.float32, t) for _ in range(3))
     flags = (XCAPE_DEVICE_PTRS if A.device else 0) | (XCAPE_FAST_MATH if fast_math else 0) \
-        | _warm_flags(warm_start)
+        | _warm_flags(warm_start) | (XCAPE_LEVEL_MAJOR if level_major else 0)
     st = Stats()
     rc = lib().xcape_cape_srh_f32(
         _ptr(p), 1 if p_shared else 0, _ptr(start_lev), _ptr(t), _ptr(td), _ptr(u), _ptr(v),

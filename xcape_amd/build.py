@@ -36,6 +36,7 @@ UNITS = {
     'ragged.hip': ['-ffp-contract=off'],
     'mlp.hip': ['-ffp-contract=fast'],
     'surrogate_log.hip': ['-ffp-contract=off'],
+    'transpose.hip': ['-ffp-contract=off'],
 }
 
 

@@ -26,13 +26,25 @@
 //                       per lane per trip), see its comment.
 //
 // The column accessor C provides P(k), Tc(k), Tdc(k) in hPa / degC for
-// k = 0 .. nk-1, k = 0 being the surface.
+// k = 0 .. nk-1, k = 0 being the surface, and raw_above(k), the same three
+// values of a level k >= 1 (the ascent's levels, read without the surface
+// select).
 #pragma once
 #include "phys.h"
 
 namespace xcape {
 
 enum { SRC_SURFACE = 1, SRC_MU = 2, SRC_ML = 3 };
+
+// Makes v's value unknown to the optimiser at this point (no code).
+XC_HD __forceinline__ void xc_opaque(int &v)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    __asm__ volatile("" : "+v"(v));
+#else
+    __asm__ volatile("" : "+r"(v));
+#endif
+}
 
 template <typename T> struct ColumnOut {
     T cape, cin, zout;
@@ -41,27 +53,47 @@ template <typename T> struct ColumnOut {
 
 struct LaneStats {
     uint32_t levels = 0, substeps = 0, iterations = 0, nonconverged = 0, fallbacks = 0;
+    uint32_t first = 0;   // iterations of the ascent's first sub-step
 };
 
 template <typename T> struct Env {
     T p, t, td, pi, q, th, thv;
 };
 
+// A level's profile values as stored (hPa, degC).  The source scans read
+// level k + 1 while they work on level k, so that the loads are in flight
+// during a level's arithmetic instead of stalling the wave at its start.
+struct Raw {
+    float p, t, td;
+};
+
+template <class C>
+XC_HD __forceinline__ Raw load_raw(const C &col, int k)
+{
+    return Raw{col.P(k), col.Tc(k), col.Tdc(k)};
+}
+
 // :232-240 for one level
-template <typename T, int MATH, class C>
-XC_HD __forceinline__ Env<T> env_at(const C &col, int k)
+template <typename T, int MATH>
+XC_HD __forceinline__ Env<T> env_from(const Raw &r)
 {
     using F = Fn<T, MATH>;
     using K = Phys<T>;
     Env<T> e;
-    e.p = T(100.0) * T(col.P(k));
-    e.t = T(273.15) + T(col.Tc(k));
-    e.td = T(273.15) + T(col.Tdc(k));
+    e.p = T(100.0) * T(r.p);
+    e.t = T(273.15) + T(r.t);
+    e.td = T(273.15) + T(r.td);
     e.pi = F::pow(e.p * K::rp00, K::rddcp);
     e.q = qvs<T, MATH>(e.p, e.td);
     e.th = F::div(e.t, e.pi);
     e.thv = F::div(e.th * (T(1.0) + K::reps * e.q), T(1.0) + e.q);
     return e;
+}
+
+template <typename T, int MATH, class C>
+XC_HD __forceinline__ Env<T> env_at(const C &col, int k)
+{
+    return env_from<T, MATH>(load_raw(col, k));
 }
 
 // layer thickness from the hydrostatic equation in Exner pressure (:246, :503)
@@ -159,19 +191,35 @@ XC_HD __forceinline__ bool parcel_source(const C &col, int nk, int source, T ml_
         T p0 = T(100.0) * T(col.P(0));
         if (!(p0 < T(50000.0))) {
             T maxthe = T(0.0);
+            // z of the levels scanned so far (the reference's z(k), :244-248),
+            // summed while every level below has been scanned: with no level
+            // skipped below kmax, z(kmax) needs no second pass
+            T zrun = T(0.0), zmax = T(0.0);
+            int kskip = nk;   // first level skipped (p < 500 hPa)
+            Env<T> ep{};
+            Raw nx = load_raw(col, 0);
             for (int k = 0; k < nk; ++k) {
-                T pk = T(100.0) * T(col.P(k));
+                const Raw cur = nx;
+                if (k + 1 < nk) nx = load_raw(col, k + 1);
+                T pk = T(100.0) * T(cur.p);
                 if (pk >= T(50000.0)) {
-                    Env<T> e = env_at<T, MATH>(col, k);
+                    Env<T> e = env_from<T, MATH>(cur);
+                    if (k > 0 && k < kskip) zrun = zrun + layer_dz(e.thv, ep.thv, e.pi, ep.pi);
+                    ep = e;
                     T the = thetae<T, MATH>(e.p, e.t, e.td, e.q);
                     if (the > maxthe) {
                         pc.mulvl = k + 1;
                         maxthe = the;
                         kmax = k;
+                        zmax = zrun;
                     }
+                } else if (kskip == nk) {
+                    kskip = k;
                 }
             }
-            if (kmax > 0) {
+            if (kmax < kskip) {
+                zk = zmax;
+            } else {
                 Env<T> ep = env_at<T, MATH>(col, 0);
                 for (int k = 1; k <= kmax; ++k) {
                     Env<T> e = env_at<T, MATH>(col, k);
@@ -185,8 +233,11 @@ XC_HD __forceinline__ bool parcel_source(const C &col, int nk, int source, T ml_
         Env<T> e0 = env_at<T, MATH>(col, 0);
         Env<T> ep = e0;
         T z1 = T(0.0), ztop = T(0.0);
+        Raw nx = nk > 1 ? load_raw(col, 1) : Raw{};
         for (int k = 1; k < nk; ++k) {
-            Env<T> e = env_at<T, MATH>(col, k);
+            const Raw cur = nx;
+            if (k + 1 < nk) nx = load_raw(col, k + 1);
+            Env<T> e = env_from<T, MATH>(cur);
             ztop = ztop + layer_dz(e.thv, ep.thv, e.pi, ep.pi);
             if (k == 1) z1 = ztop;
             ep = e;
@@ -320,22 +371,30 @@ XC_HD void cape_column(const C &col, int nk, T pinc, int source, T ml_depth,
     T p_prev = pc.e.p, pi_prev = pc.e.pi, thv_prev = pc.e.thv;
     int k = pc.kmax;
     bool doit = true;
+    bool first_sub = true;
 
     // ascent (:403-559)
     while (doit && k < nk - 1) {
         k = k + 1;
-        Env<T> e = env_at<T, MATH>(col, k);
+        Env<T> e = env_from<T, MATH>(col.raw_above(k));
         const T b1 = b2;
         T dp = p_prev - e.p;
         const int nloop = substep_count<T, MATH>(dp, pinc);
         st.levels++;
         for (int n = 1; n <= nloop; ++n) {
-            const T p1 = p2, t1 = t2, th1 = th2, qv1 = qv2, ql1 = ql2, qi1 = qi2;
+            // a pseudo-adiabat carries no condensate from one sub-step to
+            // the next (:487-491), so its ql1, qi1 are the constant zero
+            const T p1 = p2, t1 = t2, th1 = th2, qv1 = qv2;
+            const T ql1 = PSEUDO ? T(0.0) : ql2, qi1 = PSEUDO ? T(0.0) : qi2;
             p2 = p2 - dp;
             pi2 = Fn<T, MATH>::pow(p2 * K::rp00, K::rddcp);
             st.substeps++;
-            if (!solve_substep<T, MATH, ICE>(p1, p2, pi2, t1, th1, qv1, ql1, qi1, qt, th1, th2,
-                                             t2, qv2, ql2, qi2, st)) {
+            const uint32_t it0 = st.iterations;
+            const bool ok = solve_substep<T, MATH, ICE>(p1, p2, pi2, t1, th1, qv1, ql1, qi1, qt,
+                                                        th1, th2, t2, qv2, ql2, qi2, st);
+            if (first_sub) st.first += st.iterations - it0;
+            first_sub = false;
+            if (!ok) {
                 st.nonconverged++;
                 out.cape = T(0.0); out.cin = T(0.0); out.mulvl = mulvl; out.zout = zout;
                 return;
@@ -399,42 +458,57 @@ XC_HD void cape_column_warm(const C &col, int nk, T pinc, const Parcel<T> &pc,
     T th2 = pc.th2, pi2 = pc.pi2, p2 = pc.p2, t2 = pc.t2, qv2 = pc.qv2, b2 = pc.b2, zk = pc.zk;
     T ql2 = T(0.0), qi2 = T(0.0), qt = qv2;
     T cape = T(0.0), cin = T(0.0), narea = T(0.0);
-    T p_prev = pc.e.p, pi_prev = pc.e.pi, thv_prev = pc.e.thv;
+    // the previous level's environment: p in e.p until the next level
+    // replaces it, pi and thv here (the layer update needs both levels)
+    T pi_prev = pc.e.pi, thv_prev = pc.e.thv;
     int k = pc.kmax;
     if (!(k < nk - 1)) {
         out.cape = cape; out.cin = cin; out.mulvl = mulvl; out.zout = zout;
         return;
     }
-    // level state
-    Env<T> e;
-    T b1 = T(0.0), dp = T(0.0);
+    // level state (b2 keeps the previous level's buoyancy until the layer
+    // update, so it is also the reference's b1)
+    Env<T> e = pc.e;
+    T dp = T(0.0);
     int nloop = 0, n = 0;
     // sub-step state
     T p1 = T(0.0), t1 = T(0.0), th1 = T(0.0), qv1 = T(0.0), ql1 = T(0.0), qi1 = T(0.0);
     T logp = T(0.0), thlast = T(0.0);
     int it = 0;
     bool warm = false, fell_back = false;
+    bool first_sub = true;
     // previous iterate (for the secant slope) and the last slope estimate
     T x_prev = T(0.0), f_prev = T(0.0), fpr = T(0.0);
     // extrapolation history: last two rates and their ln-p steps
     T r1 = T(0.0), r2 = T(0.0), dl1 = T(0.0), dl2 = T(0.0);
     int nh = 0;
-    bool new_level = true, new_sub = true;
+    // Trip state.  Every trip runs the same straight-line sequence (level
+    // set-up, sub-step set-up, one iteration, level completion), each part
+    // under the mask of the lanes whose mode asks for it; `mode` is made
+    // opaque at the top of the trip so that the compiler cannot thread the
+    // iterate-again path into an inner loop (which would make the lanes that
+    // converged wait for the slowest one again).
+    enum { M_LEVEL = 0, M_SUB = 1, M_ITER = 2, M_LAYER = 3 };
+    int mode = M_LEVEL;
     for (;;) {
-        if (new_level) {
+        xc_opaque(mode);
+        if (mode == M_LEVEL) {
             k = k + 1;
-            e = env_at<T, MATH>(col, k);
-            b1 = b2;
+            const T p_prev = e.p;
+            e = env_from<T, MATH>(col.raw_above(k));
             dp = p_prev - e.p;
             nloop = substep_count<T, MATH>(dp, pinc);
             n = 0;
             st.levels++;
-            new_level = false;
-            new_sub = nloop >= 1;
+            // zero sub-steps (NaN/overflow in the step count): the reference's
+            // loop body does not run; straight to the buoyancy update
+            mode = nloop >= 1 ? M_SUB : M_LAYER;
         }
-        if (new_sub) {
+        if (mode == M_SUB) {
             n = n + 1;
-            p1 = p2; t1 = t2; th1 = th2; qv1 = qv2; ql1 = ql2; qi1 = qi2;
+            p1 = p2; t1 = t2; th1 = th2; qv1 = qv2;
+            ql1 = PSEUDO ? T(0.0) : ql2;
+            qi1 = PSEUDO ? T(0.0) : qi2;
             p2 = p2 - dp;
             pi2 = F::pow(p2 * K::rp00, K::rddcp);
             logp = F::log(F::div(p2, p1));
@@ -456,17 +530,15 @@ XC_HD void cape_column_warm(const C &col, int nk, T pinc, const Parcel<T> &pc,
             fell_back = false;
             it = 0;
             st.substeps++;
-            new_sub = false;
+            mode = M_ITER;
         }
-        if (n > nloop) {
-            // zero sub-steps (NaN/overflow in the step count): the reference's
-            // loop body does not run; go straight to the buoyancy update
-        } else {
+        if (mode == M_ITER) {
             // one fixed-point iteration (:436-479)
             it = it + 1;
             fp_eval<T, MATH, ICE>(p2, pi2, logp, t1, th1, qv1, ql1, qi1, qt, thlast, th2, t2, qv2,
                                   ql2, qi2);
             st.iterations++;
+            if (first_sub) st.first++;
             if (it >= 2) {
                 const T dx = thlast - x_prev;
                 if (dx != T(0.0)) {
@@ -490,44 +562,43 @@ XC_HD void cape_column_warm(const C &col, int nk, T pinc, const Parcel<T> &pc,
                 fell_back = true;
                 it = 0;
                 st.fallbacks++;
-                continue;
-            }
-            if (ad > K::converge) {
+            } else if (ad > K::converge) {
                 thlast = thlast + T(0.3) * (th2 - thlast);
-                continue;
-            }
-            // converged: record the rate, finish the sub-step (:485-497)
-            if (logp != T(0.0)) {
-                const T xs = thlast + F::div(th2 - thlast, T(1.0) - fpr);
-                r2 = r1; dl2 = dl1;
-                r1 = F::div(xs - th1, logp);
-                dl1 = logp;
-                // a second-order guess that failed marks a change of regime
-                // (saturation, ice onset): the older rate belongs to the other
-                // side, so restart at first order; otherwise keep building
-                nh = fell_back && nh >= 2 ? 1 : (nh < 2 ? nh + 1 : 2);
             } else {
-                nh = 0;
-            }
-            if (PSEUDO) {
-                qt = qv2;
-                ql2 = T(0.0);
-                qi2 = T(0.0);
-            }
-            if (n < nloop) {
-                new_sub = true;
-                continue;
+                // converged: record the rate, finish the sub-step (:485-497)
+                first_sub = false;
+                if (logp != T(0.0)) {
+                    const T xs = thlast + F::div(th2 - thlast, T(1.0) - fpr);
+                    r2 = r1; dl2 = dl1;
+                    r1 = F::div(xs - th1, logp);
+                    dl1 = logp;
+                    // a second-order guess that failed marks a change of
+                    // regime (saturation, ice onset): the older rate belongs
+                    // to the other side, so restart at first order
+                    nh = fell_back && nh >= 2 ? 1 : (nh < 2 ? nh + 1 : 2);
+                } else {
+                    nh = 0;
+                }
+                if (PSEUDO) {
+                    qt = qv2;
+                    ql2 = T(0.0);
+                    qi2 = T(0.0);
+                }
+                mode = n < nloop ? M_SUB : M_LAYER;
             }
         }
-        // level complete (:501-558)
-        T dz;
-        layer_update<T, MATH>(th2, qv2, ql2, qi2, e, thv_prev, pi_prev, b1, b2, dz, cape, cin,
-                              narea);
-        zk = zk + dz;
-        zout = zk;
-        p_prev = e.p; pi_prev = e.pi; thv_prev = e.thv;
-        if ((e.p <= T(10000.0) && b2 < T(0.0)) || !(k < nk - 1)) break;
-        new_level = true;
+        if (mode == M_LAYER) {
+            // level complete (:501-558)
+            T dz;
+            const T b1 = b2;
+            layer_update<T, MATH>(th2, qv2, ql2, qi2, e, thv_prev, pi_prev, b1, b2, dz, cape,
+                                  cin, narea);
+            zk = zk + dz;
+            zout = zk;
+            pi_prev = e.pi; thv_prev = e.thv;
+            if ((e.p <= T(10000.0) && b2 < T(0.0)) || !(k < nk - 1)) break;
+            mode = M_LEVEL;
+        }
     }
     out.cape = cape; out.cin = cin; out.mulvl = mulvl; out.zout = zout;
 }

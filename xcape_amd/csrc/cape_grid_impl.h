@@ -1,11 +1,17 @@
-// Rectangular-grid CAPE kernels (sigma or shared pressure levels), included
-// by one translation unit per math mode with XC_MATH defined.
+// Rectangular-grid kernels (sigma or shared pressure levels), included by
+// one translation unit per math mode with XC_MATH defined.
 //
 // Mapping: one column per lane, 64 columns per wavefront, blockDim columns
-// per workgroup.  A column's levels are contiguous in HBM (the reference's
-// (nlev, ncol) Fortran-order layout), so a lane walks its own column; the
-// lines it touches stay in L1/L2 across the ~16 levels each 64-byte segment
-// holds, and HBM traffic stays at the compulsory bytes.
+// per workgroup.  The ascent is compute-bound (one column's ~2000
+// fixed-point iterations against ~400 bytes of profile), so it reads the
+// profiles in whichever layout the caller holds them.  The helicity chain is
+// light arithmetic over many levels: it is given level-major profiles (the
+// C-ABI stages column-contiguous input through transpose_levels first), so
+// that the 64 lanes of a wave read 64 consecutive words.
+//
+// The CAPE, helicity and surrogate kernels are separate launches: each keeps
+// its own register budget (the FP64 helicity chain and the MFMA evaluation
+// would otherwise cap the ascent's occupancy).
 #pragma once
 #include "cape_column.h"
 #include "cape_launch.h"
@@ -15,14 +21,30 @@
 namespace xcape {
 namespace XC_NS {
 
+// Level k of a column (k = 0: the surface values).  Level k >= 1 is element
+// off + (k - 1) * ls of the (uniform) array bases t, td, u, v, and element
+// poff + (k - 1) * pls of p (the shared level vector on a pressure grid).
+// IDX: uint32_t when every element index fits, so that a lane carries one
+// 32-bit offset per array family instead of a 64-bit pointer per array.
+template <typename IDX>
 struct SigmaCol {
     const float *p, *t, *td, *u, *v;
+    IDX off, poff, ls, pls;
     float ps, ts, tds, us, vs;
-    __device__ __forceinline__ float P(int k) const { return k == 0 ? ps : p[k - 1]; }
-    __device__ __forceinline__ float Tc(int k) const { return k == 0 ? ts : t[k - 1]; }
-    __device__ __forceinline__ float Tdc(int k) const { return k == 0 ? tds : td[k - 1]; }
-    __device__ __forceinline__ float U(int k) const { return k == 0 ? us : u[k - 1]; }
-    __device__ __forceinline__ float V(int k) const { return k == 0 ? vs : v[k - 1]; }
+    __device__ __forceinline__ IDX at(int k) const { return off + (IDX)(k - 1) * ls; }
+    __device__ __forceinline__ float P(int k) const
+    {
+        return k == 0 ? ps : p[poff + (IDX)(k - 1) * pls];
+    }
+    __device__ __forceinline__ float Tc(int k) const { return k == 0 ? ts : t[at(k)]; }
+    __device__ __forceinline__ float Tdc(int k) const { return k == 0 ? tds : td[at(k)]; }
+    __device__ __forceinline__ float U(int k) const { return k == 0 ? us : u[at(k)]; }
+    __device__ __forceinline__ float V(int k) const { return k == 0 ? vs : v[at(k)]; }
+    // a level above the surface (k >= 1): no surface select
+    __device__ __forceinline__ Raw raw_above(int k) const
+    {
+        return Raw{p[poff + (IDX)(k - 1) * pls], t[at(k)], td[at(k)]};
+    }
 };
 
 // First level at or above the ground, 1-based: the reference's
@@ -46,10 +68,10 @@ __device__ __forceinline__ int start_level(const float *p1d, int nk_in, float ps
 __device__ __forceinline__ void block_stats_add(const LaneStats &st, bool ran,
                                                 unsigned long long *stats)
 {
-    constexpr int NC = 6;
+    constexpr int NC = 7;
     __shared__ unsigned long long red[NC][16];
     unsigned long long v[NC] = {(unsigned long long)ran, st.levels, st.substeps, st.iterations,
-                                st.nonconverged, st.fallbacks};
+                                st.nonconverged, st.fallbacks, st.first};
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
@@ -65,154 +87,173 @@ __device__ __forceinline__ void block_stats_add(const LaneStats &st, bool ran,
     }
 }
 
-// KIND: GRID_CAPE, GRID_SRH or GRID_CAPE_SRH (both from one read of the
-// column: the helicity chain runs after the ascent on the same lane).
-// WARM: warm-started ascent; with a surrogate loaded (a.surrogate), the first
-// sub-step's iterate of every lane comes from one wave-wide MLP evaluation
-// (MFMA), done at a point every lane of the block reaches.
-template <typename T, bool ICE, bool PSEUDO, int KIND, bool WARM>
+// Column set-up shared by the kernels: the accessor of column `col` and its
+// level count (surface included).  Returns false for lanes past the grid.
+template <typename IDX>
+__device__ __forceinline__ bool column_at(const CapeGridArgs &a, int64_t col, SigmaCol<IDX> &c,
+                                          int &nk)
+{
+    if (col >= a.ncol) return false;
+    const float ps = a.ps[col];
+    c.ls = (IDX)a.ls;
+    IDX off = a.ls == 1 ? (IDX)col * (IDX)a.nk_in : (IDX)col;
+    if (a.p1d) {
+        int s1 = a.start ? a.start[col] : start_level(a.p1d, (int)a.nk_in, ps);
+        // a start level outside 1..nk_in cannot come from the rule above;
+        // clamp caller-supplied ones so the column is never read outside
+        s1 = s1 < 1 ? 1 : (s1 > (int)a.nk_in ? (int)a.nk_in : s1);
+        const int s0 = s1 - 1;
+        c.p = a.p1d;
+        c.poff = (IDX)s0;
+        c.pls = 1;
+        off += (IDX)s0 * c.ls;
+        nk = (int)a.nk_in - s0 + 1;
+    } else {
+        c.p = a.p;
+        c.poff = off;
+        c.pls = c.ls;
+        nk = (int)a.nk_in + 1;
+    }
+    c.off = off;
+    c.t = a.t;
+    c.td = a.td;
+    c.u = a.u;
+    c.v = a.v;
+    c.ps = ps;
+    c.ts = a.ts[col];
+    c.tds = a.tds[col];
+    return true;
+}
+
+// CAPE/CIN, one column per lane.  WARM: the warm-started ascent
+// (cape_column_warm); a.guess, when set, holds each column's first-sub-step
+// iterate from the surrogate (NaN: none).  STATS: accumulate the work
+// counters into a.stats (a separate instantiation, so that the counters do
+// not occupy registers in the timed kernels).
+template <typename T, typename IDX, bool ICE, bool PSEUDO, bool WARM, bool STATS>
 __global__ void __launch_bounds__(256) cape_grid_kernel(CapeGridArgs a)
 {
-    extern __shared__ uint32_t dyn_lds[];
     const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     LaneStats st;
     bool ran = false;
-    const bool in_grid = col < a.ncol;
-    SigmaCol c{};
+    SigmaCol<IDX> c{};
     int nk = 0;
-    int64_t off = 0;
+    const bool in_grid = column_at(a, col, c, nk);
+    T *cape = static_cast<T *>(a.cape), *cin = static_cast<T *>(a.cin),
+      *zout = static_cast<T *>(a.zout);
     if (in_grid) {
-        const float ps = a.ps[col];
-        off = col * a.nk_in;
-        if (a.p1d) {
-            int s1 = a.start ? a.start[col] : start_level(a.p1d, (int)a.nk_in, ps);
-            // a start level outside 1..nk_in cannot come from the rule above;
-            // clamp caller-supplied ones so the column is never read outside
-            s1 = s1 < 1 ? 1 : (s1 > (int)a.nk_in ? (int)a.nk_in : s1);
-            const int s0 = s1 - 1;
-            c.p = a.p1d + s0;
-            off += s0;
-            nk = (int)a.nk_in - s0 + 1;
-        } else {
-            c.p = a.p + off;
-            nk = (int)a.nk_in + 1;
-        }
-        c.t = a.t + off;
-        c.td = a.td + off;
-        c.ps = ps;
-        c.ts = a.ts[col];
-        c.tds = a.tds[col];
-    }
-    if constexpr ((KIND & GRID_CAPE) != 0) {
-        T *cape = static_cast<T *>(a.cape), *cin = static_cast<T *>(a.cin),
-          *zout = static_cast<T *>(a.zout);
-        const bool lane_cape = in_grid && c.ts > 0.0f;
-        if constexpr (WARM) {
-            using F = Fn<T, XC_MATH>;
-            using K = Phys<T>;
-            Parcel<T> pc;
-            bool ok = false;
-            T guess = T(NAN), lp1 = T(0.0);
-            float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
-            if (lane_cape) {
-                ok = parcel_source<T, XC_MATH>(c, nk, a.source, T(a.ml_depth), pc);
-                if (ok && a.surrogate && pc.kmax < nk - 1) {
-                    // the first sub-step: pc.p2 -> pc.p2 - dp (:408-429)
-                    const Env<T> e1 = env_at<T, XC_MATH>(c, pc.kmax + 1);
-                    T dp = pc.e.p - e1.p;
-                    const int nl = substep_count<T, XC_MATH>(dp, T(a.pinc));
-                    const T p2 = pc.p2 - dp;
-                    if (nl >= 1 && p2 > T(0.0)) {
-                        lp1 = F::log(F::div(p2, pc.p2));
-                        f0 = (float)F::log(p2);
-                        f1 = (float)thetae<T, XC_MATH>(pc.p2, pc.t2, pc.t2, pc.qv2);
-                        f2 = (float)pc.qv2;
-                    }
-                }
-            }
-            if (a.surrogate) {
-                // uniform across the block: every lane takes part
-                const float r = surrogate_eval_wave(
-                    static_cast<const SurrogateDev *>(a.surrogate), f0, f1, f2,
-                    dyn_lds + (threadIdx.x >> 6) * MLP_LDS_WORDS_PER_WAVE);
-                if (lp1 != T(0.0)) guess = pc.th2 + T(r) * lp1;
-            }
-            (void)K::g;
-            if (lane_cape) {
-                ran = true;
-                ColumnOut<T> o;
+        ColumnOut<T> o;
+        if (c.ts > 0.0f) {
+            ran = true;
+            if constexpr (WARM) {
+                Parcel<T> pc;
+                const bool ok = parcel_source<T, XC_MATH>(c, nk, a.source, T(a.ml_depth), pc);
+                // stored before the ascent: one register less through it
+                a.mulvl[col] = pc.mulvl;
                 if (ok) {
-                    cape_column_warm<T, XC_MATH, ICE, PSEUDO>(c, nk, T(a.pinc), pc, guess,
+                    const T g = a.guess ? T(a.guess[col]) : T(NAN);
+                    cape_column_warm<T, XC_MATH, ICE, PSEUDO>(c, nk, T(a.pinc), pc, g,
                                                               T(a.fallback_tol), o, st);
                 } else {
-                    o.cape = T(0); o.cin = T(0); o.mulvl = pc.mulvl; o.zout = T(-999999.0);
+                    o.cape = T(0); o.cin = T(0); o.zout = T(-999999.0);
                 }
-                cape[col] = o.cape;
-                cin[col] = o.cin;
+            } else {
+                cape_column<T, XC_MATH, ICE, PSEUDO>(c, nk, T(a.pinc), a.source,
+                                                     T(a.ml_depth), o, st);
                 a.mulvl[col] = o.mulvl;
-                zout[col] = o.zout;
             }
         } else {
-            if (lane_cape) {
-                ran = true;
-                ColumnOut<T> o;
-                cape_column<T, XC_MATH, ICE, PSEUDO>(c, nk, T(a.pinc), a.source, T(a.ml_depth),
-                                                     o, st);
-                cape[col] = o.cape;
-                cin[col] = o.cin;
-                a.mulvl[col] = o.mulvl;
-                zout[col] = o.zout;
-            }
-        }
-        if (in_grid && !lane_cape) {
-            cape[col] = T(0);
-            cin[col] = T(0);
-            zout[col] = T(0);
+            o.cape = T(0); o.cin = T(0); o.zout = T(0);
             a.mulvl[col] = 0;
         }
+        cape[col] = o.cape;
+        cin[col] = o.cin;
+        zout[col] = o.zout;
     }
-    if constexpr ((KIND & GRID_SRH) != 0) {
-        if (in_grid) {
-            c.u = a.u + off;
-            c.v = a.v + off;
-            c.us = a.us[col];
-            c.vs = a.vs[col];
-            SrhOut so;
-            srh_column(c, nk, a.depth, 2.0, so);
-            a.srh_rm[col] = so.srh_rm;
-            a.srh_lm[col] = so.srh_lm;
-            if (a.rm) {
-                a.rm[2 * col] = so.rm_u;
-                a.rm[2 * col + 1] = so.rm_v;
-                a.lm[2 * col] = so.lm_u;
-                a.lm[2 * col + 1] = so.lm_v;
-                a.mean6[2 * col] = so.mean_u;
-                a.mean6[2 * col + 1] = so.mean_v;
-            }
-        }
-    }
-    if ((KIND & GRID_CAPE) && a.stats) block_stats_add(st, ran, a.stats);
+    if constexpr (STATS) block_stats_add(st, ran, a.stats);
 }
 
-template <typename T, int KIND, bool WARM>
-hipError_t launch_kind(const CapeGridArgs &a, int block, hipStream_t s)
+// The surrogate's first-sub-step iterate of every column (NaN where the
+// column has no ascent): the parcel set-up of the ascent, the network's
+// inputs for the sub-step pc.p2 -> pc.p2 - dp (CAPE_CODE_model_lev.f90:408-429)
+// and one wave-wide MFMA evaluation per 64 columns.  Every lane of the block
+// reaches the evaluation.
+__global__ void __launch_bounds__(256) surrogate_guess_kernel(CapeGridArgs a)
+{
+    extern __shared__ uint32_t dyn_lds[];
+    using T = float;
+    using F = Fn<T, XC_MATH>;
+    const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    SigmaCol<uint64_t> c{};
+    int nk = 0;
+    const bool in_grid = column_at(a, col, c, nk);
+    Parcel<T> pc;
+    T lp1 = T(0.0);
+    float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
+    if (in_grid && c.ts > 0.0f && parcel_source<T, XC_MATH>(c, nk, a.source, T(a.ml_depth), pc) &&
+        pc.kmax < nk - 1) {
+        const Env<T> e1 = env_at<T, XC_MATH>(c, pc.kmax + 1);
+        T dp = pc.e.p - e1.p;
+        const int nl = substep_count<T, XC_MATH>(dp, T(a.pinc));
+        const T p2 = pc.p2 - dp;
+        if (nl >= 1 && p2 > T(0.0)) {
+            lp1 = F::log(F::div(p2, pc.p2));
+            f0 = F::log(p2);
+            f1 = thetae<T, XC_MATH>(pc.p2, pc.t2, pc.t2, pc.qv2);
+            f2 = pc.qv2;
+        }
+    }
+    const float r = surrogate_eval_wave(static_cast<const SurrogateDev *>(a.surrogate), f0, f1,
+                                        f2, dyn_lds + (threadIdx.x >> 6) * MLP_LDS_WORDS_PER_WAVE);
+    if (in_grid) a.guess_out[col] = lp1 != T(0.0) ? pc.th2 + r * lp1 : T(NAN);
+}
+
+// Storm-relative helicity chain, one column per lane.
+__global__ void __launch_bounds__(256) srh_grid_kernel(CapeGridArgs a)
+{
+    const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    SigmaCol<uint64_t> c{};
+    int nk = 0;
+    if (!column_at(a, col, c, nk)) return;
+    c.us = a.us[col];
+    c.vs = a.vs[col];
+    SrhOut so;
+    srh_column(c, nk, a.depth, 2.0, so);
+    a.srh_rm[col] = so.srh_rm;
+    a.srh_lm[col] = so.srh_lm;
+    if (a.rm) {
+        a.rm[2 * col] = so.rm_u;
+        a.rm[2 * col + 1] = so.rm_v;
+        a.lm[2 * col] = so.lm_u;
+        a.lm[2 * col + 1] = so.lm_v;
+        a.mean6[2 * col] = so.mean_u;
+        a.mean6[2 * col + 1] = so.mean_v;
+    }
+}
+
+template <typename T, typename IDX, bool WARM, bool STATS>
+hipError_t launch_cape_s(const CapeGridArgs &a, int block, hipStream_t s)
 {
     dim3 grid((unsigned)((a.ncol + block - 1) / block));
-    const size_t lds = (WARM && a.surrogate)
-                           ? sizeof(uint32_t) * MLP_LDS_WORDS_PER_WAVE * (size_t)(block / 64)
-                           : 0;
-    if (KIND == GRID_SRH) {
-        cape_grid_kernel<T, false, true, GRID_SRH, false><<<grid, block, 0, s>>>(a);
-        return hipGetLastError();
-    }
     switch (a.adiabat) {
-    case 1: cape_grid_kernel<T, false, true, KIND, WARM><<<grid, block, lds, s>>>(a); break;
-    case 2: cape_grid_kernel<T, false, false, KIND, WARM><<<grid, block, lds, s>>>(a); break;
-    case 3: cape_grid_kernel<T, true, true, KIND, WARM><<<grid, block, lds, s>>>(a); break;
-    case 4: cape_grid_kernel<T, true, false, KIND, WARM><<<grid, block, lds, s>>>(a); break;
+    case 1: cape_grid_kernel<T, IDX, false, true, WARM, STATS><<<grid, block, 0, s>>>(a); break;
+    case 2: cape_grid_kernel<T, IDX, false, false, WARM, STATS><<<grid, block, 0, s>>>(a); break;
+    case 3: cape_grid_kernel<T, IDX, true, true, WARM, STATS><<<grid, block, 0, s>>>(a); break;
+    case 4: cape_grid_kernel<T, IDX, true, false, WARM, STATS><<<grid, block, 0, s>>>(a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+// 32-bit element offsets when every index of the profile arrays fits (the
+// stats launches, which only count work, always use 64-bit ones)
+template <typename T, bool WARM>
+hipError_t launch_cape(const CapeGridArgs &a, int block, hipStream_t s)
+{
+    if (a.stats) return launch_cape_s<T, uint64_t, WARM, true>(a, block, s);
+    const bool fits = (uint64_t)a.ncol * (uint64_t)a.nk_in < ((uint64_t)1 << 32);
+    return fits ? launch_cape_s<T, uint32_t, WARM, false>(a, block, s)
+                : launch_cape_s<T, uint64_t, WARM, false>(a, block, s);
 }
 
 template <typename T>
@@ -222,16 +263,21 @@ hipError_t launch_t(const CapeGridArgs &a, int kind, int block, hipStream_t s)
     if constexpr (sizeof(T) == 8) {
         // the FP64 yardstick covers CAPE with the reference's cold start only
         if (kind != GRID_CAPE || a.warm) return hipErrorInvalidValue;
-        return launch_kind<T, GRID_CAPE, false>(a, block, s);
+        return launch_cape<T, false>(a, block, s);
     } else {
+        dim3 grid((unsigned)((a.ncol + block - 1) / block));
         switch (kind) {
         case GRID_CAPE:
-            return a.warm ? launch_kind<T, GRID_CAPE, true>(a, block, s)
-                          : launch_kind<T, GRID_CAPE, false>(a, block, s);
-        case GRID_SRH: return launch_kind<T, GRID_SRH, false>(a, block, s);
-        case GRID_CAPE_SRH:
-            return a.warm ? launch_kind<T, GRID_CAPE_SRH, true>(a, block, s)
-                          : launch_kind<T, GRID_CAPE_SRH, false>(a, block, s);
+            return a.warm ? launch_cape<T, true>(a, block, s) : launch_cape<T, false>(a, block, s);
+        case GRID_SRH:
+            srh_grid_kernel<<<grid, block, 0, s>>>(a);
+            return hipGetLastError();
+        case GRID_GUESS:
+            if (!a.surrogate || !a.guess_out) return hipErrorInvalidValue;
+            surrogate_guess_kernel<<<grid, block,
+                                     sizeof(uint32_t) * MLP_LDS_WORDS_PER_WAVE * (size_t)(block / 64),
+                                     s>>>(a);
+            return hipGetLastError();
         default: return hipErrorInvalidValue;
         }
     }

@@ -8,7 +8,7 @@ namespace xcape {
 
 // Device counters one launch accumulates (one atomic per block per counter).
 enum { ST_COLUMNS = 0, ST_LEVELS, ST_SUBSTEPS, ST_ITERATIONS, ST_NONCONVERGED,
-       ST_FALLBACKS, ST_COUNT = 8 };
+       ST_FALLBACKS, ST_FIRST, ST_COUNT = 8 };
 
 struct CapeGridArgs {
     const float *p;        // (ncol, nk_in) sigma grid, or nullptr
@@ -17,6 +17,7 @@ struct CapeGridArgs {
     const float *t, *td;   // (ncol, nk_in)
     const float *ps, *ts, *tds;  // (ncol,)
     int64_t nk_in, ncol;
+    int64_t ls;                  // level stride of the (nk_in, ncol) arrays: 1 or ncol
     float pinc, ml_depth;
     int source, adiabat;
     void *cape, *cin, *zout;     // float or double (ncol,)
@@ -32,10 +33,18 @@ struct CapeGridArgs {
     int warm;                    // 0: reference cold start
     float fallback_tol;          // first-iteration residual that rejects a guess [K]
     const void *surrogate;       // device MLP weights (or nullptr)
+    const float *guess;          // (ncol,) first-sub-step iterates for the warm ascent, or nullptr
+    float *guess_out;            // (ncol,) written by the GRID_GUESS launch
 };
 
 // kind of work a grid launch does
-enum { GRID_CAPE = 1, GRID_SRH = 2, GRID_CAPE_SRH = 3 };
+enum { GRID_CAPE = 1, GRID_SRH = 2, GRID_CAPE_SRH = 3, GRID_GUESS = 4 };
+
+// (nk, ncol) level-major copies of column-contiguous (ncol, nk) arrays:
+// dst[j][i] = src[i][j] for up to 5 arrays, LDS-tiled so that both the
+// reads and the writes are coalesced.
+hipError_t launch_transpose_levels(const float *const *src, float *const *dst, int narr,
+                                   int64_t ncol, int64_t nk, hipStream_t s);
 
 hipError_t launch_cape_grid_accurate(const CapeGridArgs &a, bool fp64, int kind, int block,
                                      hipStream_t s);

@@ -7,6 +7,7 @@
 #include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <string>
@@ -85,7 +86,8 @@ void copy_stats(const unsigned long long *h, xcape_stats *stats)
     stats->iterations = (int64_t)h[ST_ITERATIONS];
     stats->nonconverged = (int64_t)h[ST_NONCONVERGED];
     stats->fallbacks = (int64_t)h[ST_FALLBACKS];
-    stats->reserved[0] = stats->reserved[1] = 0;
+    stats->first_substep_iterations = (int64_t)h[ST_FIRST];
+    stats->reserved[0] = 0;
 }
 
 }  // namespace xcape
@@ -106,6 +108,57 @@ int check_cape_params(double pinc, int source, int adiabat)
     if (source < 1 || source > 3) return fail(XCAPE_EINVAL, "source=%d not in 1..3", source);
     if (adiabat < 1 || adiabat > 4) return fail(XCAPE_EINVAL, "adiabat=%d not in 1..4", adiabat);
     return XCAPE_OK;
+}
+
+// Threads per block of the ascent launch.  XCAPE_CAPE_BLOCK (64, 128 or 256)
+// overrides the default, for tuning.
+int cape_block(bool warm)
+{
+    static const int env = [] {
+        const char *v = getenv("XCAPE_CAPE_BLOCK");
+        const int b = v ? atoi(v) : 0;
+        return (b == 64 || b == 128 || b == 256) ? b : 0;
+    }();
+    if (env) return env;
+    return warm ? 64 : 256;
+}
+
+// The helicity chain.  Level-major profiles go straight to the kernel;
+// column-contiguous ones are staged level-major first, in chunks of columns
+// whose copies fit in about 1 GiB (each chunk: one transpose launch for all
+// profile arrays, then the helicity launch on the copies).
+hipError_t run_srh(const CapeGridArgs &a0, hipStream_t s)
+{
+    if (a0.ls != 1 || a0.nk_in == 1) return launch_cape_grid_accurate(a0, false, GRID_SRH, 256, s);
+    const bool sigma = a0.p != nullptr;
+    const int narr = sigma ? 5 : 4;
+    const int64_t nk = a0.nk_in;
+    const int64_t per_col = (int64_t)narr * nk * (int64_t)sizeof(float);
+    int64_t chunk = ((int64_t)1 << 30) / per_col;
+    if (chunk < 65536) chunk = 65536;
+    if (chunk > a0.ncol) chunk = a0.ncol;
+    DevBuf<float> scratch;
+    hipError_t e = scratch.alloc((size_t)narr * (size_t)nk * (size_t)chunk, s);
+    if (e != hipSuccess) return e;
+    for (int64_t c0 = 0; c0 < a0.ncol; c0 += chunk) {
+        const int64_t n = chunk < a0.ncol - c0 ? chunk : a0.ncol - c0;
+        const int64_t o = c0 * nk;
+        const float *src[5] = {a0.t + o, a0.td + o, a0.u + o, a0.v + o, sigma ? a0.p + o : nullptr};
+        float *dst[5];
+        for (int i = 0; i < 5; ++i) dst[i] = scratch.ptr + (size_t)i * (size_t)nk * (size_t)n;
+        if ((e = launch_transpose_levels(src, dst, narr, n, nk, s)) != hipSuccess) return e;
+        CapeGridArgs a = a0;
+        a.ncol = n;
+        a.ls = n;
+        a.t = dst[0]; a.td = dst[1]; a.u = dst[2]; a.v = dst[3];
+        if (sigma) a.p = dst[4];
+        a.ps += c0; a.ts += c0; a.tds += c0; a.us += c0; a.vs += c0;
+        if (a.start) a.start += c0;
+        a.srh_rm += c0; a.srh_lm += c0;
+        if (a.rm) { a.rm += 2 * c0; a.lm += 2 * c0; a.mean6 += 2 * c0; }
+        if ((e = launch_cape_grid_accurate(a, false, GRID_SRH, 256, s)) != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 // Everything a grid call needs; pointers as the caller passed them.
@@ -176,6 +229,7 @@ int grid_entry(const GridCall &c)
     CapeGridArgs a{};
     a.nk_in = c.nk;
     a.ncol = c.ncol;
+    a.ls = (c.flags & XCAPE_LEVEL_MAJOR) ? c.ncol : 1;
     a.pinc = (float)c.pinc;
     a.ml_depth = (float)c.ml_depth;
     a.source = c.source;
@@ -233,16 +287,26 @@ int grid_entry(const GridCall &c)
         XC_TRY(zero_counters(dstats_.ptr, ST_COUNT, s));
         a.stats = dstats_.ptr;
     }
-    // warm launches use one-wave blocks: the flattened ascent has no
-    // cross-wave work, and the surrogate's LDS stays per wave
-    const int block = a.warm ? 64 : 256;
-    if (c.fp64)
-        e = launch_cape_grid_accurate(a, true, c.kind, block, s);
-    else if (c.flags & XCAPE_FAST_MATH)
-        e = launch_cape_grid_fast(a, c.kind, block, s);
-    else
-        e = launch_cape_grid_accurate(a, false, c.kind, block, s);
-    if (e != hipSuccess) return hip_fail(e, "kernel launch");
+    DevBuf<float> guess_;
+    if (want_cape) {
+        const bool fast = c.flags & XCAPE_FAST_MATH;
+        auto launch = [&](int kind, int block) {
+            return c.fp64 ? launch_cape_grid_accurate(a, true, kind, block, s)
+                          : fast ? launch_cape_grid_fast(a, kind, block, s)
+                                 : launch_cape_grid_accurate(a, false, kind, block, s);
+        };
+        if (a.warm && a.surrogate) {
+            // the surrogate's first iterates, one MFMA evaluation per wave,
+            // in a launch of their own (the ascent keeps its register budget)
+            XC_TRY(guess_.alloc(ncol, s));
+            a.guess_out = guess_.ptr;
+            if ((e = launch(GRID_GUESS, 256)) != hipSuccess) return hip_fail(e, "surrogate launch");
+            a.guess = guess_.ptr;
+        }
+        if ((e = launch(GRID_CAPE, cape_block(a.warm))) != hipSuccess)
+            return hip_fail(e, "CAPE launch");
+    }
+    if (want_srh && (e = run_srh(a, s)) != hipSuccess) return hip_fail(e, "helicity launch");
     if (!dev) {
         if (want_cape) {
             XC_TRY(hipMemcpyAsync(c.cape, dcape_.ptr, ncol * osz, hipMemcpyDeviceToHost, s));

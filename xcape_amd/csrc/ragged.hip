@@ -89,6 +89,11 @@ struct TiledCol {
     {
         return k == 0 ? tds : (k - 1 < tile ? ltd[(k - 1) * 64] : td[k - 1]);
     }
+    __device__ __forceinline__ Raw raw_above(int k) const
+    {
+        return k - 1 < tile ? Raw{lp[(k - 1) * 64], lt[(k - 1) * 64], ltd[(k - 1) * 64]}
+                            : Raw{p[k - 1], t[k - 1], td[k - 1]};
+    }
 };
 
 __global__ void reset_counter(unsigned int *c) { *c = 0u; }
@@ -176,10 +181,10 @@ __global__ void __launch_bounds__(256) ragged_kernel(RaggedArgs a)
         }
     }
     if (a.stats) {
-        unsigned long long v[6] = {ran, st.levels, st.substeps, st.iterations, st.nonconverged,
-                                   st.fallbacks};
+        unsigned long long v[7] = {ran, st.levels, st.substeps, st.iterations, st.nonconverged,
+                                   st.fallbacks, st.first};
 #pragma unroll
-        for (int c = 0; c < 6; ++c) {
+        for (int c = 0; c < 7; ++c) {
             unsigned long long x = v[c];
             for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
             if (lane == 0 && x) atomicAdd(&a.stats[c], x);

@@ -1,10 +1,12 @@
 // Training data for the warm-start surrogate, logged from the reference
 // (cold-start) ascent.
 //
-// For a hashed subset of the converged sub-steps the logger writes one
-// record (ln p2 [ln Pa], parcel theta-e at the sub-step's start [K], total
-// water qt [kg/kg], rate (th2 - th1) / ln(p2/p1) [K]) to a device buffer,
-// reserving slots with one atomic per record.  The parcel theta-e is the
+// For a hashed subset of the converged sub-steps (or of the ascents' first
+// sub-steps only, the ones a warm start has no history for) the logger
+// writes one record (ln p2 [ln Pa], parcel theta-e at the sub-step's start
+// [K], total water qt [kg/kg], rate (th2 - th1) / ln(p2/p1) [K], and the
+// step ln(p2/p1)) to a device buffer, reserving slots with one atomic per
+// record.  The parcel theta-e is the
 // reference's own formula for the parcel, getthe(p, t, t, qv)
 // (CAPE_CODE_model_lev.f90:505, which computes and discards it).
 #include <hip/hip_runtime.h>
@@ -23,6 +25,7 @@ struct LogCol {
     __device__ __forceinline__ float P(int k) const { return k == 0 ? ps : p[k - 1]; }
     __device__ __forceinline__ float Tc(int k) const { return k == 0 ? ts : t[k - 1]; }
     __device__ __forceinline__ float Tdc(int k) const { return k == 0 ? tds : td[k - 1]; }
+    __device__ __forceinline__ Raw raw_above(int k) const { return Raw{p[k - 1], t[k - 1], td[k - 1]}; }
 };
 
 struct SampleLogger {
@@ -31,6 +34,7 @@ struct SampleLogger {
     unsigned long long cap;
     uint32_t keep;        // keep a sub-step when its hash's low 16 bits < keep
     uint32_t seed;
+    int first_only;       // record only the ascent's first sub-step
     int64_t col;
     mutable uint32_t sub = 0;
 
@@ -42,16 +46,18 @@ struct SampleLogger {
         h *= 2246822519u;
         h ^= h >> 13;
         ++sub;
+        if (first_only && sub > 1) return;
         if ((h & 0xffffu) >= keep) return;
         const float lp = logf(p2 / p1);
         if (!(lp < 0.0f)) return;
         const unsigned long long i = atomicAdd(count, 1ull);
         if (i >= cap) return;
         const float the = thetae<float, MATH_ACCURATE>(p1, t1, t1, qv1);
-        out[4 * i + 0] = logf(p2);
-        out[4 * i + 1] = the;
-        out[4 * i + 2] = qt;
-        out[4 * i + 3] = (th2 - th1) / lp;
+        out[5 * i + 0] = logf(p2);
+        out[5 * i + 1] = the;
+        out[5 * i + 2] = qt;
+        out[5 * i + 3] = (th2 - th1) / lp;
+        out[5 * i + 4] = lp;
     }
 };
 
@@ -78,8 +84,8 @@ extern "C" int xcape_surrogate_log_f32(const float *p, const float *t, const flo
                                        const float *ps, const float *ts, const float *tds,
                                        int64_t nk, int64_t ncol, float pinc, int source,
                                        float ml_depth, int adiabat, float keep_fraction,
-                                       uint32_t seed, float *samples, int64_t capacity,
-                                       unsigned long long *count, void *stream)
+                                       uint32_t seed, int first_only, float *samples,
+                                       int64_t capacity, unsigned long long *count, void *stream)
 {
     if (nk < 1 || ncol < 1) return fail(XCAPE_EINVAL, "empty grid");
     if (!p || !t || !td || !ps || !ts || !tds || !samples || !count)
@@ -92,7 +98,7 @@ extern "C" int xcape_surrogate_log_f32(const float *p, const float *t, const flo
     int rc = ensure_device();
     if (rc) return rc;
     SampleLogger lg{samples, count, (unsigned long long)capacity,
-                    (uint32_t)(keep_fraction * 65536.0f), seed, 0};
+                    (uint32_t)(keep_fraction * 65536.0f), seed, first_only ? 1 : 0, 0};
     if (keep_fraction >= 1.0f) lg.keep = 0x10000u;
     hipStream_t s = (hipStream_t)stream;
     dim3 grid((unsigned)((ncol + 255) / 256));

@@ -13,10 +13,13 @@ cold-start ascent kernel (``xcape_surrogate_log_f32``), training uses the
 library's forward/backward/Adam kernels, evaluation its in-kernel MFMA path.
 torch only holds the device buffers.
 
-Training data recipe: ragged-batch columns (level counts 30/40/60/91/137,
-surface / mixed-layer / most-unstable parcels, pseudo-liquid and pseudo-ice
-adiabats, pinc 500 Pa), generated on the GPU from a fixed seed; the
-validation columns come from a different seed (a column-level split).
+Training data recipe: the first sub-step of the cold-start ascent of
+ragged-batch columns (level counts 30/40/60/91/137, surface / mixed-layer /
+most-unstable parcels, pseudo-liquid and pseudo-ice adiabats, pinc 500 Pa),
+generated on the GPU from a fixed seed; the validation columns come from a
+different seed (a column-level split).  The first sub-step is the one the
+network serves: every later sub-step has the column's own history to
+extrapolate from, which predicts far closer than a BF16 network can.
 """
 import ctypes
 import json
@@ -55,45 +58,59 @@ def _stream():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
-def log_samples(ncol, seed, keep_fraction=0.25, pinc=500., ml_depth=500., capacity=None):
+def log_samples(ncol, seed, keep_fraction=1.0, first_only=True, pinc=500., ml_depth=500.,
+                chunk=1 << 20):
     """Run the cold-start ascent over `ncol` ragged-recipe columns and return
-    the logged (ln p2, theta-e, qt, rate) records as an (n, 4) CUDA tensor."""
+    the logged records (ln p2, theta-e, qt, rate, ln(p2/p1)) as an (n, 5)
+    CUDA tensor.  first_only: one record per ascent, its first sub-step (the
+    sub-step the warm start has no history for); else a hashed
+    `keep_fraction` of all sub-steps.  Columns are generated on the GPU in
+    chunks of `chunk`."""
     import torch
 
     from . import synthetic
     rng = np.random.default_rng(seed)
-    nlev = rng.choice(np.asarray(NLEVS), ncol)
-    src = rng.choice(np.asarray(SOURCES), ncol)
-    adi = rng.choice(np.asarray(ADIABATS), ncol)
-    if capacity is None:
-        capacity = int(ncol * 300 * keep_fraction) + 1024
-    out = torch.empty((capacity, 4), dtype=torch.float32, device='cuda')
+    per_col = 1.0 if first_only else 300.0 * keep_fraction
+    capacity = int(ncol * per_col) + 1024
+    out = torch.empty((capacity, 5), dtype=torch.float32, device='cuda')
     count = torch.zeros(1, dtype=torch.int64, device='cuda')
     P = _hip._ptr
-    sub = rng.integers(0, 2**31, 64)
-    j = 0
-    for nk in NLEVS:
-        m = nlev == nk
-        if not m.any():
-            continue
-        cols = synthetic.sigma_columns_torch(int(m.sum()), nk, int(sub[j]), device='cuda')
-        j += 1
-        srcs, adis = src[m], adi[m]
-        for s in SOURCES:
-            for a in ADIABATS:
-                idx = np.nonzero((srcs == s) & (adis == a))[0]
-                if idx.size == 0:
-                    continue
-                it = torch.as_tensor(idx, device='cuda')
-                g = {k: v[it].contiguous() for k, v in cols.items()}
-                rc = _hip.lib().xcape_surrogate_log_f32(
-                    P(g['p']), P(g['t']), P(g['td']), P(g['ps']), P(g['ts']), P(g['tds']),
-                    nk, idx.size, pinc, s, ml_depth, a, keep_fraction, int(sub[j] & 0xffffffff),
-                    P(out), capacity, P(count), _stream())
-                _hip.check(rc)
+    for c0 in range(0, ncol, chunk):
+        n = min(chunk, ncol - c0)
+        nlev = rng.choice(np.asarray(NLEVS), n)
+        src = rng.choice(np.asarray(SOURCES), n)
+        adi = rng.choice(np.asarray(ADIABATS), n)
+        for nk in NLEVS:
+            m = nlev == nk
+            if not m.any():
+                continue
+            cols = synthetic.sigma_columns_torch(int(m.sum()), nk, int(rng.integers(2**31)),
+                                                 device='cuda')
+            srcs, adis = src[m], adi[m]
+            for s in SOURCES:
+                for a in ADIABATS:
+                    idx = np.nonzero((srcs == s) & (adis == a))[0]
+                    if idx.size == 0:
+                        continue
+                    it = torch.as_tensor(idx, device='cuda')
+                    g = {k: v[it].contiguous() for k, v in cols.items()}
+                    _hip.check(_hip.lib().xcape_surrogate_log_f32(
+                        P(g['p']), P(g['t']), P(g['td']), P(g['ps']), P(g['ts']), P(g['tds']),
+                        nk, idx.size, pinc, s, ml_depth, a, keep_fraction,
+                        int(rng.integers(2**32)), 1 if first_only else 0, P(out), capacity,
+                        P(count), _stream()))
+            del cols
     torch.cuda.synchronize()
     n = min(int(count.item()), capacity)
     return out[:n]
+
+
+def guess_error(s, samples):
+    """|theta guess - converged theta| [K] of the first iterate the surrogate
+    gives for logged sub-steps (rate error times the sub-step's ln-p step)."""
+    import torch
+    r = s.predict(samples[:, :3].contiguous())
+    return (r - samples[:, 3]).abs() * samples[:, 4].abs()
 
 
 class Surrogate:
@@ -172,7 +189,7 @@ def loss_grad(theta, u, t):
 
 
 def fit(train, val, epochs=12, batch=16384, lr=3e-3, seed=0, log=None):
-    """Train on (n, 4) sample tensors with the library's kernels (Adam, cosine
+    """Train on (n, >=4) sample tensors with the library's kernels (Adam, cosine
     learning-rate decay).  Returns a Surrogate with its training record."""
     import torch
 
