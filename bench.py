@@ -3,21 +3,29 @@
 Workload (one "step" = one pass of the hot path over one batch): 2048 x 1024
 columns x 137 sigma levels, float32, most-unstable parcel, pseudo-adiabatic
 ascent with ice, pinc 500 Pa, plus storm-relative helicity 0-3 km with Bunkers
-motion, all in one fused kernel launch per step.  Inputs are generated in HBM
+motion on the same columns (xcape_cape_srh_f32).  Inputs are generated in HBM
 (seeded synthetic soundings and winds) before the timed region; outputs stay
 in HBM.
 
 Prints one JSON line:
-  value        columns/s of the timed steps (whole job, one GPU)
-  roofline     the fused kernel against the FP32 compute peak: algorithmic
-               operations per launch (unit-weighted FP32 ops of the plain
-               method, from the kernel's own level/sub-step/iteration counts)
-               divided by the mean launch time measured with HIP events on the
-               launch stream
-  cpu_baseline the reference Fortran (compiled from source, oracle/_ref) on the
-               host cores over a bounded sample of the same workload, or the C
-               restatement of the oracle when that build is absent
-  accuracy     CAPE/CIN error of a column sample against the FP32 oracle
+  value         columns/s of the warm-started path (surrogate + extrapolation)
+  plain_kernel  the same step with the reference's cold-started method
+  roofline      the dominant kernel (the warm-started ascent, CAPE only) against
+                the FP32 peak: algorithmic operations per launch (unit-weighted
+                FP32 ops of the reference method, from the kernel's own level /
+                sub-step / iteration counters) over its mean launch time,
+                measured with HIP events on the launch stream
+  attribution   the CAPE-only times of the reference loops, the warm start's
+                loop with cold starts, extrapolation only, and with the surrogate
+  accuracy      CAPE/CIN/MUlev/zMUlev of a column sample against the FP32 oracle
+                and against the FP64 re-run of the GPU kernel; SRH and Bunkers
+                against the oracle
+  cpu_baseline  the reference Fortran (compiled from source, oracle/_ref) on the
+                host cores over a bounded sample of the same workload, or the C
+                restatement of the oracle when that build is absent
+
+--profile-pass runs each kernel variant once (plus a 1 GiB device copy for
+counter calibration) and prints nothing else, for rocprofv3 counter runs.
 """
 import argparse
 import json
@@ -34,19 +42,20 @@ NCOL_X, NCOL_Y, NLEV = 2048, 1024, 137
 PINC, SOURCE, ADIABAT, DEPTH = 500.0, 2, 3, 3000.0
 FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
+SEED = 4004
 
-# unit-weighted FP32 operations per unit of plain-method work: environment per
-# level (26), ascent bookkeeping per level (35), per sub-step (6), per
-# fixed-point iteration with the ice fraction (74), per most-unstable level
-# scanned (26, theta-e)
+# unit-weighted FP32 operations per unit of reference-method work:
+# environment per level (26), ascent bookkeeping per level (35), per
+# sub-step (6), per fixed-point iteration with the ice fraction (74) or
+# without (69), per most-unstable level scanned (26, theta-e)
 W_ENV, W_LEVEL, W_SUBSTEP, W_ITER_ICE, W_ITER_LIQ, W_MU = 26, 35, 6, 74, 69, 26
+PMC_FILE = os.path.join(ROOT, 'profiles', 'cape_warm_combined_2048x1024x137_pmc.json')
 
 
-def algorithmic_ops(stats, nk, ncol, mu_levels):
-    iters = stats['iterations']
+def algorithmic_ops(stats, nk, mu_levels):
     w_it = W_ITER_ICE if ADIABAT in (3, 4) else W_ITER_LIQ
     return (W_ENV * (nk + 1) * stats['columns'] + W_MU * mu_levels + W_LEVEL * stats['levels']
-            + W_SUBSTEP * stats['substeps'] + w_it * iters)
+            + W_SUBSTEP * stats['substeps'] + w_it * stats['iterations'])
 
 
 def bytes_per_column(nk):
@@ -111,20 +120,66 @@ def cpu_baseline(host, threads, seconds):
                       f'on {threads} threads'}
 
 
-def accuracy(outs, inp, sample):
-    """CAPE/CIN of the first `sample` columns against the FP32 oracle."""
+def accuracy(outs, inp, sample, fp64_outs):
+    """Outputs of the first `sample` columns against the FP32 oracle (CAPE,
+    CIN, MUlev, zMUlev, SRH, Bunkers) and against the FP64 re-run of the GPU
+    kernel (CAPE, CIN, MUlev, zMUlev)."""
     from oracle import oracle as O
     a = {k: v[:sample].cpu().numpy() for k, v in inp.items()}
     o = O.cape(a['p'], a['t'], a['td'], a['ps'], a['ts'], a['tds'], PINC, SOURCE, 500., ADIABAT)
+    g = [x[:sample].cpu().numpy() for x in outs]
+    f64 = [x[:sample].cpu().numpy() for x in fp64_outs]
     res = {}
-    for name, g, r in (('cape', outs[0], o[0]), ('cin', outs[1], o[1])):
-        g = g[:sample].cpu().numpy().astype(np.float64)
-        r = r.astype(np.float64)
-        d = np.abs(g - r)
-        tol = np.maximum(0.5, 1e-3 * np.abs(r))
-        res[name] = {'max_abs_err': float(d.max()), 'frac_within_tol': float(np.mean(d <= tol))}
+    for i, name in enumerate(('cape', 'cin')):
+        d = np.abs(g[i].astype(np.float64) - o[i])
+        tol = np.maximum(0.5, 1e-3 * np.abs(o[i].astype(np.float64)))
+        res[name] = {'max_abs_err': float(d.max()), 'frac_within_tol': float(np.mean(d <= tol)),
+                     'max_abs_err_vs_fp64_gpu': float(np.abs(g[i] - f64[i]).max())}
+    res['mulev'] = {'frac_equal': float(np.mean(g[2] == o[2])),
+                    'frac_equal_fp64_gpu': float(np.mean(g[2] == f64[2]))}
+    res['zmulev'] = {'max_abs_err': float(np.abs(g[3].astype(np.float64) - o[3]).max()),
+                     'max_abs_err_vs_fp64_gpu': float(np.abs(g[3] - f64[3]).max())}
+    srh = O.srh(a['p'], a['t'], a['td'], a['u'], a['v'], a['ps'], a['ts'], a['tds'], a['us'],
+                a['vs'], DEPTH)
+    res['srh'] = {'max_abs_err': float(max(np.abs(g[4] - srh[0]).max(),
+                                           np.abs(g[5] - srh[1]).max()))}
+    res['bunkers'] = {'max_abs_err': float(max(np.abs(g[6] - srh[2]).max(),
+                                               np.abs(g[7] - srh[3]).max(),
+                                               np.abs(g[8] - srh[4]).max()))}
     res['columns'] = sample
     return res
+
+
+def load_surrogate(path, log):
+    """Activate the committed surrogate, or train one on the GPU when the
+    file is absent.  Returns a description for the JSON line."""
+    from xcape_amd import surrogate
+    if path and os.path.exists(path):
+        s = surrogate.Surrogate.load(path)
+        src = os.path.relpath(path, ROOT)
+    else:
+        log('[bench] no surrogate file: training one on the GPU (2 M columns)')
+        s = surrogate.train(2_000_000, 200_000, epochs=6, log=None)
+        src = 'trained in this run'
+    s.activate()
+    return {'source': src, 'val_guess_error_K': s.meta.get('val_guess_error_K'),
+            'train_samples': s.meta.get('train_samples')}
+
+
+def timed(fn, steps, warmup, stream):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(steps):
+        out = fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return out, e0.elapsed_time(e1) / steps, time.perf_counter() - t0
 
 
 def main():
@@ -134,12 +189,16 @@ def main():
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--fast-math', action='store_true')
     ap.add_argument('--plain', action='store_true',
-                    help='report the cold-start kernel as the value (default: warm start)')
+                    help='report the cold-started reference method as the value')
+    ap.add_argument('--surrogate', default=os.path.join(ROOT, 'xcape_amd', 'data',
+                                                        'surrogate_v1.npz'))
     ap.add_argument('--cpu-sample', type=int, default=1 << 20,
                     help='columns copied to the host for the CPU baseline (upper bound)')
     ap.add_argument('--cpu-seconds', type=float, default=10.0)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--ncol', type=int, default=NCOL_X * NCOL_Y)
+    ap.add_argument('--profile-pass', action='store_true',
+                    help='one launch of each kernel variant, for counter runs')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py runs on one GPU; --gpus {args.gpus} is not supported '
@@ -150,56 +209,88 @@ def main():
     import torch
 
     from xcape_amd import _hip, synthetic
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
     torch.cuda.set_device(0)
     ncol = args.ncol
-    print(f'[bench] generating {ncol} columns x {NLEV} levels on the GPU', file=sys.stderr,
-          flush=True)
-    inp = synthetic.sigma_columns_torch(ncol, NLEV, 4004, device='cuda', winds=True)
+    log(f'[bench] generating {ncol} columns x {NLEV} levels on the GPU')
+    inp = synthetic.sigma_columns_torch(ncol, NLEV, SEED, device='cuda', winds=True)
     torch.cuda.synchronize()
     keys = ('p', 't', 'td', 'u', 'v', 'ps', 'ts', 'tds', 'us', 'vs')
     a = [inp[k] for k in keys]
+    cape_in = (a[0], a[1], a[2], a[5], a[6], a[7])
+    kw = dict(pinc=PINC, source=SOURCE, adiabat=ADIABAT, fast_math=args.fast_math)
+    stream = torch.cuda.current_stream()
+    sur = load_surrogate(args.surrogate, log)
 
     def step(warm, stats=False):
-        return _hip.cape_srh(*a, pinc=PINC, source=SOURCE, adiabat=ADIABAT, depth=DEPTH,
-                             fast_math=args.fast_math, warm_start=warm, stats=stats)
+        return _hip.cape_srh(*a, depth=DEPTH, warm_start=warm, stats=stats, **kw)
 
-    def timed(warm):
-        for _ in range(args.warmup):
-            step(warm)
-        torch.cuda.synchronize()
-        stream = torch.cuda.current_stream()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        t0 = time.perf_counter()
-        e0.record(stream)
-        for _ in range(args.steps):
-            outs = step(warm)
-        e1.record(stream)
-        torch.cuda.synchronize()
-        return outs, e0.elapsed_time(e1) / args.steps, time.perf_counter() - t0
+    def cape_only(warm, stats=False):
+        return _hip.cape(*cape_in, warm_start=warm, stats=stats, **kw)
 
-    # counting passes: level / sub-step / iteration totals of both methods
-    st_plain = step(False, stats=True)[-1]
-    st_warm = step(True, stats=True)[-1]
-    torch.cuda.synchronize()
+    if args.profile_pass:
+        x = torch.ones(1 << 28, dtype=torch.float32, device='cuda')
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        for warm in (False, True):
+            cape_only(warm)
+            torch.cuda.synchronize()
+        step(True)
+        torch.cuda.synchronize()
+        log('[bench] profile pass done')
+        return
+
+    # counting passes: level / sub-step / iteration totals of each method
+    log('[bench] counting passes')
+    counts = {m: cape_only(w, stats=True)[-1]
+              for m, w in (('plain', False), ('cold_loop', 'cold'),
+                           ('extrapolation', 'extrapolation'), ('warm', True))}
     mu_levels = int((inp['p'] >= 500.0).sum().item()) + ncol   # levels scanned at p >= 500 hPa
-    # the plain method's work is credited to both kernels
-    ops = algorithmic_ops(st_plain, NLEV, ncol, mu_levels)
+    ops = algorithmic_ops(counts['plain'], NLEV, mu_levels)
 
-    outs_p, ms_plain, _ = timed(False)
+    log('[bench] timing the CAPE-only launches')
+    ms_cape = {}
+    for m, w in (('plain', False), ('cold_loop', 'cold'), ('extrapolation', 'extrapolation'),
+                 ('warm', True)):
+        _, ms_cape[m], _ = timed(lambda: cape_only(w), args.steps, args.warmup, stream)
+
+    log('[bench] timing the combined step')
+    outs_p, ms_plain, _ = timed(lambda: step(False), args.steps, args.warmup, stream)
     warm = not args.plain
-    outs, ms, wall = timed(warm) if warm else (outs_p, ms_plain, None)
+    if warm:
+        outs, ms, wall = timed(lambda: step(True), args.steps, args.warmup, stream)
+    else:
+        outs, ms, wall = outs_p, ms_plain, None
     value = ncol / (ms / 1e3)
-    achieved = ops / (ms / 1e3) / 1e12
-    acc = accuracy(outs, inp, 2048)
-    acc_plain = accuracy(outs_p, inp, 2048)
+
+    log('[bench] accuracy')
+    sample = 2048
+    sl = [x[:sample] for x in cape_in]
+    fp64 = _hip.cape(*sl, pinc=PINC, source=SOURCE, adiabat=ADIABAT, fp64=True)
+    acc = accuracy(outs, inp, sample, fp64)
+    acc_plain = accuracy(outs_p, inp, sample, fp64)
+
+    # the dominant kernel: the ascent (CAPE-only launch of the reported method)
+    dom = 'warm' if warm else 'plain'
+    achieved = ops / (ms_cape[dom] / 1e3) / 1e12
+    traffic = None
+    if os.path.exists(PMC_FILE):
+        with open(PMC_FILE) as f:
+            pmc = json.load(f)
+        if pmc.get('ncol') == ncol and pmc.get('fast_math') == bool(args.fast_math):
+            traffic = pmc.get(f'{dom}_hbm_bytes_per_launch')
+
     cpu = None
     if not args.no_cpu:
         threads = len(os.sched_getaffinity(0))
         nhost = min(ncol, args.cpu_sample)
         host = {k: v[:nhost].cpu().numpy() for k, v in inp.items()}
-        print(f'[bench] CPU baseline on {threads} threads, {args.cpu_seconds} s', file=sys.stderr,
-              flush=True)
+        log(f'[bench] CPU baseline on {threads} threads, {args.cpu_seconds} s')
         cpu = cpu_baseline(host, threads, args.cpu_seconds)
     del a, inp
 
@@ -220,19 +311,22 @@ def main():
         'data': 'synthetic (seeded soundings and winds generated in HBM)',
         'config': {'workload': 'combined: 2048x1024 columns x 137 sigma levels, most-unstable, '
                                'pseudo-ice, pinc 500 Pa, + calc_srh depth 3000 m with Bunkers '
-                               'motion, fused kernel',
+                               'motion (xcape_cape_srh_f32)',
                    'ncol': ncol, 'nlev': NLEV, 'warm_start': warm,
                    'fast_math': bool(args.fast_math), 'parallelism': 'single GPU'},
-        'roofline': {'bound': 'mfma', 'pipe': 'VALU FP32 (no matrix work in the plain method)',
+        'roofline': {'bound': 'mfma', 'pipe': 'VALU FP32 (the ascent has no matrix work)',
+                     'kernel': f'cape_grid_kernel ({dom}), CAPE-only launch',
                      'achieved': achieved, 'peak': FP32_PEAK_TFLOPS, 'unit': 'TFLOP/s',
-                     'frac': achieved / FP32_PEAK_TFLOPS, 'traffic': None,
-                     'ops_per_launch': ops,
+                     'frac': achieved / FP32_PEAK_TFLOPS, 'traffic': traffic,
+                     'ops_per_launch': ops, 'ms_per_launch': ms_cape[dom],
                      'hbm_gbs_algorithmic': bytes_per_column(NLEV) * value / 1e9},
         'plain_kernel': {'value': ncol / (ms_plain / 1e3), 'ms_per_step': ms_plain,
-                         'counts': st_plain, 'accuracy_vs_oracle': acc_plain},
+                         'accuracy': acc_plain},
         'speedup_vs_plain': ms_plain / ms,
-        'kernel_counts': st_warm if warm else st_plain,
-        'accuracy_vs_oracle': acc,
+        'attribution_cape_only_ms': ms_cape,
+        'kernel_counts': counts,
+        'surrogate': sur,
+        'accuracy': acc,
         'cpu_baseline': cpu,
     }
     print(json.dumps(line))

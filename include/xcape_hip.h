@@ -46,6 +46,10 @@ extern "C" {
 
 #define XCAPE_NO_SURROGATE 0x8u /* with XCAPE_WARM_START: ignore a loaded
                                    surrogate (extrapolation only)              */
+#define XCAPE_NO_PREDICTION 0x20u /* with XCAPE_WARM_START: run the warm
+                                   start's loop with every sub-step started
+                                   cold (no prediction, no surrogate); for
+                                   measuring what the prediction itself gains */
 #define XCAPE_LEVEL_MAJOR 0x10u /* level-major profiles: the (nk, ncol) arrays
                                    (p on a sigma grid, t, td, u, v) hold level
                                    j of column i at [j * ncol + i] instead of

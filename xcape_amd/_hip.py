@@ -14,13 +14,14 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, 'libxcape_hip.so')
+LIB_PATH = os.environ.get('XCAPE_HIP_LIB', os.path.join(HERE, 'libxcape_hip.so'))
 
 XCAPE_DEVICE_PTRS = 0x1
 XCAPE_FAST_MATH = 0x2
 XCAPE_WARM_START = 0x4
 XCAPE_NO_SURROGATE = 0x8
 XCAPE_LEVEL_MAJOR = 0x10
+XCAPE_NO_PREDICTION = 0x20
 
 
 class XcapeError(RuntimeError):
@@ -188,13 +189,18 @@ class _Args:
 
 
 def _warm_flags(warm_start):
-    """warm_start: False (cold), True (extrapolation + loaded surrogate) or
-    'extrapolation' (ignore a loaded surrogate)."""
+    """warm_start: False (the reference's loops, cold), True (extrapolation +
+    loaded surrogate), 'extrapolation' (ignore a loaded surrogate) or 'cold'
+    (the warm start's loop with every sub-step cold)."""
     if not warm_start:
         return 0
     f = XCAPE_WARM_START
     if warm_start == 'extrapolation':
         f |= XCAPE_NO_SURROGATE
+    elif warm_start == 'cold':
+        f |= XCAPE_NO_SURROGATE | XCAPE_NO_PREDICTION
+    elif warm_start is not True:
+        raise ValueError(f'warm_start={warm_start!r}')
     return f
 
 

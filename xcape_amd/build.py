@@ -31,8 +31,10 @@ COMMON = ['--offload-arch=' + ARCH, '-O3', '-std=c++17', '-fPIC',
 # translation unit -> extra flags
 UNITS = {
     'capi.hip': ['-ffp-contract=off'],
-    'cape_accurate.hip': ['-ffp-contract=off'],
-    'cape_fast.hip': ['-ffp-contract=fast-honor-pragmas'],
+    # MFMA operands and accumulators in VGPRs (not AGPRs): the surrogate's
+    # evaluation shares the ascent kernel's register budget
+    'cape_accurate.hip': ['-ffp-contract=off', '-mllvm', '-amdgpu-mfma-vgpr-form=1'],
+    'cape_fast.hip': ['-ffp-contract=fast-honor-pragmas', '-mllvm', '-amdgpu-mfma-vgpr-form=1'],
     'ragged.hip': ['-ffp-contract=off'],
     'mlp.hip': ['-ffp-contract=fast'],
     'surrogate_log.hip': ['-ffp-contract=off'],
@@ -40,9 +42,9 @@ UNITS = {
 }
 
 
-def _compile(unit, flags, outdir):
+def _compile(unit, flags, outdir, csrc=CSRC):
     obj = os.path.join(outdir, unit.replace('.hip', '.o'))
-    cmd = [HIPCC] + COMMON + flags + ['-c', os.path.join(CSRC, unit), '-o', obj]
+    cmd = [HIPCC] + COMMON + flags + ['-c', os.path.join(csrc, unit), '-o', obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     noise = 'packed-fp32-ops'
     err = '\n'.join(l for l in r.stderr.splitlines() if noise not in l)
@@ -53,12 +55,13 @@ def _compile(unit, flags, outdir):
     return obj
 
 
-def build(verbose=True):
-    """Compile every unit and link the shared library (always rebuilds)."""
+def build(verbose=True, csrc=CSRC, out=LIB):
+    """Compile every unit of `csrc` and link the shared library `out`
+    (always rebuilds)."""
     with tempfile.TemporaryDirectory(prefix='xcape_build_') as tmp:
         with ThreadPoolExecutor(max_workers=min(8, len(UNITS))) as ex:
-            objs = list(ex.map(lambda kv: _compile(kv[0], kv[1], tmp), UNITS.items()))
-        out_tmp = LIB + '.tmp'
+            objs = list(ex.map(lambda kv: _compile(kv[0], kv[1], tmp, csrc), UNITS.items()))
+        out_tmp = out + '.tmp'
         # Link without a DT_NEEDED on libamdhip64: the HIP symbols bind to the
         # runtime already loaded with global scope (the process's only one;
         # xcape_amd._hip arranges that, a C caller links -lamdhip64).  A hard
@@ -68,10 +71,10 @@ def build(verbose=True):
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError('link failed:\n' + r.stderr)
-        shutil.move(out_tmp, LIB)
+        shutil.move(out_tmp, out)
     if verbose:
-        print('built', LIB)
-    return LIB
+        print('built', out)
+    return out
 
 
 if __name__ == '__main__':

@@ -171,9 +171,11 @@ XC_HD __forceinline__ bool solve_substep(T p1, T p2, T pi2, T t1, T th1, T qv1, 
 // Parcel source (:257-379): chooses the starting level kmax, the parcel's
 // initial state and the height of kmax.  Returns false for an unknown source
 // (unreachable through the Python layer; the reference stops there).
+// The parcel starts at the pressure of kmax, so its p2 and pi2 are e.p and
+// e.pi (not stored twice).
 template <typename T> struct Parcel {
     int kmax, mulvl;
-    T th2, pi2, p2, t2, qv2, b2, zk;
+    T th2, t2, qv2, b2, zk;
     Env<T> e;       // environment at kmax
 };
 
@@ -284,14 +286,13 @@ XC_HD __forceinline__ bool parcel_source(const C &col, int nk, int source, T ml_
     // parcel at its initial location (:355-375)
     pc.e = env_at<T, MATH>(col, kmax);
     if (source != SRC_ML) {
-        pc.th2 = pc.e.th; pc.pi2 = pc.e.pi; pc.p2 = pc.e.p; pc.t2 = pc.e.t; pc.qv2 = pc.e.q;
+        pc.th2 = pc.e.th; pc.t2 = pc.e.t; pc.qv2 = pc.e.q;
         pc.b2 = T(0.0);
     } else {
         pc.th2 = (T)avgth_d;
         pc.qv2 = (T)avgqv_d;
         T thv2 = F::div(pc.th2 * (T(1.0) + K::reps * pc.qv2), T(1.0) + pc.qv2);
-        pc.pi2 = pc.e.pi; pc.p2 = pc.e.p;
-        pc.t2 = pc.th2 * pc.pi2;
+        pc.t2 = pc.th2 * pc.e.pi;
         pc.b2 = F::div(K::g * (thv2 - pc.e.thv), pc.e.thv);
     }
     pc.kmax = kmax;
@@ -365,7 +366,7 @@ XC_HD void cape_column(const C &col, int nk, T pinc, int source, T ml_depth,
     }
     const int mulvl = pc.mulvl;
     T zout = T(-999999.0);
-    T th2 = pc.th2, pi2 = pc.pi2, p2 = pc.p2, t2 = pc.t2, qv2 = pc.qv2, b2 = pc.b2, zk = pc.zk;
+    T th2 = pc.th2, pi2 = pc.e.pi, p2 = pc.e.p, t2 = pc.t2, qv2 = pc.qv2, b2 = pc.b2, zk = pc.zk;
     T ql2 = T(0.0), qi2 = T(0.0), qt = qv2;
     T cape = T(0.0), cin = T(0.0), narea = T(0.0);
     T p_prev = pc.e.p, pi_prev = pc.e.pi, thv_prev = pc.e.thv;
@@ -445,17 +446,19 @@ XC_HD void cape_column(const C &col, int nk, T pinc, int source, T ml_depth,
 // on its own (the set-up blocks execute under the mask of the lanes that need
 // them).  The wave runs until its longest column is done.
 // `pc` comes from parcel_source (the caller may have used it to build the
-// first sub-step's surrogate inputs).
+// first sub-step's surrogate inputs).  predict = false starts every sub-step
+// cold: the same loop without prediction, which separates the loop's share
+// of the speed-up from the prediction's.
 template <typename T, int MATH, bool ICE, bool PSEUDO, class C>
 XC_HD void cape_column_warm(const C &col, int nk, T pinc, const Parcel<T> &pc,
                                  T first_guess, T fallback_tol, ColumnOut<T> &out,
-                                 LaneStats &st)
+                                 LaneStats &st, bool predict = true)
 {
     using F = Fn<T, MATH>;
     using K = Phys<T>;
     const int mulvl = pc.mulvl;
     T zout = T(-999999.0);
-    T th2 = pc.th2, pi2 = pc.pi2, p2 = pc.p2, t2 = pc.t2, qv2 = pc.qv2, b2 = pc.b2, zk = pc.zk;
+    T th2 = pc.th2, pi2 = pc.e.pi, p2 = pc.e.p, t2 = pc.t2, qv2 = pc.qv2, b2 = pc.b2, zk = pc.zk;
     T ql2 = T(0.0), qi2 = T(0.0), qt = qv2;
     T cape = T(0.0), cin = T(0.0), narea = T(0.0);
     // the previous level's environment: p in e.p until the next level
@@ -513,7 +516,9 @@ XC_HD void cape_column_warm(const C &col, int nk, T pinc, const Parcel<T> &pc,
             pi2 = F::pow(p2 * K::rp00, K::rddcp);
             logp = F::log(F::div(p2, p1));
             T g = th1;
-            if (nh >= 2) {
+            if (!predict) {
+                // every sub-step cold: the flattened loop without prediction
+            } else if (nh >= 2) {
                 const T r = r1 + (r1 - r2) * F::div(logp + dl1, dl1 + dl2);
                 g = th1 + r * logp;
             } else if (nh == 1) {

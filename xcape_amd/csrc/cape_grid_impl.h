@@ -125,87 +125,100 @@ __device__ __forceinline__ bool column_at(const CapeGridArgs &a, int64_t col, Si
     return true;
 }
 
-// CAPE/CIN, one column per lane.  WARM: the warm-started ascent
-// (cape_column_warm); a.guess, when set, holds each column's first-sub-step
-// iterate from the surrogate (NaN: none).  STATS: accumulate the work
-// counters into a.stats (a separate instantiation, so that the counters do
-// not occupy registers in the timed kernels).
-template <typename T, typename IDX, bool ICE, bool PSEUDO, bool WARM, bool STATS>
+// First-sub-step inputs of the surrogate for a parcel: ln p2, the parcel's
+// theta-e at its start (the reference's parcel formula, getthe(p, t, t, qv),
+// :505) and its total water, for the sub-step p -> p - dp (p = pc.e.p) of the
+// first ascent level (:408-429); lp1 = ln(p2/p1) of that sub-step (0: none).
+template <typename T, class C>
+__device__ __forceinline__ void surrogate_inputs(const C &c, int nk, const Parcel<T> &pc,
+                                                 float pinc, float &f0, float &f1, float &f2,
+                                                 T &lp1)
+{
+    using F = Fn<T, XC_MATH>;
+    const Env<T> e1 = env_from<T, XC_MATH>(c.raw_above(pc.kmax + 1));
+    T dp = pc.e.p - e1.p;
+    const int nl = substep_count<T, XC_MATH>(dp, T(pinc));
+    const T p2 = pc.e.p - dp;
+    if (nl >= 1 && p2 > T(0.0)) {
+        lp1 = F::log(F::div(p2, pc.e.p));
+        f0 = (float)F::log(p2);
+        f1 = (float)thetae<T, XC_MATH>(pc.e.p, pc.t2, pc.t2, pc.qv2);
+        f2 = (float)pc.qv2;
+    }
+}
+
+// CAPE/CIN, one column per lane.  MODE: 0 the reference's loops with the
+// cold start (cape_column); 1 the warm-started ascent (cape_column_warm);
+// 2 the same loop with every sub-step cold (to separate the loop's share of
+// the speed-up from the prediction's).  SUR (with MODE 1): a surrogate is
+// loaded; every wave first evaluates it for its 64 columns' first sub-steps
+// (one wave-wide BF16 MFMA evaluation, a point every lane of the block
+// reaches), and each lane starts its ascent from that iterate.  STATS:
+// accumulate the work counters into a.stats (a separate instantiation, so
+// that the counters do not occupy registers in the timed kernels).
+template <typename T, typename IDX, bool ICE, bool PSEUDO, int MODE, bool SUR, bool STATS>
 __global__ void __launch_bounds__(256) cape_grid_kernel(CapeGridArgs a)
 {
+    extern __shared__ uint32_t dyn_lds[];
     const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    LaneStats st;
-    bool ran = false;
     SigmaCol<IDX> c{};
     int nk = 0;
     const bool in_grid = column_at(a, col, c, nk);
     T *cape = static_cast<T *>(a.cape), *cin = static_cast<T *>(a.cin),
       *zout = static_cast<T *>(a.zout);
-    if (in_grid) {
-        ColumnOut<T> o;
-        if (c.ts > 0.0f) {
-            ran = true;
-            if constexpr (WARM) {
-                Parcel<T> pc;
-                const bool ok = parcel_source<T, XC_MATH>(c, nk, a.source, T(a.ml_depth), pc);
-                // stored before the ascent: one register less through it
-                a.mulvl[col] = pc.mulvl;
-                if (ok) {
-                    const T g = a.guess ? T(a.guess[col]) : T(NAN);
-                    cape_column_warm<T, XC_MATH, ICE, PSEUDO>(c, nk, T(a.pinc), pc, g,
-                                                              T(a.fallback_tol), o, st);
-                } else {
-                    o.cape = T(0); o.cin = T(0); o.zout = T(-999999.0);
-                }
+    if constexpr (MODE == 0) {
+        LaneStats st;
+        bool ran = false;
+        if (in_grid) {
+            ColumnOut<T> o;
+            if (c.ts > 0.0f) {
+                ran = true;
+                cape_column<T, XC_MATH, ICE, PSEUDO>(c, nk, T(a.pinc), a.source, T(a.ml_depth),
+                                                     o, st);
             } else {
-                cape_column<T, XC_MATH, ICE, PSEUDO>(c, nk, T(a.pinc), a.source,
-                                                     T(a.ml_depth), o, st);
-                a.mulvl[col] = o.mulvl;
+                o.cape = T(0); o.cin = T(0); o.zout = T(0); o.mulvl = 0;
             }
-        } else {
-            o.cape = T(0); o.cin = T(0); o.zout = T(0);
-            a.mulvl[col] = 0;
+            cape[col] = o.cape;
+            cin[col] = o.cin;
+            a.mulvl[col] = o.mulvl;
+            zout[col] = o.zout;
         }
-        cape[col] = o.cape;
-        cin[col] = o.cin;
-        zout[col] = o.zout;
-    }
-    if constexpr (STATS) block_stats_add(st, ran, a.stats);
-}
-
-// The surrogate's first-sub-step iterate of every column (NaN where the
-// column has no ascent): the parcel set-up of the ascent, the network's
-// inputs for the sub-step pc.p2 -> pc.p2 - dp (CAPE_CODE_model_lev.f90:408-429)
-// and one wave-wide MFMA evaluation per 64 columns.  Every lane of the block
-// reaches the evaluation.
-__global__ void __launch_bounds__(256) surrogate_guess_kernel(CapeGridArgs a)
-{
-    extern __shared__ uint32_t dyn_lds[];
-    using T = float;
-    using F = Fn<T, XC_MATH>;
-    const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    SigmaCol<uint64_t> c{};
-    int nk = 0;
-    const bool in_grid = column_at(a, col, c, nk);
-    Parcel<T> pc;
-    T lp1 = T(0.0);
-    float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
-    if (in_grid && c.ts > 0.0f && parcel_source<T, XC_MATH>(c, nk, a.source, T(a.ml_depth), pc) &&
-        pc.kmax < nk - 1) {
-        const Env<T> e1 = env_at<T, XC_MATH>(c, pc.kmax + 1);
-        T dp = pc.e.p - e1.p;
-        const int nl = substep_count<T, XC_MATH>(dp, T(a.pinc));
-        const T p2 = pc.p2 - dp;
-        if (nl >= 1 && p2 > T(0.0)) {
-            lp1 = F::log(F::div(p2, pc.p2));
-            f0 = F::log(p2);
-            f1 = thetae<T, XC_MATH>(pc.p2, pc.t2, pc.t2, pc.qv2);
-            f2 = pc.qv2;
+        if constexpr (STATS) block_stats_add(st, ran, a.stats);
+    } else {
+        const bool ran = in_grid && c.ts > 0.0f;
+        Parcel<T> pc;
+        bool ok = false;
+        T lp1 = T(0.0);
+        float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
+        if (ran) {
+            ok = parcel_source<T, XC_MATH>(c, nk, a.source, T(a.ml_depth), pc);
+            // stored before the ascent: one register less through it
+            a.mulvl[col] = pc.mulvl;
+            if (SUR && ok && pc.kmax < nk - 1) surrogate_inputs(c, nk, pc, a.pinc, f0, f1, f2, lp1);
         }
+        float r = 0.0f;
+        if constexpr (SUR)
+            r = surrogate_eval_wave(static_cast<const SurrogateDev *>(a.surrogate), f0, f1, f2,
+                                    dyn_lds + (threadIdx.x >> 6) * MLP_LDS_WORDS_PER_WAVE);
+        LaneStats st;
+        if (in_grid) {
+            ColumnOut<T> o{};
+            if (ok) {
+                const T g = lp1 != T(0.0) ? pc.th2 + T(r) * lp1 : T(NAN);
+                cape_column_warm<T, XC_MATH, ICE, PSEUDO>(c, nk, T(a.pinc), pc, g,
+                                                          T(a.fallback_tol), o, st, MODE == 1);
+            } else if (ran) {
+                // unknown source (unreachable through the C-ABI's checks)
+                o.zout = T(-999999.0);
+            } else {
+                a.mulvl[col] = 0;
+            }
+            cape[col] = o.cape;
+            cin[col] = o.cin;
+            zout[col] = o.zout;
+        }
+        if constexpr (STATS) block_stats_add(st, ran, a.stats);
     }
-    const float r = surrogate_eval_wave(static_cast<const SurrogateDev *>(a.surrogate), f0, f1,
-                                        f2, dyn_lds + (threadIdx.x >> 6) * MLP_LDS_WORDS_PER_WAVE);
-    if (in_grid) a.guess_out[col] = lp1 != T(0.0) ? pc.th2 + r * lp1 : T(NAN);
 }
 
 // Storm-relative helicity chain, one column per lane.
@@ -231,15 +244,16 @@ __global__ void __launch_bounds__(256) srh_grid_kernel(CapeGridArgs a)
     }
 }
 
-template <typename T, typename IDX, bool WARM, bool STATS>
+template <typename T, typename IDX, int MODE, bool SUR, bool STATS>
 hipError_t launch_cape_s(const CapeGridArgs &a, int block, hipStream_t s)
 {
     dim3 grid((unsigned)((a.ncol + block - 1) / block));
+    const size_t lds = SUR ? sizeof(uint32_t) * MLP_LDS_WORDS_PER_WAVE * (size_t)(block / 64) : 0;
     switch (a.adiabat) {
-    case 1: cape_grid_kernel<T, IDX, false, true, WARM, STATS><<<grid, block, 0, s>>>(a); break;
-    case 2: cape_grid_kernel<T, IDX, false, false, WARM, STATS><<<grid, block, 0, s>>>(a); break;
-    case 3: cape_grid_kernel<T, IDX, true, true, WARM, STATS><<<grid, block, 0, s>>>(a); break;
-    case 4: cape_grid_kernel<T, IDX, true, false, WARM, STATS><<<grid, block, 0, s>>>(a); break;
+    case 1: cape_grid_kernel<T, IDX, false, true, MODE, SUR, STATS><<<grid, block, lds, s>>>(a); break;
+    case 2: cape_grid_kernel<T, IDX, false, false, MODE, SUR, STATS><<<grid, block, lds, s>>>(a); break;
+    case 3: cape_grid_kernel<T, IDX, true, true, MODE, SUR, STATS><<<grid, block, lds, s>>>(a); break;
+    case 4: cape_grid_kernel<T, IDX, true, false, MODE, SUR, STATS><<<grid, block, lds, s>>>(a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -247,13 +261,13 @@ hipError_t launch_cape_s(const CapeGridArgs &a, int block, hipStream_t s)
 
 // 32-bit element offsets when every index of the profile arrays fits (the
 // stats launches, which only count work, always use 64-bit ones)
-template <typename T, bool WARM>
+template <typename T, int MODE, bool SUR>
 hipError_t launch_cape(const CapeGridArgs &a, int block, hipStream_t s)
 {
-    if (a.stats) return launch_cape_s<T, uint64_t, WARM, true>(a, block, s);
+    if (a.stats) return launch_cape_s<T, uint64_t, MODE, SUR, true>(a, block, s);
     const bool fits = (uint64_t)a.ncol * (uint64_t)a.nk_in < ((uint64_t)1 << 32);
-    return fits ? launch_cape_s<T, uint32_t, WARM, false>(a, block, s)
-                : launch_cape_s<T, uint64_t, WARM, false>(a, block, s);
+    return fits ? launch_cape_s<T, uint32_t, MODE, SUR, false>(a, block, s)
+                : launch_cape_s<T, uint64_t, MODE, SUR, false>(a, block, s);
 }
 
 template <typename T>
@@ -263,20 +277,17 @@ hipError_t launch_t(const CapeGridArgs &a, int kind, int block, hipStream_t s)
     if constexpr (sizeof(T) == 8) {
         // the FP64 yardstick covers CAPE with the reference's cold start only
         if (kind != GRID_CAPE || a.warm) return hipErrorInvalidValue;
-        return launch_cape<T, false>(a, block, s);
+        return launch_cape<T, 0, false>(a, block, s);
     } else {
         dim3 grid((unsigned)((a.ncol + block - 1) / block));
         switch (kind) {
         case GRID_CAPE:
-            return a.warm ? launch_cape<T, true>(a, block, s) : launch_cape<T, false>(a, block, s);
+            if (a.warm == 0) return launch_cape<T, 0, false>(a, block, s);
+            if (a.warm == 2) return launch_cape<T, 2, false>(a, block, s);
+            return a.surrogate ? launch_cape<T, 1, true>(a, block, s)
+                               : launch_cape<T, 1, false>(a, block, s);
         case GRID_SRH:
             srh_grid_kernel<<<grid, block, 0, s>>>(a);
-            return hipGetLastError();
-        case GRID_GUESS:
-            if (!a.surrogate || !a.guess_out) return hipErrorInvalidValue;
-            surrogate_guess_kernel<<<grid, block,
-                                     sizeof(uint32_t) * MLP_LDS_WORDS_PER_WAVE * (size_t)(block / 64),
-                                     s>>>(a);
             return hipGetLastError();
         default: return hipErrorInvalidValue;
         }

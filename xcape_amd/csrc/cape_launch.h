@@ -30,15 +30,14 @@ struct CapeGridArgs {
     double *srh_rm, *srh_lm;     // (ncol,)
     float *rm, *lm, *mean6;      // (ncol, 2) u, v pairs, or nullptr
     // warm start of the moist-adiabat iteration
-    int warm;                    // 0: reference cold start
+    int warm;                    // 0: reference loops, cold start; 1: warm-started
+                                 // loop; 2: the same loop, every sub-step cold
     float fallback_tol;          // first-iteration residual that rejects a guess [K]
     const void *surrogate;       // device MLP weights (or nullptr)
-    const float *guess;          // (ncol,) first-sub-step iterates for the warm ascent, or nullptr
-    float *guess_out;            // (ncol,) written by the GRID_GUESS launch
 };
 
 // kind of work a grid launch does
-enum { GRID_CAPE = 1, GRID_SRH = 2, GRID_CAPE_SRH = 3, GRID_GUESS = 4 };
+enum { GRID_CAPE = 1, GRID_SRH = 2, GRID_CAPE_SRH = 3 };
 
 // (nk, ncol) level-major copies of column-contiguous (ncol, nk) arrays:
 // dst[j][i] = src[i][j] for up to 5 arrays, LDS-tiled so that both the

@@ -120,7 +120,8 @@ int cape_block(bool warm)
         return (b == 64 || b == 128 || b == 256) ? b : 0;
     }();
     if (env) return env;
-    return warm ? 64 : 256;
+    (void)warm;
+    return 256;
 }
 
 // The helicity chain.  Level-major profiles go straight to the kernel;
@@ -200,6 +201,9 @@ int grid_entry(const GridCall &c)
                          (c.rm != nullptr) != (c.mean6 != nullptr)))
             return fail(XCAPE_EINVAL, "rm, lm and mean6 must be all NULL or all set");
     }
+    const uint32_t known = XCAPE_DEVICE_PTRS | XCAPE_FAST_MATH | XCAPE_WARM_START |
+                           XCAPE_NO_SURROGATE | XCAPE_LEVEL_MAJOR | XCAPE_NO_PREDICTION;
+    if (c.flags & ~known) return fail(XCAPE_EINVAL, "unknown flags 0x%x", c.flags & ~known);
     if (c.fp64 && (c.flags & (XCAPE_FAST_MATH | XCAPE_WARM_START)))
         return fail(XCAPE_EINVAL, "the FP64 entry point supports neither FAST_MATH nor WARM_START");
     if (!(c.flags & XCAPE_DEVICE_PTRS) && c.p_shared && c.start) {
@@ -235,7 +239,7 @@ int grid_entry(const GridCall &c)
     a.source = c.source;
     a.adiabat = c.adiabat;
     a.depth = c.depth;
-    a.warm = (c.flags & XCAPE_WARM_START) ? 1 : 0;
+    a.warm = (c.flags & XCAPE_WARM_START) ? ((c.flags & XCAPE_NO_PREDICTION) ? 2 : 1) : 0;
     a.fallback_tol = 1.0e-3f;
     a.surrogate = (a.warm && !(c.flags & XCAPE_NO_SURROGATE)) ? surrogate_device() : nullptr;
     hipError_t e = hipSuccess;
@@ -287,24 +291,15 @@ int grid_entry(const GridCall &c)
         XC_TRY(zero_counters(dstats_.ptr, ST_COUNT, s));
         a.stats = dstats_.ptr;
     }
-    DevBuf<float> guess_;
     if (want_cape) {
-        const bool fast = c.flags & XCAPE_FAST_MATH;
-        auto launch = [&](int kind, int block) {
-            return c.fp64 ? launch_cape_grid_accurate(a, true, kind, block, s)
-                          : fast ? launch_cape_grid_fast(a, kind, block, s)
-                                 : launch_cape_grid_accurate(a, false, kind, block, s);
-        };
-        if (a.warm && a.surrogate) {
-            // the surrogate's first iterates, one MFMA evaluation per wave,
-            // in a launch of their own (the ascent keeps its register budget)
-            XC_TRY(guess_.alloc(ncol, s));
-            a.guess_out = guess_.ptr;
-            if ((e = launch(GRID_GUESS, 256)) != hipSuccess) return hip_fail(e, "surrogate launch");
-            a.guess = guess_.ptr;
-        }
-        if ((e = launch(GRID_CAPE, cape_block(a.warm))) != hipSuccess)
-            return hip_fail(e, "CAPE launch");
+        const int block = cape_block(a.warm);
+        if (c.fp64)
+            e = launch_cape_grid_accurate(a, true, GRID_CAPE, block, s);
+        else if (c.flags & XCAPE_FAST_MATH)
+            e = launch_cape_grid_fast(a, GRID_CAPE, block, s);
+        else
+            e = launch_cape_grid_accurate(a, false, GRID_CAPE, block, s);
+        if (e != hipSuccess) return hip_fail(e, "CAPE launch");
     }
     if (want_srh && (e = run_srh(a, s)) != hipSuccess) return hip_fail(e, "helicity launch");
     if (!dev) {

@@ -81,77 +81,83 @@ __device__ __forceinline__ float gelu_grad_f(float z)
 typedef __bf16 xc_bf16x8 __attribute__((ext_vector_type(8)));
 typedef float xc_f32x16 __attribute__((ext_vector_type(16)));
 
-// LDS the wave needs for one evaluation: 64 samples x 64 hidden-1 values as
-// BF16, rows padded to 33 words so that 64 lanes writing their own rows hit
-// distinct banks.
+// LDS the wave needs for one evaluation: the BF16 hidden-1 rows of 32
+// samples (one half of the wave at a time), padded to 33 words per row so
+// that 32 lanes writing their own rows hit distinct banks.  4.2 KB per wave,
+// which keeps 8 waves per SIMD within the CU's LDS when the evaluation
+// shares its kernel with the ascent.
 constexpr int MLP_LDS_ROW_WORDS = 33;
-constexpr int MLP_LDS_WORDS_PER_WAVE = 64 * MLP_LDS_ROW_WORDS;
+constexpr int MLP_LDS_WORDS_PER_WAVE = 32 * MLP_LDS_ROW_WORDS;
 
 // Evaluate the network for the 64 samples of a wavefront (one per lane).
-// Must be called by all 64 lanes of the wave together (the MFMA and the LDS
-// exchange use every lane); `lds` points to this wave's
-// MLP_LDS_WORDS_PER_WAVE words.  `sync` is a block-wide barrier the caller
-// provides (every wave of the block calls this function once).
+// Must be called by every lane of the block together (the MFMA uses all 64
+// lanes of a wave and the LDS hand-over uses block barriers); `lds` points
+// to this wave's MLP_LDS_WORDS_PER_WAVE words.
 //
 // Layer 2 runs as v_mfma_f32_32x32x16_bf16: Z2^T (hidden-2 x samples) =
 // W2 (hidden-2 x hidden-1) . H1^T (hidden-1 x samples), four 32x32 output
-// tiles, each accumulated over 4 K-steps of 16.  Operand layout per lane l:
-// A[m = l%32][k = 8(l/32) .. +7], B[k = 8(l/32) .. +7][n = l%32]; result
-// register r of lane l is C[8(r/4) + 4(l/32) + r%4][l%32].
+// tiles (tm: hidden-2 rows, tn: samples), each accumulated over 4 K-steps of
+// 16.  Operand layout per lane l: A[m = l%32][k = 8(l/32) .. +7],
+// B[k = 8(l/32) .. +7][n = l%32]; result register r of lane l is
+// C[8(r/4) + 4(l/32) + r%4][l%32].  Phase tn stages the hidden-1 rows of
+// samples 32 tn .. 32 tn + 31 (computed by the lanes of half tn).  Tiles and
+// K-steps are not unrolled, so the live set is one accumulator and two
+// operands (the ascent that follows is register-bound).
 __device__ __forceinline__ float surrogate_eval_wave(const SurrogateDev *S, float x0, float x1,
                                                      float x2, uint32_t *lds)
 {
     const int lane = threadIdx.x & 63;
-    // ---- layer 1 on the VALU: this lane's sample --------------------------
+    const int half = lane >> 5, col = lane & 31;
     const float u0 = (x0 - S->in_mean[0]) * S->in_scale[0];
     const float u1 = (x1 - S->in_mean[1]) * S->in_scale[1];
     const float u2 = (x2 - S->in_mean[2]) * S->in_scale[2];
-    uint32_t *row = lds + lane * MLP_LDS_ROW_WORDS;
-#pragma unroll 4
-    for (int j = 0; j < MLP_H; j += 2) {
-        const float za = S->b1[j] + S->w1[3 * j] * u0 + S->w1[3 * j + 1] * u1 +
-                         S->w1[3 * j + 2] * u2;
-        const float zb = S->b1[j + 1] + S->w1[3 * j + 3] * u0 + S->w1[3 * j + 4] * u1 +
-                         S->w1[3 * j + 5] * u2;
-        row[j >> 1] = (uint32_t)bf16_bits(gelu_f(za)) | ((uint32_t)bf16_bits(gelu_f(zb)) << 16);
-    }
-    __syncthreads();
-    // ---- layer 2 (MFMA) + layer 3 ------------------------------------------
-    const int half = lane >> 5, col = lane & 31;
-    float out[2] = {0.0f, 0.0f};
-#pragma unroll
+    float out0 = 0.0f, out1 = 0.0f;
+#pragma unroll 1
     for (int tn = 0; tn < 2; ++tn) {
-        float part = 0.0f;
-#pragma unroll
+        // ---- layer 1 on the VALU: the lanes of half tn write their rows ----
+        if (half == tn) {
+            uint32_t *row = lds + col * MLP_LDS_ROW_WORDS;
+#pragma unroll 4
+            for (int j = 0; j < MLP_H; j += 2) {
+                const float za = S->b1[j] + S->w1[3 * j] * u0 + S->w1[3 * j + 1] * u1 +
+                                 S->w1[3 * j + 2] * u2;
+                const float zb = S->b1[j + 1] + S->w1[3 * j + 3] * u0 + S->w1[3 * j + 4] * u1 +
+                                 S->w1[3 * j + 5] * u2;
+                row[j >> 1] = (uint32_t)bf16_bits(gelu_f(za)) |
+                              ((uint32_t)bf16_bits(gelu_f(zb)) << 16);
+            }
+        }
+        __syncthreads();
+        // ---- layer 2 (MFMA) + layer 3 for the two tiles of these samples ----
+#pragma unroll 1
         for (int tm = 0; tm < 2; ++tm) {
             xc_f32x16 acc = {};
-#pragma unroll
+#pragma unroll 1
             for (int kb = 0; kb < 4; ++kb) {
                 const int k0 = 16 * kb + 8 * half;
-                // A: W2[32 tm + col][k0 .. k0+7]
-                const uint16_t *wa = S->w2 + (32 * tm + col) * MLP_H + k0;
+                // A: W2[32 tm + col][k0 .. k0+7]; B: H1[sample 32 tn + col][k0 .. k0+7]
+                const uint4 wq =
+                    *reinterpret_cast<const uint4 *>(S->w2 + (32 * tm + col) * MLP_H + k0);
+                const uint32_t *hb = lds + col * MLP_LDS_ROW_WORDS + (k0 >> 1);
+                const uint4 hq = make_uint4(hb[0], hb[1], hb[2], hb[3]);
                 xc_bf16x8 a, b;
-                // B: H1[sample 32 tn + col][k0 .. k0+7] from LDS (4 words)
-                const uint32_t *hb = lds + (32 * tn + col) * MLP_LDS_ROW_WORDS + (k0 >> 1);
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    uint32_t wq = (uint32_t)wa[2 * q] | ((uint32_t)wa[2 * q + 1] << 16);
-                    uint32_t hq = hb[q];
-                    __builtin_memcpy((char *)&a + 4 * q, &wq, 4);
-                    __builtin_memcpy((char *)&b + 4 * q, &hq, 4);
-                }
+                __builtin_memcpy(&a, &wq, 16);
+                __builtin_memcpy(&b, &hq, 16);
                 acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
             }
+            float part = 0.0f;
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const int i = 32 * tm + 8 * (r >> 2) + 4 * half + (r & 3);
                 part += S->w3[i] * gelu_f(acc[r] + S->b2[i]);
             }
+            if (tn == 0) out0 += part; else out1 += part;
         }
-        out[tn] = part + __shfl_xor(part, 32, 64);
+        __syncthreads();   // the rows are rewritten by the next phase / caller
     }
-    __syncthreads();   // the LDS rows may be reused after this call
-    const float y = (half ? out[1] : out[0]) + S->b3;
+    out0 += __shfl_xor(out0, 32, 64);
+    out1 += __shfl_xor(out1, 32, 64);
+    const float y = (half ? out1 : out0) + S->b3;
     return y * S->out_scale + S->out_offset;
 }
 

@@ -242,3 +242,48 @@ def fit(train, val, epochs=12, batch=16384, lr=3e-3, seed=0, log=None):
             'batch': batch, 'lr': lr, 'seed': seed, 'history': history,
             'seconds': time.perf_counter() - t0}
     return Surrogate(theta.cpu().numpy(), norm, meta)
+
+
+def train(train_columns=10_000_000, val_columns=1_000_000, epochs=8, batch=16384, lr=3e-3,
+          seed=2024, log=print):
+    """The warm-start surrogate of the ragged-batch recipe: first sub-steps of
+    `train_columns` ascents (seed), validated on `val_columns` ascents of
+    another seed.  Returns the Surrogate with its record in .meta."""
+    t0 = time.perf_counter()
+    tr = log_samples(train_columns, seed)
+    va = log_samples(val_columns, seed + 1)
+    if log:
+        log(f'[surrogate] {tr.shape[0]} train / {va.shape[0]} validation samples '
+            f'logged in {time.perf_counter() - t0:.1f} s')
+    s = fit(tr, va, epochs=epochs, batch=batch, lr=lr, seed=seed, log=log)
+    err = guess_error(s, va)
+    import torch
+    q = torch.quantile(err[:1_000_000].double(), torch.tensor([0.5, 0.9, 0.99], dtype=torch.float64,
+                                                           device=err.device))
+    s.meta.update({'recipe': 'first sub-steps of cold-start ascents, ragged-batch columns '
+                             '(nlev 30/40/60/91/137, SB/ML/MU, pseudo-liquid/pseudo-ice, '
+                             'pinc 500 Pa)',
+                   'train_columns': train_columns, 'val_columns': val_columns,
+                   'val_guess_error_K': {'p50': float(q[0]), 'p90': float(q[1]),
+                                         'p99': float(q[2])}})
+    if log:
+        log(f'[surrogate] validation first-iterate error [K]: {s.meta["val_guess_error_K"]}')
+    return s
+
+
+def main(argv=None):
+    import argparse
+    ap = argparse.ArgumentParser(description='train the warm-start surrogate on the GPU')
+    ap.add_argument('--out', default=DEFAULT_PATH)
+    ap.add_argument('--train-columns', type=int, default=10_000_000)
+    ap.add_argument('--val-columns', type=int, default=1_000_000)
+    ap.add_argument('--epochs', type=int, default=8)
+    ap.add_argument('--seed', type=int, default=2024)
+    args = ap.parse_args(argv)
+    s = train(args.train_columns, args.val_columns, epochs=args.epochs, seed=args.seed)
+    s.save(args.out)
+    print(f'[surrogate] saved {args.out}')
+
+
+if __name__ == '__main__':
+    main()
