@@ -150,6 +150,87 @@ def accuracy(outs, inp, sample, fp64_outs):
     return res
 
 
+def pmc_summary(fetch_dir, write_dir, out, ncol, fast_math):
+    """HBM bytes per launch of the ascent kernels from two rocprofv3 counter
+    runs of --profile-pass (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch),
+    each scaled by what the same counter reads for the pass's 1 GiB device
+    copy.  Writes the JSON summary that the timed runs report as
+    roofline.traffic."""
+    import csv
+    import glob
+
+    def per_dispatch(d, counter):
+        rows = []
+        for f in glob.glob(os.path.join(d, '**', '*counter_collection.csv'), recursive=True):
+            with open(f) as fh:
+                rows += [r for r in csv.DictReader(fh) if r['Counter_Name'] == counter]
+        acc = {}
+        for r in rows:
+            k = int(r['Dispatch_Id'])
+            name, v = r['Kernel_Name'], float(r['Counter_Value'])
+            acc[k] = (name, acc.get(k, (name, 0.0))[1] + v)
+        return acc
+
+    res = {'ncol': ncol, 'fast_math': fast_math}
+    gib = float(1 << 30)
+    for label, d, counter in (('fetch', fetch_dir, 'FETCH_SIZE'), ('write', write_dir, 'WRITE_SIZE')):
+        disp = per_dispatch(d, counter)
+        copies = [v for n, v in disp.values() if 'direct_copy_kernel' in n]
+        cal = max(copies) * 1024.0 if copies else None
+        factor = gib / cal if cal else 1.0
+        res[f'{label}_calibration'] = {'copy_bytes': gib, 'counter_bytes': cal, 'factor': factor}
+        for mode, tag in (('plain', ', 0, false, false>'), ('warm', ', 1, true, false>')):
+            vals = [v for n, v in disp.values() if 'cape_grid_kernel' in n and tag in n]
+            if vals:
+                res[f'{mode}_{label}_bytes'] = vals[0] * 1024.0 * factor
+    for mode in ('plain', 'warm'):
+        if f'{mode}_fetch_bytes' in res and f'{mode}_write_bytes' in res:
+            res[f'{mode}_hbm_bytes_per_launch'] = (res[f'{mode}_fetch_bytes'] +
+                                                    res[f'{mode}_write_bytes'])
+    with open(out, 'w') as f:
+        json.dump(res, f, indent=1)
+    print(json.dumps(res))
+
+
+def ragged_section(fast_math, log, ncol=200_000):
+    """The ragged-batch workload (200 000 columns, level counts 30/40/60/91/137,
+    SB/ML/MU sources, pseudo-liquid/pseudo-ice): autotuned persistent plans,
+    graph-replayed, plain and warm-started; accuracy on a column sample."""
+    import torch
+
+    from oracle import oracle as O
+    from xcape_amd import ragged, synthetic
+    t0 = time.perf_counter()
+    c = synthetic.ragged_columns(ncol, 1002)
+    b = ragged.RaggedBatch.from_dict(c)
+    res = {'columns': ncol, 'generate_s': time.perf_counter() - t0}
+    for name, warm in (('plain', False), ('warm', True)):
+        best, times = ragged.autotune(b, fast_math=fast_math, use_cache=False, warm_start=warm)
+        plan = ragged.RaggedPlan(b, fast_math=fast_math, warm_start=warm, **best)
+        ms = ragged._time_plan(plan, reps=10)
+        st = plan.run(stats=True)
+        plan.close()
+        torch.cuda.synchronize()
+        res[name] = {'best': best, 'ms': ms, 'columns_per_s': ncol / (ms / 1e3),
+                     'configs_swept': len(times), 'iterations': st['iterations'],
+                     'fallbacks': st['fallbacks']}
+        log(f'[bench] ragged {name}: {best} {ms:.3f} ms')
+    # accuracy of the warm plan on the first 2000 columns (grouped oracle)
+    sel = np.arange(2000)
+    got = b.cape.cpu().numpy()[sel]
+    off = c['offsets']
+    err = 0.0
+    for i in sel:
+        s0, s1 = off[i], off[i + 1]
+        r = O.cape(c['p'][None, s0:s1], c['t'][None, s0:s1], c['td'][None, s0:s1],
+                   c['ps'][i:i + 1], c['ts'][i:i + 1], c['tds'][i:i + 1], 500.,
+                   int(c['source'][i]), 500., int(c['adiabat'][i]))
+        err = max(err, abs(float(got[i]) - float(r[0][0])))
+    res['warm_cape_max_abs_err_2000'] = err
+    res['speedup_warm_vs_plain'] = res['plain']['ms'] / res['warm']['ms']
+    return res
+
+
 def load_surrogate(path, log):
     """Activate the committed surrogate, or train one on the GPU when the
     file is absent.  Returns a description for the JSON line."""
@@ -196,10 +277,18 @@ def main():
                     help='columns copied to the host for the CPU baseline (upper bound)')
     ap.add_argument('--cpu-seconds', type=float, default=10.0)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-ragged', action='store_true',
+                    help='skip the ragged-batch (autotuned persistent kernel) section')
     ap.add_argument('--ncol', type=int, default=NCOL_X * NCOL_Y)
     ap.add_argument('--profile-pass', action='store_true',
                     help='one launch of each kernel variant, for counter runs')
+    ap.add_argument('--pmc-summary', nargs=2, metavar=('FETCH_DIR', 'WRITE_DIR'),
+                    help='summarise two rocprofv3 counter runs of --profile-pass')
+    ap.add_argument('--pmc-out', default=PMC_FILE)
     args = ap.parse_args()
+    if args.pmc_summary:
+        pmc_summary(*args.pmc_summary, args.pmc_out, args.ncol, bool(args.fast_math))
+        return
     if args.gpus != 1:
         print(f'bench.py runs on one GPU; --gpus {args.gpus} is not supported '
               f'(columns are independent: a multi-GPU run would shard them with no exchange)',
@@ -285,6 +374,11 @@ def main():
         if pmc.get('ncol') == ncol and pmc.get('fast_math') == bool(args.fast_math):
             traffic = pmc.get(f'{dom}_hbm_bytes_per_launch')
 
+    rag = None
+    if not args.no_ragged:
+        log('[bench] ragged batches')
+        rag = ragged_section(bool(args.fast_math), log)
+
     cpu = None
     if not args.no_cpu:
         threads = len(os.sched_getaffinity(0))
@@ -327,6 +421,7 @@ def main():
         'kernel_counts': counts,
         'surrogate': sur,
         'accuracy': acc,
+        'ragged_200k': rag,
         'cpu_baseline': cpu,
     }
     print(json.dumps(line))

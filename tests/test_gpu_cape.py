@@ -176,6 +176,30 @@ def test_device_pointer_path(hip):
         np.testing.assert_array_equal(x, y.cpu().numpy())
 
 
+@pytest.mark.parametrize('warm', [False, True])
+def test_level_major_input_matches_column_contiguous(hip, warm):
+    """(nk, ncol) profiles with XCAPE_LEVEL_MAJOR give the same results as
+    the column-contiguous ones (which the library stages level-major), on a
+    sigma and on a pressure grid, CAPE and helicity."""
+    c = synthetic.sigma_columns(3000, 60, 17, winds=True)
+    args = (c['p'], c['t'], c['td'], c['ps'], c['ts'], c['tds'])
+    lm = (c['p'].T.copy(), c['t'].T.copy(), c['td'].T.copy(), c['ps'], c['ts'], c['tds'])
+    kw = dict(pinc=500., source=2, adiabat=3, warm_start=warm)
+    for x, y in zip(hip.cape(*args, **kw), hip.cape(*lm, level_major=True, **kw)):
+        np.testing.assert_array_equal(x, y)
+    sa = (c['p'], c['t'], c['td'], c['u'], c['v'], c['ps'], c['ts'], c['tds'], c['us'], c['vs'])
+    sl = (c['p'].T.copy(), c['t'].T.copy(), c['td'].T.copy(), c['u'].T.copy(), c['v'].T.copy(),
+          c['ps'], c['ts'], c['tds'], c['us'], c['vs'])
+    for x, y in zip(hip.srh(*sa), hip.srh(*sl, level_major=True)):
+        np.testing.assert_array_equal(x, y)
+    pc = synthetic.pressure_columns(2000, 18)
+    pa = (np.asarray(pc['p'], np.float32), pc['t'], pc['td'], pc['ps'], pc['ts'], pc['tds'])
+    pl = (pa[0], pc['t'].T.copy(), pc['td'].T.copy(), pc['ps'], pc['ts'], pc['tds'])
+    for x, y in zip(hip.cape(*pa, p_shared=True, **kw),
+                    hip.cape(*pl, p_shared=True, level_major=True, **kw)):
+        np.testing.assert_array_equal(x, y)
+
+
 @pytest.mark.parametrize('adiabat', [1, 2, 3, 4])
 @pytest.mark.parametrize('source', [1, 2, 3])
 def test_warm_start_within_tolerance(hip, oracle_lib, source, adiabat):

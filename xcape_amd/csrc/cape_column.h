@@ -36,6 +36,11 @@ namespace xcape {
 
 enum { SRC_SURFACE = 1, SRC_MU = 2, SRC_ML = 3 };
 
+// Fixed-point iterations per trip of the warm-started loop.
+#ifndef XC_ITERS_PER_TRIP
+#define XC_ITERS_PER_TRIP 2
+#endif
+
 // Makes v's value unknown to the optimiser at this point (no code).
 XC_HD __forceinline__ void xc_opaque(int &v)
 {
@@ -312,22 +317,27 @@ XC_HD __forceinline__ void layer_update(T th2, T qv2, T ql2, T qi2, const Env<T>
     const T thv2 = F::div(th2 * (T(1.0) + K::reps * qv2), T(1.0) + qv2 + ql2 + qi2);
     b2 = F::div(K::g * (thv2 - e.thv), e.thv);
     dz = layer_dz(e.thv, thv_prev, e.pi, pi_prev);
+    // the reference's four cases (:509-545) as selects over one division:
+    // each case's expressions are evaluated as written there, and the wave
+    // does not run the four branches one after the other
+    const bool enter = b2 >= T(0.0) && b1 < T(0.0);   // entering positive area
+    const bool leave = b2 < T(0.0) && b1 > T(0.0);    // leaving it
+    const bool neg = b2 < T(0.0) && !leave;
+    const T frac = F::div(enter ? b2 : b1, enter ? b2 - b1 : b1 - b2);
+    const T half_dz_sum = T(0.5) * dz * (b1 + b2);
     T parea;
-    if (b2 >= T(0.0) && b1 < T(0.0)) {
-        const T frac = F::div(b2, b2 - b1);
+    if (enter) {
         parea = T(0.5) * b2 * dz * frac;
-        narea = narea - T(0.5) * b1 * dz * (T(1.0) - frac);
-        cin = cin + narea;
+        cin = cin + (narea - T(0.5) * b1 * dz * (T(1.0) - frac));
         narea = T(0.0);
-    } else if (b2 < T(0.0) && b1 > T(0.0)) {
-        const T frac = F::div(b1, b1 - b2);
+    } else if (leave) {
         parea = T(0.5) * b1 * dz * frac;
         narea = -T(0.5) * b2 * dz * (T(1.0) - frac);
-    } else if (b2 < T(0.0)) {
+    } else if (neg) {
         parea = T(0.0);
-        narea = narea - T(0.5) * dz * (b1 + b2);
+        narea = narea - half_dz_sum;
     } else {
-        parea = T(0.5) * dz * (b1 + b2);
+        parea = half_dz_sum;
         narea = T(0.0);
     }
     cape = cape + fmx(T(0.0), parea);
@@ -456,6 +466,10 @@ XC_HD void cape_column_warm(const C &col, int nk, T pinc, const Parcel<T> &pc,
 {
     using F = Fn<T, MATH>;
     using K = Phys<T>;
+    // the predictor's own arithmetic (slopes, rates, extrapolation weights)
+    // only shapes the first iterate, never the fixed point: it divides with
+    // the hardware reciprocal in every math mode
+    const auto pdiv = [](T a, T b) { return pred_div(a, b); };
     const int mulvl = pc.mulvl;
     T zout = T(-999999.0);
     T th2 = pc.th2, pi2 = pc.e.pi, p2 = pc.e.p, t2 = pc.t2, qv2 = pc.qv2, b2 = pc.b2, zk = pc.zk;
@@ -519,7 +533,7 @@ XC_HD void cape_column_warm(const C &col, int nk, T pinc, const Parcel<T> &pc,
             if (!predict) {
                 // every sub-step cold: the flattened loop without prediction
             } else if (nh >= 2) {
-                const T r = r1 + (r1 - r2) * F::div(logp + dl1, dl1 + dl2);
+                const T r = r1 + (r1 - r2) * pdiv(logp + dl1, dl1 + dl2);
                 g = th1 + r * logp;
             } else if (nh == 1) {
                 g = th1 + r1 * logp;
@@ -537,23 +551,19 @@ XC_HD void cape_column_warm(const C &col, int nk, T pinc, const Parcel<T> &pc,
             st.substeps++;
             mode = M_ITER;
         }
-        if (mode == M_ITER) {
-            // one fixed-point iteration (:436-479)
+        // up to XC_ITERS_PER_TRIP fixed-point iterations (:436-479): most
+        // warm-started sub-steps converge at the first or second, so two per
+        // trip halve the trips that pay for the set-up blocks around them
+#pragma unroll
+        for (int rep = 0; rep < XC_ITERS_PER_TRIP; ++rep) {
+            if (mode != M_ITER) continue;
             it = it + 1;
-            fp_eval<T, MATH, ICE>(p2, pi2, logp, t1, th1, qv1, ql1, qi1, qt, thlast, th2, t2, qv2,
+            const T x = thlast;
+            fp_eval<T, MATH, ICE>(p2, pi2, logp, t1, th1, qv1, ql1, qi1, qt, x, th2, t2, qv2,
                                   ql2, qi2);
             st.iterations++;
             if (first_sub) st.first++;
-            if (it >= 2) {
-                const T dx = thlast - x_prev;
-                if (dx != T(0.0)) {
-                    const T sl = F::div(th2 - f_prev, dx);
-                    fpr = sl < T(-10.0) ? T(-10.0) : (sl > T(0.9) ? T(0.9) : sl);
-                }
-            }
-            x_prev = thlast;
-            f_prev = th2;
-            const T d = th2 - thlast;
+            const T d = th2 - x;
             const T ad = d < T(0.0) ? -d : d;
             if (it > 100) {
                 st.nonconverged++;
@@ -568,14 +578,26 @@ XC_HD void cape_column_warm(const C &col, int nk, T pinc, const Parcel<T> &pc,
                 it = 0;
                 st.fallbacks++;
             } else if (ad > K::converge) {
-                thlast = thlast + T(0.3) * (th2 - thlast);
+                thlast = x + T(0.3) * (th2 - x);
+                x_prev = x;
+                f_prev = th2;
             } else {
-                // converged: record the rate, finish the sub-step (:485-497)
+                // converged: record the rate, finish the sub-step (:485-497).
+                // The slope f' of the map from this sub-step's last two
+                // iterates (kept from an earlier sub-step when it converged
+                // at its first)
                 first_sub = false;
+                if (it >= 2) {
+                    const T dx = x - x_prev;
+                    if (dx != T(0.0)) {
+                        const T sl = pdiv(th2 - f_prev, dx);
+                        fpr = sl < T(-10.0) ? T(-10.0) : (sl > T(0.9) ? T(0.9) : sl);
+                    }
+                }
                 if (logp != T(0.0)) {
-                    const T xs = thlast + F::div(th2 - thlast, T(1.0) - fpr);
+                    const T xs = x + pdiv(th2 - x, T(1.0) - fpr);
                     r2 = r1; dl2 = dl1;
-                    r1 = F::div(xs - th1, logp);
+                    r1 = pdiv(xs - th1, logp);
                     dl1 = logp;
                     // a second-order guess that failed marks a change of
                     // regime (saturation, ice onset): the older rate belongs

@@ -124,19 +124,49 @@ int cape_block(bool warm)
     return 256;
 }
 
-// The helicity chain.  Level-major profiles go straight to the kernel;
-// column-contiguous ones are staged level-major first, in chunks of columns
-// whose copies fit in about 1 GiB (each chunk: one transpose launch for all
-// profile arrays, then the helicity launch on the copies).
-hipError_t run_srh(const CapeGridArgs &a0, hipStream_t s)
+// The grid kernels read level-major profiles: with a column per lane, 64
+// lanes reading 64 different columns of column-contiguous arrays fetch 64
+// cache lines per level each, and on large grids the L2 cannot hold them
+// for the ~32 levels each line serves (measured on 2048x1024x137: 142 GB
+// fetched per ascent launch for 3.4 GB of profiles).  Level-major profiles go
+// straight to the kernels; column-contiguous ones are staged level-major
+// first through the LDS-tiled transpose, in chunks of columns whose copies
+// fit in about 8 GiB (each chunk: one transpose launch for every profile
+// array the call needs, shared by the CAPE and helicity launches, then those
+// launches on the copies).
+hipError_t launch_grid(const CapeGridArgs &a, bool cape, bool srh, int math, hipStream_t s)
 {
-    if (a0.ls != 1 || a0.nk_in == 1) return launch_cape_grid_accurate(a0, false, GRID_SRH, 256, s);
+    hipError_t e = hipSuccess;
+    if (cape) {
+        const int block = cape_block(a.warm);
+        if (math == 2)
+            e = launch_cape_grid_accurate(a, true, GRID_CAPE, block, s);
+        else if (math == 1)
+            e = launch_cape_grid_fast(a, GRID_CAPE, block, s);
+        else
+            e = launch_cape_grid_accurate(a, false, GRID_CAPE, block, s);
+        if (e != hipSuccess) return e;
+    }
+    // the helicity chain keeps the reference's precisions in every math mode
+    if (srh) e = launch_cape_grid_accurate(a, false, GRID_SRH, 256, s);
+    return e;
+}
+
+hipError_t run_grid(const CapeGridArgs &a0, bool cape, bool srh, int math, hipStream_t s)
+{
+    if (a0.ls != 1 || a0.nk_in == 1) return launch_grid(a0, cape, srh, math, s);
     const bool sigma = a0.p != nullptr;
-    const int narr = sigma ? 5 : 4;
+    const float *src0[5];
+    int narr = 0;
+    int it = -1, itd = -1, iu = -1, iv = -1, ip = -1;
+    it = narr; src0[narr++] = a0.t;
+    itd = narr; src0[narr++] = a0.td;
+    if (sigma) { ip = narr; src0[narr++] = a0.p; }
+    if (srh) { iu = narr; src0[narr++] = a0.u; iv = narr; src0[narr++] = a0.v; }
     const int64_t nk = a0.nk_in;
     const int64_t per_col = (int64_t)narr * nk * (int64_t)sizeof(float);
-    int64_t chunk = ((int64_t)1 << 30) / per_col;
-    if (chunk < 65536) chunk = 65536;
+    int64_t chunk = ((int64_t)8 << 30) / per_col;
+    if (chunk < ((int64_t)1 << 20)) chunk = (int64_t)1 << 20;
     if (chunk > a0.ncol) chunk = a0.ncol;
     DevBuf<float> scratch;
     hipError_t e = scratch.alloc((size_t)narr * (size_t)nk * (size_t)chunk, s);
@@ -144,20 +174,35 @@ hipError_t run_srh(const CapeGridArgs &a0, hipStream_t s)
     for (int64_t c0 = 0; c0 < a0.ncol; c0 += chunk) {
         const int64_t n = chunk < a0.ncol - c0 ? chunk : a0.ncol - c0;
         const int64_t o = c0 * nk;
-        const float *src[5] = {a0.t + o, a0.td + o, a0.u + o, a0.v + o, sigma ? a0.p + o : nullptr};
+        const float *src[5];
         float *dst[5];
-        for (int i = 0; i < 5; ++i) dst[i] = scratch.ptr + (size_t)i * (size_t)nk * (size_t)n;
+        for (int i = 0; i < narr; ++i) {
+            src[i] = src0[i] + o;
+            dst[i] = scratch.ptr + (size_t)i * (size_t)nk * (size_t)n;
+        }
         if ((e = launch_transpose_levels(src, dst, narr, n, nk, s)) != hipSuccess) return e;
         CapeGridArgs a = a0;
         a.ncol = n;
         a.ls = n;
-        a.t = dst[0]; a.td = dst[1]; a.u = dst[2]; a.v = dst[3];
-        if (sigma) a.p = dst[4];
-        a.ps += c0; a.ts += c0; a.tds += c0; a.us += c0; a.vs += c0;
+        a.t = dst[it];
+        a.td = dst[itd];
+        if (sigma) a.p = dst[ip];
+        a.ps += c0; a.ts += c0; a.tds += c0;
         if (a.start) a.start += c0;
-        a.srh_rm += c0; a.srh_lm += c0;
-        if (a.rm) { a.rm += 2 * c0; a.lm += 2 * c0; a.mean6 += 2 * c0; }
-        if ((e = launch_cape_grid_accurate(a, false, GRID_SRH, 256, s)) != hipSuccess) return e;
+        if (cape) {
+            a.cape = static_cast<char *>(a.cape) + c0 * (math == 2 ? 8 : 4);
+            a.cin = static_cast<char *>(a.cin) + c0 * (math == 2 ? 8 : 4);
+            a.zout = static_cast<char *>(a.zout) + c0 * (math == 2 ? 8 : 4);
+            a.mulvl += c0;
+        }
+        if (srh) {
+            a.u = dst[iu];
+            a.v = dst[iv];
+            a.us += c0; a.vs += c0;
+            a.srh_rm += c0; a.srh_lm += c0;
+            if (a.rm) { a.rm += 2 * c0; a.lm += 2 * c0; a.mean6 += 2 * c0; }
+        }
+        if ((e = launch_grid(a, cape, srh, math, s)) != hipSuccess) return e;
     }
     return hipSuccess;
 }
@@ -291,17 +336,9 @@ int grid_entry(const GridCall &c)
         XC_TRY(zero_counters(dstats_.ptr, ST_COUNT, s));
         a.stats = dstats_.ptr;
     }
-    if (want_cape) {
-        const int block = cape_block(a.warm);
-        if (c.fp64)
-            e = launch_cape_grid_accurate(a, true, GRID_CAPE, block, s);
-        else if (c.flags & XCAPE_FAST_MATH)
-            e = launch_cape_grid_fast(a, GRID_CAPE, block, s);
-        else
-            e = launch_cape_grid_accurate(a, false, GRID_CAPE, block, s);
-        if (e != hipSuccess) return hip_fail(e, "CAPE launch");
-    }
-    if (want_srh && (e = run_srh(a, s)) != hipSuccess) return hip_fail(e, "helicity launch");
+    const int math = c.fp64 ? 2 : ((c.flags & XCAPE_FAST_MATH) ? 1 : 0);
+    if ((e = run_grid(a, want_cape, want_srh, math, s)) != hipSuccess)
+        return hip_fail(e, "kernel launch");
     if (!dev) {
         if (want_cape) {
             XC_TRY(hipMemcpyAsync(c.cape, dcape_.ptr, ncol * osz, hipMemcpyDeviceToHost, s));

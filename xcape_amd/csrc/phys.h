@@ -104,6 +104,17 @@ template <> struct Fn<float, MATH_FAST> {
     }
 };
 
+// Division for arithmetic that never reaches a result, only a starting
+// guess: the hardware reciprocal plus one correction on the device.
+template <typename T>
+XC_HD __forceinline__ T pred_div(T a, T b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (sizeof(T) == 4) return Fn<float, MATH_FAST>::div(a, b);
+#endif
+    return a / b;
+}
+
 template <int MATH> struct Fn<double, MATH> {
     static XC_HD __forceinline__ double exp(double x) { return ::exp(x); }
     static XC_HD __forceinline__ double log(double x) { return ::log(x); }

@@ -1,29 +1,39 @@
 // Ragged batches: columns with different level counts, parcel sources and
-// adiabats in one persistent launch.
+// adiabats in one graph-replayed sequence of persistent launches.
 //
-// Layout (CSR): column i owns levels offsets[i] .. offsets[i+1]-1 of the
-// concatenated p, t, td arrays (hPa, degC); surface values, source and
+// Caller layout (CSR): column i owns levels offsets[i] .. offsets[i+1]-1 of
+// the concatenated p, t, td arrays (hPa, degC); surface values, source and
 // adiabat are per column.
 //
-// Scheduling: a plan sorts the columns once (host, stable) by adiabat, then
-// source, then descending level count, so that the 64 lanes of a wavefront
-// run the same code path on columns of similar length, and long columns are
-// dealt first.  The kernel is persistent: a grid sized to fill the device
-// once, every wavefront taking the next `chunk` sorted columns from a device
-// counter until the batch is drained (every wave reaches the exit test, so
-// the grid always drains).  The counter is reset by a one-thread kernel in
-// the same stream (graph-safe: no memset node).
+// Plan (built once on the host): the columns are sorted, stably, by adiabat,
+// then source, then descending level count, so that the 64 lanes of a
+// wavefront run the same code path on columns of similar length and long
+// columns are dealt first.  Each adiabat is a segment of that order with its
+// own kernel instantiation (one ICE/PSEUDO code path: a kernel that switched
+// over all four per column would carry all four in its registers).
 //
-// Profiles are staged through LDS: when a wavefront takes a chunk it loads
-// the first `tile` levels of its columns into LDS with each column's levels
-// read as contiguous 16-byte vectors, so the source selection and the lowest
-// ascent levels (where most columns start) read LDS; levels above the tile
-// are read from global memory (L2-resident after the first touch).
+// Run (one captured hipGraph, replayed):
+//  1. reset the segments' work counters (a one-thread kernel: no memset node);
+//  2. stage the batch level-major in sorted order: level k of sorted column j
+//     at [k * ncol + j] of a padded (max level count, ncol) copy, through
+//     64 x 64 LDS tiles so that the CSR reads (consecutive levels of one
+//     column) and the level-major writes (consecutive columns of one level)
+//     are both coalesced; the surface values are gathered in the same order.
+//     A wavefront's 64 lanes then read 64 consecutive words per level (the
+//     CSR layout read one column per lane makes every lane fetch its own
+//     cache lines, which the L2 cannot keep for large batches);
+//  3. per segment, a persistent ascent kernel: a grid sized to fill the
+//     device once, every wavefront taking the next `chunk` columns of its
+//     segment from the counter until the segment is drained (every wave
+//     reaches the exit test, so the grid always drains).  Optionally the
+//     first `tile` levels of a wave's columns are copied to LDS first (the
+//     autotuner decides whether that pays).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <numeric>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/xcape_hip.h"
@@ -35,70 +45,118 @@ namespace xcape {
 namespace {
 
 constexpr int MAX_TILE = 64;   // upper bound of the LDS level-tile depth
+constexpr int NSEG = 4;        // one segment per adiabat
+constexpr int STAGE_TILE = 64;
 
-struct RaggedArgs {
+struct StageArgs {
     const int64_t *offsets;
-    const int32_t *order;      // sorted column order
-    const int32_t *source, *adiabat;
+    const int32_t *order, *nk_sorted;
     const float *p, *t, *td, *ps, *ts, *tds;
-    int64_t ncol;
-    float pinc, ml_depth;
-    float *cape, *cin, *zout;
-    int32_t *mulvl;
-    unsigned int *counter;
-    unsigned long long *stats;
-    int chunk;                 // columns a wavefront takes per grab (64 or 128)
-    int tile;                  // levels staged in LDS (0 = none)
-    int warm;                  // warm-started iteration (see cape_column_warm)
-    float fallback_tol;
+    float *lp, *lt, *ltd;          // (maxnk, ncol) level-major, sorted order
+    float *sps, *sts, *stds;       // (ncol,) sorted order
+    int64_t ncol, maxnk;
 };
 
-template <int MATH, bool ICE, bool PSEUDO, class C>
-__device__ __forceinline__ void run_column(const RaggedArgs &a, const C &c, int nk, int src,
-                                           ColumnOut<float> &o, LaneStats &st)
+// blockIdx.x: 64 sorted columns, blockIdx.y: 64 levels
+__global__ void __launch_bounds__(256) stage_kernel(StageArgs a)
 {
-    if (a.warm) {
-        Parcel<float> pc;
-        if (parcel_source<float, MATH>(c, nk, src, a.ml_depth, pc))
-            cape_column_warm<float, MATH, ICE, PSEUDO>(c, nk, a.pinc, pc, NAN, a.fallback_tol,
-                                                       o, st);
-        else {
-            o.cape = 0.0f; o.cin = 0.0f; o.mulvl = pc.mulvl; o.zout = -999999.0f;
+    __shared__ float tile[STAGE_TILE][STAGE_TILE + 1];
+    const int64_t j0 = (int64_t)blockIdx.x * STAGE_TILE;
+    const int64_t k0 = (int64_t)blockIdx.y * STAGE_TILE;
+    const int tx = threadIdx.x & (STAGE_TILE - 1), ty = threadIdx.x >> 6;
+    if (blockIdx.y == 0 && threadIdx.x < STAGE_TILE) {
+        const int64_t j = j0 + tx;
+        if (j < a.ncol) {
+            const int32_t c = a.order[j];
+            a.sps[j] = a.ps[c];
+            a.sts[j] = a.ts[c];
+            a.stds[j] = a.tds[c];
         }
-    } else {
-        cape_column<float, MATH, ICE, PSEUDO>(c, nk, a.pinc, src, a.ml_depth, o, st);
+    }
+    for (int arr = 0; arr < 3; ++arr) {
+        const float *src = arr == 0 ? a.p : (arr == 1 ? a.t : a.td);
+        float *dst = arr == 0 ? a.lp : (arr == 1 ? a.lt : a.ltd);
+        for (int r = ty; r < STAGE_TILE; r += 4) {
+            const int64_t j = j0 + r, k = k0 + tx;
+            if (j < a.ncol && k < a.nk_sorted[j]) tile[r][tx] = src[a.offsets[a.order[j]] + k];
+        }
+        __syncthreads();
+        for (int r = ty; r < STAGE_TILE; r += 4) {
+            const int64_t k = k0 + r, j = j0 + tx;
+            // entries past a column's level count are never read
+            if (j < a.ncol && k < a.maxnk) dst[k * a.ncol + j] = tile[tx][r];
+        }
+        __syncthreads();
     }
 }
 
-// Column accessor: levels below `tile` from this lane's LDS slice, the rest
-// from global memory.
-struct TiledCol {
-    const float *p, *t, *td;       // global, level k-1 at [k-1]
-    const float *lp, *lt, *ltd;    // LDS slice, level k-1 at [(k-1) * 64]
+struct RaggedArgs {
+    const int32_t *order, *nk_sorted, *src_sorted;
+    const float *lp, *lt, *ltd, *sps, *sts, *stds;
+    int64_t ncol;                  // the level stride of the staged copies
+    int64_t seg_begin, seg_end;    // this launch's segment of the sorted order
+    float pinc, ml_depth;
+    float *cape, *cin, *zout;
+    int32_t *mulvl;
+    unsigned int *counter;         // this segment's work counter
+    unsigned long long *stats;
+    int chunk;                     // columns a wavefront takes per grab
+    int tile;                      // levels staged in LDS (0 = none)
+    float fallback_tol;
+};
+
+// Sorted column j of the staged batch: level k >= 1 at [(k-1) * ls + j] of
+// the level-major copies.
+struct StagedCol {
+    const float *p, *t, *td;
+    uint32_t j, ls;
     float ps, ts, tds;
+    __device__ __forceinline__ uint32_t at(int k) const { return (uint32_t)(k - 1) * ls + j; }
+    __device__ __forceinline__ float P(int k) const { return k == 0 ? ps : p[at(k)]; }
+    __device__ __forceinline__ float Tc(int k) const { return k == 0 ? ts : t[at(k)]; }
+    __device__ __forceinline__ float Tdc(int k) const { return k == 0 ? tds : td[at(k)]; }
+    __device__ __forceinline__ Raw raw_above(int k) const
+    {
+        return Raw{p[at(k)], t[at(k)], td[at(k)]};
+    }
+};
+
+// The same with the levels below `tile` read from this lane's LDS slice.
+struct TiledCol {
+    const float *p, *t, *td;
+    const float *lp, *lt, *ltd;
+    uint32_t j, ls;
     int tile;
+    float ps, ts, tds;
+    __device__ __forceinline__ uint32_t at(int k) const { return (uint32_t)(k - 1) * ls + j; }
     __device__ __forceinline__ float P(int k) const
     {
-        return k == 0 ? ps : (k - 1 < tile ? lp[(k - 1) * 64] : p[k - 1]);
+        return k == 0 ? ps : (k - 1 < tile ? lp[(k - 1) * 64] : p[at(k)]);
     }
     __device__ __forceinline__ float Tc(int k) const
     {
-        return k == 0 ? ts : (k - 1 < tile ? lt[(k - 1) * 64] : t[k - 1]);
+        return k == 0 ? ts : (k - 1 < tile ? lt[(k - 1) * 64] : t[at(k)]);
     }
     __device__ __forceinline__ float Tdc(int k) const
     {
-        return k == 0 ? tds : (k - 1 < tile ? ltd[(k - 1) * 64] : td[k - 1]);
+        return k == 0 ? tds : (k - 1 < tile ? ltd[(k - 1) * 64] : td[at(k)]);
     }
     __device__ __forceinline__ Raw raw_above(int k) const
     {
         return k - 1 < tile ? Raw{lp[(k - 1) * 64], lt[(k - 1) * 64], ltd[(k - 1) * 64]}
-                            : Raw{p[k - 1], t[k - 1], td[k - 1]};
+                            : Raw{p[at(k)], t[at(k)], td[at(k)]};
     }
 };
 
-__global__ void reset_counter(unsigned int *c) { *c = 0u; }
+__global__ void reset_counters(unsigned int *c)
+{
+    if (threadIdx.x < NSEG) c[threadIdx.x] = 0u;
+}
 
-template <int MATH>
+// MODE 0: the reference's loops (cape_column); 1: warm-started
+// (cape_column_warm).  TILED: stage the first a.tile levels in LDS.  STATS:
+// accumulate the work counters.
+template <int MATH, bool ICE, bool PSEUDO, int MODE, bool TILED, bool STATS>
 __global__ void __launch_bounds__(256) ragged_kernel(RaggedArgs a)
 {
     // LDS: per wave, 3 arrays x tile levels x 64 lanes, level-major so the
@@ -115,72 +173,66 @@ __global__ void __launch_bounds__(256) ragged_kernel(RaggedArgs a)
         unsigned int base = 0;
         if (lane == 0) base = atomicAdd(a.counter, (unsigned int)a.chunk);
         base = __shfl(base, 0, 64);
-        if ((int64_t)base >= a.ncol) break;
+        const int64_t b0 = a.seg_begin + (int64_t)base;
+        if (b0 >= a.seg_end) break;
         for (int sub = 0; sub < a.chunk; sub += 64) {
-            const int64_t i = (int64_t)base + sub + lane;
-            const bool active = i < a.ncol;
-            int col = 0, nk_in = 0;
-            int64_t off = 0;
-            if (active) {
-                col = a.order[i];
-                off = a.offsets[col];
-                nk_in = (int)(a.offsets[col + 1] - off);
-            }
-            const int tile = min(a.tile, nk_in);
-            // stage levels 0..tile-1 of this lane's column: 16-byte reads where
-            // the column start is aligned, scalar reads otherwise
-            if (active) {
-                const float *gp = a.p + off, *gt = a.t + off, *gtd = a.td + off;
-                int k = 0;
-                if ((off & 3) == 0) {
-                    for (; k + 4 <= tile; k += 4) {
-                        const float4 vp = *reinterpret_cast<const float4 *>(gp + k);
-                        const float4 vt = *reinterpret_cast<const float4 *>(gt + k);
-                        const float4 vd = *reinterpret_cast<const float4 *>(gtd + k);
-                        lp[(k + 0) * 64] = vp.x; lp[(k + 1) * 64] = vp.y;
-                        lp[(k + 2) * 64] = vp.z; lp[(k + 3) * 64] = vp.w;
-                        lt[(k + 0) * 64] = vt.x; lt[(k + 1) * 64] = vt.y;
-                        lt[(k + 2) * 64] = vt.z; lt[(k + 3) * 64] = vt.w;
-                        ltd[(k + 0) * 64] = vd.x; ltd[(k + 1) * 64] = vd.y;
-                        ltd[(k + 2) * 64] = vd.z; ltd[(k + 3) * 64] = vd.w;
+            const int64_t j = b0 + sub + lane;
+            const bool active = j < a.seg_end;
+            const int nk_in = active ? a.nk_sorted[j] : 0;
+            const int tile = TILED ? min(a.tile, nk_in) : 0;
+            // levels 0..tile-1: the wave's lanes read consecutive columns
+            if constexpr (TILED) {
+                for (int k = 0; k < a.tile; ++k) {
+                    if (k < tile) {
+                        const int64_t g = (int64_t)k * a.ncol + j;
+                        lp[k * 64] = a.lp[g];
+                        lt[k * 64] = a.lt[g];
+                        ltd[k * 64] = a.ltd[g];
                     }
-                }
-                for (; k < tile; ++k) {
-                    lp[k * 64] = gp[k];
-                    lt[k * 64] = gt[k];
-                    ltd[k * 64] = gtd[k];
                 }
             }
             // each lane reads back only its own LDS slots: no barrier needed
-            if (active) {
-                const float ts = a.ts[col];
-                if (ts > 0.0f) {
-                    ++ran;
-                    TiledCol c{a.p + off, a.t + off, a.td + off, lp, lt, ltd,
-                               a.ps[col], ts, a.tds[col], tile};
-                    ColumnOut<float> o;
-                    const int nk = nk_in + 1;
-                    const int src = a.source[col];
-                    switch (a.adiabat[col]) {
-                    case 1: run_column<MATH, false, true>(a, c, nk, src, o, st); break;
-                    case 2: run_column<MATH, false, false>(a, c, nk, src, o, st); break;
-                    case 3: run_column<MATH, true, true>(a, c, nk, src, o, st); break;
-                    default: run_column<MATH, true, false>(a, c, nk, src, o, st); break;
-                    }
-                    a.cape[col] = o.cape;
-                    a.cin[col] = o.cin;
-                    a.mulvl[col] = o.mulvl;
-                    a.zout[col] = o.zout;
-                } else {
-                    a.cape[col] = 0.0f;
-                    a.cin[col] = 0.0f;
-                    a.zout[col] = 0.0f;
-                    a.mulvl[col] = 0;
-                }
+            if (!active) continue;
+            const int32_t col = a.order[j];
+            const float ts = a.sts[j];
+            if (!(ts > 0.0f)) {
+                a.cape[col] = 0.0f;
+                a.cin[col] = 0.0f;
+                a.zout[col] = 0.0f;
+                a.mulvl[col] = 0;
+                continue;
             }
+            ++ran;
+            using Col = std::conditional_t<TILED, TiledCol, StagedCol>;
+            Col c;
+            if constexpr (TILED)
+                c = TiledCol{a.lp, a.lt, a.ltd, lp, lt, ltd, (uint32_t)j, (uint32_t)a.ncol,
+                             tile, a.sps[j], ts, a.stds[j]};
+            else
+                c = StagedCol{a.lp, a.lt, a.ltd, (uint32_t)j, (uint32_t)a.ncol, a.sps[j], ts,
+                              a.stds[j]};
+            ColumnOut<float> o{};
+            const int nk = nk_in + 1;
+            const int src = a.src_sorted[j];
+            if constexpr (MODE == 0) {
+                cape_column<float, MATH, ICE, PSEUDO>(c, nk, a.pinc, src, a.ml_depth, o, st);
+                a.mulvl[col] = o.mulvl;
+            } else {
+                Parcel<float> pc;
+                const bool ok = parcel_source<float, MATH>(c, nk, src, a.ml_depth, pc);
+                a.mulvl[col] = pc.mulvl;
+                if (ok)
+                    cape_column_warm<float, MATH, ICE, PSEUDO>(c, nk, a.pinc, pc, NAN,
+                                                               a.fallback_tol, o, st, true);
+                else
+                    o.zout = -999999.0f;
+            }
+            a.cape[col] = o.cape;
+            a.cin[col] = o.cin;
+            a.zout[col] = o.zout;
         }
     }
-    if (a.stats) {
+    if constexpr (STATS) {
         unsigned long long v[7] = {ran, st.levels, st.substeps, st.iterations, st.nonconverged,
                                    st.fallbacks, st.first};
 #pragma unroll
@@ -192,35 +244,83 @@ __global__ void __launch_bounds__(256) ragged_kernel(RaggedArgs a)
     }
 }
 
+using Kernel = void (*)(RaggedArgs);
+
+template <int MATH, int MODE, bool TILED, bool STATS>
+Kernel pick_adiabat(int adiabat)
+{
+    switch (adiabat) {
+    case 1: return ragged_kernel<MATH, false, true, MODE, TILED, STATS>;
+    case 2: return ragged_kernel<MATH, false, false, MODE, TILED, STATS>;
+    case 3: return ragged_kernel<MATH, true, true, MODE, TILED, STATS>;
+    default: return ragged_kernel<MATH, true, false, MODE, TILED, STATS>;
+    }
+}
+
+template <int MATH, int MODE>
+Kernel pick_mode(int adiabat, bool tiled, bool stats)
+{
+    if (tiled) return stats ? pick_adiabat<MATH, MODE, true, true>(adiabat)
+                            : pick_adiabat<MATH, MODE, true, false>(adiabat);
+    return stats ? pick_adiabat<MATH, MODE, false, true>(adiabat)
+                 : pick_adiabat<MATH, MODE, false, false>(adiabat);
+}
+
+Kernel pick(bool fast, int adiabat, bool warm, bool tiled, bool stats)
+{
+    if (fast) return warm ? pick_mode<MATH_FAST, 1>(adiabat, tiled, stats)
+                          : pick_mode<MATH_FAST, 0>(adiabat, tiled, stats);
+    return warm ? pick_mode<MATH_ACCURATE, 1>(adiabat, tiled, stats)
+                : pick_mode<MATH_ACCURATE, 0>(adiabat, tiled, stats);
+}
+
 }  // namespace
 }  // namespace xcape
 
 using namespace xcape;
 
 struct xcape_ragged_plan {
-    RaggedArgs args{};
-    int32_t *d_order = nullptr;
+    StageArgs stage{};
+    RaggedArgs args{};                 // common part of every segment's launch
+    int64_t seg_begin[NSEG] = {}, seg_end[NSEG] = {};
+    int seg_grid[NSEG] = {};
+    int32_t *d_order = nullptr, *d_nk = nullptr, *d_src = nullptr;
+    float *d_stage = nullptr;          // lp, lt, ltd, sps, sts, stds
     unsigned int *d_counter = nullptr;
     unsigned long long *d_stats = nullptr;
-    int grid = 0, block = 256;
+    int block = 256;
     size_t lds_bytes = 0;
-    bool fast = false;
+    bool fast = false, warm = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
 
 namespace {
 
-hipError_t enqueue(const xcape_ragged_plan *pl, hipStream_t s)
+hipError_t enqueue(const xcape_ragged_plan *pl, bool stats, hipStream_t s)
 {
-    reset_counter<<<1, 1, 0, s>>>(pl->d_counter);
+    reset_counters<<<1, 64, 0, s>>>(pl->d_counter);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    if (pl->fast)
-        ragged_kernel<MATH_FAST><<<pl->grid, pl->block, pl->lds_bytes, s>>>(pl->args);
-    else
-        ragged_kernel<MATH_ACCURATE><<<pl->grid, pl->block, pl->lds_bytes, s>>>(pl->args);
-    return hipGetLastError();
+    const StageArgs &sa = pl->stage;
+    const dim3 sgrid((unsigned)((sa.ncol + STAGE_TILE - 1) / STAGE_TILE),
+                     (unsigned)((sa.maxnk + STAGE_TILE - 1) / STAGE_TILE));
+    stage_kernel<<<sgrid, 256, 0, s>>>(sa);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    for (int g = 0; g < NSEG; ++g) {
+        if (pl->seg_end[g] <= pl->seg_begin[g]) continue;
+        RaggedArgs a = pl->args;
+        a.seg_begin = pl->seg_begin[g];
+        a.seg_end = pl->seg_end[g];
+        a.counter = pl->d_counter + g;
+        a.stats = stats ? pl->d_stats : nullptr;
+        const Kernel k = pick(pl->fast, g + 1, pl->warm, pl->args.tile > 0, stats);
+        void *kargs[] = {&a};
+        e = hipLaunchKernel(reinterpret_cast<const void *>(k), dim3(pl->seg_grid[g]),
+                            dim3(pl->block), kargs, pl->lds_bytes, s);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 void destroy(xcape_ragged_plan *pl)
@@ -229,6 +329,9 @@ void destroy(xcape_ragged_plan *pl)
     if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
     if (pl->graph) (void)hipGraphDestroy(pl->graph);
     if (pl->d_order) (void)hipFree(pl->d_order);
+    if (pl->d_nk) (void)hipFree(pl->d_nk);
+    if (pl->d_src) (void)hipFree(pl->d_src);
+    if (pl->d_stage) (void)hipFree(pl->d_stage);
     if (pl->d_counter) (void)hipFree(pl->d_counter);
     if (pl->d_stats) (void)hipFree(pl->d_stats);
     delete pl;
@@ -267,6 +370,7 @@ int xcape_ragged_plan_create(const int64_t *offsets_host, const int32_t *source_
     // validate the layout on the host copies and build the work order
     if (offsets_host[0] != 0) return fail(XCAPE_EINVAL, "offsets[0] must be 0");
     std::vector<int32_t> order((size_t)ncol);
+    int64_t maxnk = 0;
     for (int64_t i = 0; i < ncol; ++i) {
         const int64_t n = offsets_host[i + 1] - offsets_host[i];
         if (n < 1 || n > (1 << 20))
@@ -276,51 +380,85 @@ int xcape_ragged_plan_create(const int64_t *offsets_host, const int32_t *source_
         if (adiabat_host[i] < 1 || adiabat_host[i] > 4)
             return fail(XCAPE_EINVAL, "adiabat[%lld]=%d not in 1..4", (long long)i,
                         adiabat_host[i]);
+        maxnk = std::max(maxnk, n);
         order[(size_t)i] = (int32_t)i;
     }
+    // the staged copies are indexed with 32-bit offsets
+    if ((uint64_t)maxnk * (uint64_t)ncol >= ((uint64_t)1 << 32))
+        return fail(XCAPE_EINVAL, "batch too large: %lld columns x %lld levels (split it)",
+                    (long long)ncol, (long long)maxnk);
     std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
         if (adiabat_host[x] != adiabat_host[y]) return adiabat_host[x] < adiabat_host[y];
         if (source_host[x] != source_host[y]) return source_host[x] < source_host[y];
         return offsets_host[x + 1] - offsets_host[x] > offsets_host[y + 1] - offsets_host[y];
     });
+    std::vector<int32_t> nk_sorted((size_t)ncol), src_sorted((size_t)ncol);
+    for (int64_t j = 0; j < ncol; ++j) {
+        const int32_t c = order[(size_t)j];
+        nk_sorted[(size_t)j] = (int32_t)(offsets_host[c + 1] - offsets_host[c]);
+        src_sorted[(size_t)j] = source_host[c];
+    }
     int rc = ensure_device();
     if (rc) return rc;
     auto *pl = new xcape_ragged_plan;
+    for (int g = 0; g < NSEG; ++g) {
+        const auto lower = [&](int a) {
+            return std::lower_bound(order.begin(), order.end(), a,
+                                    [&](int32_t c, int v) { return adiabat_host[c] < v; }) -
+                   order.begin();
+        };
+        pl->seg_begin[g] = lower(g + 1);
+        pl->seg_end[g] = lower(g + 2);
+    }
+    const size_t lvl = (size_t)maxnk * (size_t)ncol;
+    const size_t stage_floats = 3 * lvl + 3 * (size_t)ncol;
+    const size_t bi = sizeof(int32_t) * (size_t)ncol;
     hipError_t e;
-    if ((e = hipMalloc((void **)&pl->d_order, sizeof(int32_t) * (size_t)ncol)) ||
-        (e = hipMalloc((void **)&pl->d_counter, sizeof(unsigned int))) ||
+    if ((e = hipMalloc((void **)&pl->d_order, bi)) || (e = hipMalloc((void **)&pl->d_nk, bi)) ||
+        (e = hipMalloc((void **)&pl->d_src, bi)) ||
+        (e = hipMalloc((void **)&pl->d_stage, sizeof(float) * stage_floats)) ||
+        (e = hipMalloc((void **)&pl->d_counter, sizeof(unsigned int) * NSEG)) ||
         (e = hipMalloc((void **)&pl->d_stats, sizeof(unsigned long long) * ST_COUNT)) ||
-        (e = hipMemcpy(pl->d_order, order.data(), sizeof(int32_t) * (size_t)ncol,
-                       hipMemcpyHostToDevice))) {
+        (e = hipMemcpy(pl->d_order, order.data(), bi, hipMemcpyHostToDevice)) ||
+        (e = hipMemcpy(pl->d_nk, nk_sorted.data(), bi, hipMemcpyHostToDevice)) ||
+        (e = hipMemcpy(pl->d_src, src_sorted.data(), bi, hipMemcpyHostToDevice))) {
         destroy(pl);
         return hip_fail(e, "plan allocation");
     }
+    pl->fast = flags & XCAPE_FAST_MATH;
+    pl->warm = flags & XCAPE_WARM_START;
+    pl->block = 64 * waves_per_block;
+    pl->lds_bytes = tile > 0 ? sizeof(float) * 3 * (size_t)tile * 64 * waves_per_block : 0;
+    StageArgs &sa = pl->stage;
+    sa.offsets = offsets; sa.order = pl->d_order; sa.nk_sorted = pl->d_nk;
+    sa.p = p; sa.t = t; sa.td = td; sa.ps = ps; sa.ts = ts; sa.tds = tds;
+    sa.lp = pl->d_stage; sa.lt = sa.lp + lvl; sa.ltd = sa.lt + lvl;
+    sa.sps = sa.ltd + lvl; sa.sts = sa.sps + ncol; sa.stds = sa.sts + ncol;
+    sa.ncol = ncol; sa.maxnk = maxnk;
+    RaggedArgs &a = pl->args;
+    a.order = pl->d_order; a.nk_sorted = pl->d_nk; a.src_sorted = pl->d_src;
+    a.lp = sa.lp; a.lt = sa.lt; a.ltd = sa.ltd; a.sps = sa.sps; a.sts = sa.sts; a.stds = sa.stds;
+    a.ncol = ncol; a.pinc = pinc; a.ml_depth = ml_depth;
+    a.cape = cape; a.cin = cin; a.zout = zmulev; a.mulvl = mulev;
+    a.chunk = chunk; a.tile = tile; a.fallback_tol = 1.0e-3f;
     int dev = 0, ncu = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-    pl->block = 64 * waves_per_block;
-    pl->lds_bytes = sizeof(float) * 3 * (size_t)(tile > 0 ? tile : 1) * 64 * waves_per_block;
-    int per_cu = 0;
-    if (flags & XCAPE_FAST_MATH)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ragged_kernel<MATH_FAST>,
-                                                         pl->block, pl->lds_bytes);
-    else
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ragged_kernel<MATH_ACCURATE>,
-                                                         pl->block, pl->lds_bytes);
-    if (e != hipSuccess || per_cu < 1) per_cu = 1;
-    // enough blocks to fill every CU once, never more than the work needs
-    const int64_t waves_needed = (ncol + chunk - 1) / chunk;
-    const int64_t blocks_needed = (waves_needed + waves_per_block - 1) / waves_per_block;
-    pl->grid = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)ncu * per_cu, blocks_needed));
-    pl->fast = flags & XCAPE_FAST_MATH;
-    RaggedArgs &a = pl->args;
-    a.offsets = offsets; a.order = pl->d_order; a.source = source; a.adiabat = adiabat;
-    a.p = p; a.t = t; a.td = td; a.ps = ps; a.ts = ts; a.tds = tds;
-    a.ncol = ncol; a.pinc = pinc; a.ml_depth = ml_depth;
-    a.cape = cape; a.cin = cin; a.zout = zmulev; a.mulvl = mulev;
-    a.counter = pl->d_counter; a.stats = nullptr; a.chunk = chunk; a.tile = tile;
-    a.warm = (flags & XCAPE_WARM_START) ? 1 : 0;
-    a.fallback_tol = 1.0e-3f;
+    for (int g = 0; g < NSEG; ++g) {
+        const int64_t n = pl->seg_end[g] - pl->seg_begin[g];
+        if (n <= 0) continue;
+        int per_cu = 0;
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu,
+            reinterpret_cast<const void *>(pick(pl->fast, g + 1, pl->warm, tile > 0, false)),
+            pl->block, pl->lds_bytes);
+        if (e != hipSuccess || per_cu < 1) per_cu = 1;
+        // enough blocks to fill every CU once, never more than the work needs
+        const int64_t waves_needed = (n + chunk - 1) / chunk;
+        const int64_t blocks_needed = (waves_needed + waves_per_block - 1) / waves_per_block;
+        pl->seg_grid[g] =
+            (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)ncu * per_cu, blocks_needed));
+    }
     *out = pl;
     return XCAPE_OK;
 }
@@ -331,10 +469,8 @@ int xcape_ragged_plan_run(xcape_ragged_plan *pl, xcape_stats *stats, void *strea
     hipStream_t s = (hipStream_t)stream;   // NULL: the default stream
     hipError_t e;
     if (stats) {
-        // counted run: plain launch with the counters attached (not the graph)
-        xcape_ragged_plan tmp = *pl;
-        tmp.args.stats = pl->d_stats;
-        if ((e = zero_counters(pl->d_stats, ST_COUNT, s)) || (e = enqueue(&tmp, s)))
+        // counted run: direct launches with the counters attached (not the graph)
+        if ((e = zero_counters(pl->d_stats, ST_COUNT, s)) || (e = enqueue(pl, true, s)))
             return hip_fail(e, "ragged launch");
         unsigned long long h[ST_COUNT];
         if ((e = hipMemcpyAsync(h, pl->d_stats, sizeof h, hipMemcpyDeviceToHost, s)) ||
@@ -344,13 +480,13 @@ int xcape_ragged_plan_run(xcape_ragged_plan *pl, xcape_stats *stats, void *strea
         return XCAPE_OK;
     }
     if (!pl->exec) {
-        // capture counter reset + persistent kernel once, replay afterwards
+        // capture counter reset + staging + the segments' kernels once
         hipStream_t cs;
         if ((e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking)))
             return hip_fail(e, "capture stream");
         e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
         if (e == hipSuccess) {
-            hipError_t le = enqueue(pl, cs);
+            hipError_t le = enqueue(pl, false, cs);
             e = hipStreamEndCapture(cs, &pl->graph);
             if (le != hipSuccess) e = le;
         }
@@ -365,7 +501,9 @@ int xcape_ragged_plan_run(xcape_ragged_plan *pl, xcape_stats *stats, void *strea
 int xcape_ragged_plan_info(const xcape_ragged_plan *pl, int *grid, int *block)
 {
     if (!pl) return fail(XCAPE_EINVAL, "plan is NULL");
-    if (grid) *grid = pl->grid;
+    int gmax = 0;
+    for (int g = 0; g < NSEG; ++g) gmax = std::max(gmax, pl->seg_grid[g]);
+    if (grid) *grid = gmax;
     if (block) *block = pl->block;
     return XCAPE_OK;
 }
