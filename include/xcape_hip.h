@@ -140,11 +140,24 @@ int xcape_srh_f32(const float *p, int p_shared, const int32_t *start_lev,
                   double *srh_rm, double *srh_lm, float *rm, float *lm, float *mean6,
                   uint32_t flags, void *stream);
 
+/* Same helicity chain with the Bunkers storm motion (and the mean wind) in
+ * FP64 instead of the reference's FP32; heights and SREH are FP64 in both.
+ * The accuracy yardstick for xcape_srh_f32.  No reference counterpart.
+ * FAST_MATH is ignored, as in xcape_srh_f32. */
+int xcape_srh_f64(const float *p, int p_shared, const int32_t *start_lev,
+                  const float *t, const float *td, const float *u, const float *v,
+                  const float *ps, const float *ts, const float *tds,
+                  const float *us, const float *vs, int64_t nk, int64_t ncol, double depth,
+                  double *srh_rm, double *srh_lm, double *rm, double *lm, double *mean6,
+                  uint32_t flags, void *stream);
+
 /*
  * CAPE/CIN (as xcape_cape_f32) and the helicity chain (as xcape_srh_f32) of
- * the same columns in one pass over the profiles: one kernel, each lane runs
- * the ascent and then the helicity of its column.  No single reference
- * counterpart: it fuses the two calls above for callers that need both.
+ * the same columns in one call: column-contiguous profiles are staged
+ * level-major once for both, and the helicity launch runs on a second stream
+ * beside the ascent (forked after the work queued on `stream`, joined back
+ * before the call returns or anything later on `stream` runs).  No single
+ * reference counterpart: it serves callers that need both calls above.
  */
 int xcape_cape_srh_f32(const float *p, int p_shared, const int32_t *start_lev,
                        const float *t, const float *td, const float *u, const float *v,

@@ -200,6 +200,37 @@ def test_level_major_input_matches_column_contiguous(hip, warm):
         np.testing.assert_array_equal(x, y)
 
 
+def test_staged_chunks_match_level_major(hip, monkeypatch):
+    """Column-contiguous grids larger than one staging chunk are staged and
+    computed chunk by chunk (outputs offset per chunk, 4- or 8-byte
+    elements): forced here to 700 columns a chunk, 3000 columns give five
+    chunks, the last one partial.  Every output equals the unstaged
+    level-major call's."""
+    c = synthetic.sigma_columns(3000, 60, 23, winds=True)
+    lmj = {k: (c[k].T.copy() if c[k].ndim == 2 else c[k]) for k in c}
+    monkeypatch.setenv('XCAPE_STAGE_COLUMNS', '700')
+    cols = ('p', 't', 'td', 'ps', 'ts', 'tds')
+    for kw in (dict(), dict(warm_start='extrapolation'), dict(fp64=True),
+               dict(fast_math=True)):
+        kw.update(pinc=500., source=2, adiabat=3)
+        x = hip.cape(*(c[k] for k in cols), **kw)
+        y = hip.cape(*(lmj[k] for k in cols), level_major=True, **kw)
+        for a, b in zip(x, y):
+            np.testing.assert_array_equal(a, b)
+    wcols = ('p', 't', 'td', 'u', 'v', 'ps', 'ts', 'tds', 'us', 'vs')
+    for fp64 in (False, True):
+        x = hip.srh(*(c[k] for k in wcols), fp64=fp64)
+        y = hip.srh(*(lmj[k] for k in wcols), level_major=True, fp64=fp64)
+        for a, b in zip(x, y):
+            np.testing.assert_array_equal(a, b)
+    x = hip.cape_srh(*(c[k] for k in wcols), pinc=500., source=2, adiabat=3,
+                     warm_start='extrapolation')
+    y = hip.cape_srh(*(lmj[k] for k in wcols), pinc=500., source=2, adiabat=3,
+                     warm_start='extrapolation', level_major=True)
+    for a, b in zip(x, y):
+        np.testing.assert_array_equal(a, b)
+
+
 @pytest.mark.parametrize('adiabat', [1, 2, 3, 4])
 @pytest.mark.parametrize('source', [1, 2, 3])
 def test_warm_start_within_tolerance(hip, oracle_lib, source, adiabat):

@@ -94,6 +94,28 @@ def test_shallow_column_srh_is_zero(hip, oracle_lib):
     _cmp_srh(g, oracle_lib.srh(*args, 1.0e6), 'shallow')
 
 
+@pytest.mark.parametrize('p_shared', [False, True])
+def test_fp64_storm_motion_rerun(hip, oracle_lib, p_shared):
+    """xcape_srh_f64 (Bunkers motion in FP64) against the FP32 chain and the
+    oracle: the FP32 motion within 0.05 m/s of the FP64 one and SRH within
+    1 m^2/s^2; the FP64 motion also within 0.05 m/s of the oracle's FP32."""
+    if p_shared:
+        c = synthetic.pressure_columns(2048, 1002, winds=True)
+        st = oracle_lib.start_levels(c['p'], c['ps'])
+        o_kw = dict(p1d=c['p'], start_lev=st)
+    else:
+        c = synthetic.sigma_columns(2048, 137, 1004, winds=True)
+        o_kw = {}
+    args = [c[k] for k in ('p', 't', 'td', 'u', 'v', 'ps', 'ts', 'tds', 'us', 'vs')]
+    g64 = hip.srh(*args, depth=3000., p_shared=p_shared, fp64=True)
+    g32 = hip.srh(*args, depth=3000., p_shared=p_shared)
+    assert all(x.dtype == np.float64 for x in g64)
+    _cmp_srh(g32, g64, f'fp32 vs fp64 chain (p_shared={p_shared})')
+    o = (oracle_lib.srh(None, *args[1:], 3000., **o_kw) if p_shared
+         else oracle_lib.srh(*args, 3000.))
+    _cmp_srh(g64, o, f'fp64 chain vs oracle (p_shared={p_shared})')
+
+
 @pytest.mark.parametrize('fast', [False, True])
 def test_fused_cape_srh_matches_separate_calls(hip, fast):
     c = synthetic.sigma_columns(2048, 137, 1004, winds=True)

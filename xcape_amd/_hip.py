@@ -62,6 +62,8 @@ SIGNATURES = {
                             _P, _P, _P, _P, _U32, _P, _P]),
     'xcape_srh_f32': (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I64, _I64, _D,
                            _P, _P, _P, _P, _P, _U32, _P]),
+    'xcape_srh_f64': (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I64, _I64, _D,
+                           _P, _P, _P, _P, _P, _U32, _P]),
     'xcape_cape_srh_f32': (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I64, _I64,
                                 _F, _I, _F, _I, _D, _P, _P, _P, _P, _P, _P, _P, _P, _P, _U32,
                                 _P, _P]),
@@ -232,9 +234,10 @@ def cape(p, t, td, ps, ts, tds, *, pinc=500., source=1, ml_depth=500., adiabat=1
 
 
 def srh(p, t, td, u, v, ps, ts, tds, us, vs, *, depth=3000., p_shared=False, start_lev=None,
-        motions=True, level_major=False):
-    """Run xcape_srh_f32.  Returns srh_rm, srh_lm (float64) and, when
-    ``motions``, rm, lm, mean6 as (ncol, 2) float32 (u, v) pairs."""
+        motions=True, level_major=False, fp64=False):
+    """Run xcape_srh_f32 (xcape_srh_f64 when ``fp64``).  Returns srh_rm,
+    srh_lm (float64) and, when ``motions``, rm, lm, mean6 as (ncol, 2) (u, v)
+    pairs, float32 (float64 when ``fp64``)."""
     A = _Args([p, t, td, u, v, ps, ts, tds, us, vs, start_lev])
     p, t, td, u, v, ps, ts, tds, us, vs = (A.f32(a) for a in (p, t, td, u, v, ps, ts, tds,
                                                               us, vs))
@@ -242,14 +245,15 @@ def srh(p, t, td, u, v, ps, ts, tds, us, vs, *, depth=3000., p_shared=False, sta
     ncol, nk = t.shape[::-1] if level_major else t.shape
     srm, slm = (A.empty(ncol, np.float64, t) for _ in range(2))
     if motions:
-        rm, lm, mw = (A.empty(2 * ncol, np.float32, t) for _ in range(3))
+        mdt = np.float64 if fp64 else np.float32
+        rm, lm, mw = (A.empty(2 * ncol, mdt, t) for _ in range(3))
     else:
         rm = lm = mw = None
     flags = (XCAPE_DEVICE_PTRS if A.device else 0) | (XCAPE_LEVEL_MAJOR if level_major else 0)
-    rc = lib().xcape_srh_f32(_ptr(p), 1 if p_shared else 0, _ptr(start_lev), _ptr(t), _ptr(td),
-                             _ptr(u), _ptr(v), _ptr(ps), _ptr(ts), _ptr(tds), _ptr(us),
-                             _ptr(vs), nk, ncol, depth, _ptr(srm), _ptr(slm), _ptr(rm),
-                             _ptr(lm), _ptr(mw), flags, A.stream)
+    fn = lib().xcape_srh_f64 if fp64 else lib().xcape_srh_f32
+    rc = fn(_ptr(p), 1 if p_shared else 0, _ptr(start_lev), _ptr(t), _ptr(td),
+            _ptr(u), _ptr(v), _ptr(ps), _ptr(ts), _ptr(tds), _ptr(us), _ptr(vs), nk, ncol,
+            depth, _ptr(srm), _ptr(slm), _ptr(rm), _ptr(lm), _ptr(mw), flags, A.stream)
     check(rc)
     if not motions:
         return srm, slm

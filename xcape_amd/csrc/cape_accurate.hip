@@ -9,7 +9,7 @@ hipError_t launch_cape_grid_accurate(const CapeGridArgs &a, bool fp64, int kind,
                                      hipStream_t s)
 {
     if (fp64) {
-        if (kind != GRID_CAPE) return hipErrorInvalidValue;
+        if (kind != GRID_CAPE && kind != GRID_SRH) return hipErrorInvalidValue;
         return acc::launch_t<double>(a, kind, block, s);
     }
     return acc::launch_t<float>(a, kind, block, s);

@@ -221,7 +221,9 @@ __global__ void __launch_bounds__(256) cape_grid_kernel(CapeGridArgs a)
     }
 }
 
-// Storm-relative helicity chain, one column per lane.
+// Storm-relative helicity chain, one column per lane.  TB: the storm-motion
+// precision (float: the reference's; double: the FP64 re-run).
+template <typename TB>
 __global__ void __launch_bounds__(256) srh_grid_kernel(CapeGridArgs a)
 {
     const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -230,17 +232,19 @@ __global__ void __launch_bounds__(256) srh_grid_kernel(CapeGridArgs a)
     if (!column_at(a, col, c, nk)) return;
     c.us = a.us[col];
     c.vs = a.vs[col];
-    SrhOut so;
-    srh_column(c, nk, a.depth, 2.0, so);
+    SrhOutT<TB> so;
+    srh_column<TB>(c, nk, a.depth, 2.0, so);
     a.srh_rm[col] = so.srh_rm;
     a.srh_lm[col] = so.srh_lm;
     if (a.rm) {
-        a.rm[2 * col] = so.rm_u;
-        a.rm[2 * col + 1] = so.rm_v;
-        a.lm[2 * col] = so.lm_u;
-        a.lm[2 * col + 1] = so.lm_v;
-        a.mean6[2 * col] = so.mean_u;
-        a.mean6[2 * col + 1] = so.mean_v;
+        TB *rm = static_cast<TB *>(a.rm), *lm = static_cast<TB *>(a.lm),
+           *mw = static_cast<TB *>(a.mean6);
+        rm[2 * col] = so.rm_u;
+        rm[2 * col + 1] = so.rm_v;
+        lm[2 * col] = so.lm_u;
+        lm[2 * col + 1] = so.lm_v;
+        mw[2 * col] = so.mean_u;
+        mw[2 * col + 1] = so.mean_v;
     }
 }
 
@@ -275,7 +279,13 @@ hipError_t launch_t(const CapeGridArgs &a, int kind, int block, hipStream_t s)
 {
     if (a.ncol <= 0) return hipSuccess;
     if constexpr (sizeof(T) == 8) {
-        // the FP64 yardstick covers CAPE with the reference's cold start only
+        // the FP64 yardstick: CAPE with the reference's cold start, and the
+        // helicity chain with FP64 storm motion
+        if (kind == GRID_SRH) {
+            dim3 grid((unsigned)((a.ncol + block - 1) / block));
+            srh_grid_kernel<double><<<grid, block, 0, s>>>(a);
+            return hipGetLastError();
+        }
         if (kind != GRID_CAPE || a.warm) return hipErrorInvalidValue;
         return launch_cape<T, 0, false>(a, block, s);
     } else {
@@ -287,7 +297,7 @@ hipError_t launch_t(const CapeGridArgs &a, int kind, int block, hipStream_t s)
             return a.surrogate ? launch_cape<T, 1, true>(a, block, s)
                                : launch_cape<T, 1, false>(a, block, s);
         case GRID_SRH:
-            srh_grid_kernel<<<grid, block, 0, s>>>(a);
+            srh_grid_kernel<float><<<grid, block, 0, s>>>(a);
             return hipGetLastError();
         default: return hipErrorInvalidValue;
         }

@@ -28,7 +28,8 @@ struct CapeGridArgs {
     const float *us, *vs;        // (ncol,)
     double depth;
     double *srh_rm, *srh_lm;     // (ncol,)
-    float *rm, *lm, *mean6;      // (ncol, 2) u, v pairs, or nullptr
+    void *rm, *lm, *mean6;       // (ncol, 2) u, v pairs (float; double in the FP64
+                                 // re-run), or nullptr
     // warm start of the moist-adiabat iteration
     int warm;                    // 0: reference loops, cold start; 1: warm-started
                                  // loop; 2: the same loop, every sub-step cold

@@ -66,6 +66,15 @@ hipStream_t thread_stream()
     return s;
 }
 
+// A second stream of the calling thread (default priority), for work that
+// runs beside the call's own stream and joins it again through events.
+hipStream_t aux_stream()
+{
+    static thread_local hipStream_t s = nullptr;
+    if (!s && hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) s = nullptr;
+    return s;
+}
+
 __global__ void zero_u64(unsigned long long *p, int n)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -137,6 +146,31 @@ int cape_block(bool warm)
 hipError_t launch_grid(const CapeGridArgs &a, bool cape, bool srh, int math, hipStream_t s)
 {
     hipError_t e = hipSuccess;
+    if (cape && srh) {
+        // the helicity chain (memory- and FP64-bound) beside the ascent
+        // (FP32-bound): forked onto the thread's second stream after the
+        // work already queued on s, joined back before anything later on s
+        hipStream_t s2 = aux_stream();
+        // the thread's own fork/join events, created once (recording an
+        // event again only moves it to the new point of its stream)
+        static thread_local hipEvent_t fork = nullptr, join = nullptr;
+        if (!fork && hipEventCreateWithFlags(&fork, hipEventDisableTiming) != hipSuccess)
+            fork = nullptr;
+        if (!join && hipEventCreateWithFlags(&join, hipEventDisableTiming) != hipSuccess)
+            join = nullptr;
+        if (s2 && s2 != s && fork && join) {
+            if ((e = hipEventRecord(fork, s)) != hipSuccess ||
+                (e = hipStreamWaitEvent(s2, fork, 0)) != hipSuccess)
+                return e;
+            e = launch_cape_grid_accurate(a, false, GRID_SRH, 256, s2);
+            const hipError_t er = hipEventRecord(join, s2);
+            if (e == hipSuccess) e = er;
+            if (e == hipSuccess) e = launch_grid(a, true, false, math, s);
+            // join even when a launch failed: s must not run ahead of s2
+            const hipError_t ej = hipStreamWaitEvent(s, join, 0);
+            return e != hipSuccess ? e : ej;
+        }
+    }
     if (cape) {
         const int block = cape_block(a.warm);
         if (math == 2)
@@ -148,7 +182,8 @@ hipError_t launch_grid(const CapeGridArgs &a, bool cape, bool srh, int math, hip
         if (e != hipSuccess) return e;
     }
     // the helicity chain keeps the reference's precisions in every math mode
-    if (srh) e = launch_cape_grid_accurate(a, false, GRID_SRH, 256, s);
+    // but the FP64 one (storm motion in double)
+    if (srh) e = launch_cape_grid_accurate(a, math == 2, GRID_SRH, 256, s);
     return e;
 }
 
@@ -167,6 +202,12 @@ hipError_t run_grid(const CapeGridArgs &a0, bool cape, bool srh, int math, hipSt
     const int64_t per_col = (int64_t)narr * nk * (int64_t)sizeof(float);
     int64_t chunk = ((int64_t)8 << 30) / per_col;
     if (chunk < ((int64_t)1 << 20)) chunk = (int64_t)1 << 20;
+    // XCAPE_STAGE_COLUMNS: columns per staged chunk, overriding the 8 GiB
+    // sizing (read on every call; for tests of the chunked path)
+    if (const char *env = getenv("XCAPE_STAGE_COLUMNS")) {
+        const long long v = atoll(env);
+        if (v > 0) chunk = v;
+    }
     if (chunk > a0.ncol) chunk = a0.ncol;
     DevBuf<float> scratch;
     hipError_t e = scratch.alloc((size_t)narr * (size_t)nk * (size_t)chunk, s);
@@ -200,7 +241,12 @@ hipError_t run_grid(const CapeGridArgs &a0, bool cape, bool srh, int math, hipSt
             a.v = dst[iv];
             a.us += c0; a.vs += c0;
             a.srh_rm += c0; a.srh_lm += c0;
-            if (a.rm) { a.rm += 2 * c0; a.lm += 2 * c0; a.mean6 += 2 * c0; }
+            if (a.rm) {
+                const int64_t b = 2 * c0 * (math == 2 ? 8 : 4);
+                a.rm = static_cast<char *>(a.rm) + b;
+                a.lm = static_cast<char *>(a.lm) + b;
+                a.mean6 = static_cast<char *>(a.mean6) + b;
+            }
         }
         if ((e = launch_grid(a, cape, srh, math, s)) != hipSuccess) return e;
     }
@@ -222,7 +268,7 @@ struct GridCall {
     void *cape = nullptr, *cin = nullptr, *zout = nullptr;
     int32_t *mulev = nullptr;
     double *srh_rm = nullptr, *srh_lm = nullptr;
-    float *rm = nullptr, *lm = nullptr, *mean6 = nullptr;
+    void *rm = nullptr, *lm = nullptr, *mean6 = nullptr;  // float, or double when fp64
     uint32_t flags = 0;
     xcape_stats *stats = nullptr;
     void *stream = nullptr;
@@ -270,9 +316,9 @@ int grid_entry(const GridCall &c)
     const size_t np = c.p_shared ? (size_t)c.nk : n2;
     const size_t ncol = (size_t)c.ncol;
 
-    DevBuf<float> dp_, dt_, dtd_, dps_, dts_, dtds_, du_, dv_, dus_, dvs_, drm_, dlm_, dmw_;
+    DevBuf<float> dp_, dt_, dtd_, dps_, dts_, dtds_, du_, dv_, dus_, dvs_;
     DevBuf<int32_t> dst_, dmul_;
-    DevBuf<unsigned char> dcape_, dcin_, dz_;
+    DevBuf<unsigned char> dcape_, dcin_, dz_, drm_, dlm_, dmw_;
     DevBuf<double> dsrm_, dslm_;
     DevBuf<unsigned long long> dstats_;
     CapeGridArgs a{};
@@ -325,8 +371,8 @@ int grid_entry(const GridCall &c)
             a.u = du_.ptr; a.v = dv_.ptr; a.us = dus_.ptr; a.vs = dvs_.ptr;
             a.srh_rm = dsrm_.ptr; a.srh_lm = dslm_.ptr;
             if (c.rm) {
-                XC_TRY(drm_.alloc(2 * ncol, s)); XC_TRY(dlm_.alloc(2 * ncol, s));
-                XC_TRY(dmw_.alloc(2 * ncol, s));
+                XC_TRY(drm_.alloc(2 * ncol * osz, s)); XC_TRY(dlm_.alloc(2 * ncol * osz, s));
+                XC_TRY(dmw_.alloc(2 * ncol * osz, s));
                 a.rm = drm_.ptr; a.lm = dlm_.ptr; a.mean6 = dmw_.ptr;
             }
         }
@@ -350,9 +396,10 @@ int grid_entry(const GridCall &c)
             XC_TRY(dsrm_.download(c.srh_rm, ncol, s));
             XC_TRY(dslm_.download(c.srh_lm, ncol, s));
             if (c.rm) {
-                XC_TRY(drm_.download(c.rm, 2 * ncol, s));
-                XC_TRY(dlm_.download(c.lm, 2 * ncol, s));
-                XC_TRY(dmw_.download(c.mean6, 2 * ncol, s));
+                const size_t nb = 2 * ncol * osz;
+                XC_TRY(hipMemcpyAsync(c.rm, drm_.ptr, nb, hipMemcpyDeviceToHost, s));
+                XC_TRY(hipMemcpyAsync(c.lm, dlm_.ptr, nb, hipMemcpyDeviceToHost, s));
+                XC_TRY(hipMemcpyAsync(c.mean6, dmw_.ptr, nb, hipMemcpyDeviceToHost, s));
             }
         }
     }
@@ -432,6 +479,24 @@ int xcape_srh_f32(const float *p, int p_shared, const int32_t *start_lev, const 
     c.nk = nk; c.ncol = ncol; c.depth = depth;
     c.srh_rm = srh_rm; c.srh_lm = srh_lm; c.rm = rm; c.lm = lm; c.mean6 = mean6;
     c.flags = flags & ~XCAPE_FAST_MATH; c.stream = stream;
+    return grid_entry(c);
+}
+
+int xcape_srh_f64(const float *p, int p_shared, const int32_t *start_lev, const float *t,
+                  const float *td, const float *u, const float *v, const float *ps,
+                  const float *ts, const float *tds, const float *us, const float *vs,
+                  int64_t nk, int64_t ncol, double depth, double *srh_rm, double *srh_lm,
+                  double *rm, double *lm, double *mean6, uint32_t flags, void *stream)
+{
+    GridCall c;
+    c.kind = GRID_SRH;
+    c.fp64 = true;
+    c.p = p; c.p_shared = p_shared; c.start = start_lev;
+    c.t = t; c.td = td; c.ps = ps; c.ts = ts; c.tds = tds;
+    c.u = u; c.v = v; c.us = us; c.vs = vs;
+    c.nk = nk; c.ncol = ncol; c.depth = depth;
+    c.srh_rm = srh_rm; c.srh_lm = srh_lm; c.rm = rm; c.lm = lm; c.mean6 = mean6;
+    c.flags = flags; c.stream = stream;
     return grid_entry(c);
 }
 

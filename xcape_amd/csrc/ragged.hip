@@ -27,7 +27,10 @@
 //     segment from the counter until the segment is drained (every wave
 //     reaches the exit test, so the grid always drains).  Optionally the
 //     first `tile` levels of a wave's columns are copied to LDS first (the
-//     autotuner decides whether that pays).
+//     autotuner decides whether that pays).  The segments' launches are
+//     parallel branches of the graph (forked after the staging, joined
+//     before the graph ends): one segment alone is often too small to fill
+//     the device, and a segment's tail then overlaps the others' work.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -297,7 +300,12 @@ struct xcape_ragged_plan {
 
 namespace {
 
-hipError_t enqueue(const xcape_ragged_plan *pl, bool stats, hipStream_t s)
+// seg_streams: one stream per segment to fork the segments' launches onto
+// (after the staging on s, joined back to s), or nullptr to queue them all
+// on s.
+hipError_t enqueue(const xcape_ragged_plan *pl, bool stats, hipStream_t s,
+                   const hipStream_t *seg_streams = nullptr, hipEvent_t fork = nullptr,
+                   const hipEvent_t *join = nullptr)
 {
     reset_counters<<<1, 64, 0, s>>>(pl->d_counter);
     hipError_t e = hipGetLastError();
@@ -307,8 +315,14 @@ hipError_t enqueue(const xcape_ragged_plan *pl, bool stats, hipStream_t s)
                      (unsigned)((sa.maxnk + STAGE_TILE - 1) / STAGE_TILE));
     stage_kernel<<<sgrid, 256, 0, s>>>(sa);
     if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (seg_streams && (e = hipEventRecord(fork, s)) != hipSuccess) return e;
     for (int g = 0; g < NSEG; ++g) {
         if (pl->seg_end[g] <= pl->seg_begin[g]) continue;
+        hipStream_t sg = s;
+        if (seg_streams) {
+            sg = seg_streams[g];
+            if ((e = hipStreamWaitEvent(sg, fork, 0)) != hipSuccess) return e;
+        }
         RaggedArgs a = pl->args;
         a.seg_begin = pl->seg_begin[g];
         a.seg_end = pl->seg_end[g];
@@ -317,8 +331,11 @@ hipError_t enqueue(const xcape_ragged_plan *pl, bool stats, hipStream_t s)
         const Kernel k = pick(pl->fast, g + 1, pl->warm, pl->args.tile > 0, stats);
         void *kargs[] = {&a};
         e = hipLaunchKernel(reinterpret_cast<const void *>(k), dim3(pl->seg_grid[g]),
-                            dim3(pl->block), kargs, pl->lds_bytes, s);
+                            dim3(pl->block), kargs, pl->lds_bytes, sg);
         if (e != hipSuccess) return e;
+        if (seg_streams && ((e = hipEventRecord(join[g], sg)) != hipSuccess ||
+                            (e = hipStreamWaitEvent(s, join[g], 0)) != hipSuccess))
+            return e;
     }
     return hipSuccess;
 }
@@ -480,18 +497,30 @@ int xcape_ragged_plan_run(xcape_ragged_plan *pl, xcape_stats *stats, void *strea
         return XCAPE_OK;
     }
     if (!pl->exec) {
-        // capture counter reset + staging + the segments' kernels once
-        hipStream_t cs;
-        if ((e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking)))
-            return hip_fail(e, "capture stream");
-        e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+        // capture counter reset + staging + the segments' kernels once, the
+        // segments forked onto streams of their own (parallel graph branches)
+        hipStream_t cs = nullptr, ss[NSEG] = {};
+        hipEvent_t fork = nullptr, join[NSEG] = {};
+        e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+        for (int g = 0; g < NSEG && e == hipSuccess; ++g)
+            if ((e = hipStreamCreateWithFlags(&ss[g], hipStreamNonBlocking)) == hipSuccess)
+                e = hipEventCreateWithFlags(&join[g], hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&fork, hipEventDisableTiming);
         if (e == hipSuccess) {
-            hipError_t le = enqueue(pl, false, cs);
-            e = hipStreamEndCapture(cs, &pl->graph);
-            if (le != hipSuccess) e = le;
+            e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+            if (e == hipSuccess) {
+                hipError_t le = enqueue(pl, false, cs, ss, fork, join);
+                e = hipStreamEndCapture(cs, &pl->graph);
+                if (le != hipSuccess) e = le;
+            }
         }
         if (e == hipSuccess) e = hipGraphInstantiate(&pl->exec, pl->graph, nullptr, nullptr, 0);
-        (void)hipStreamDestroy(cs);
+        for (int g = 0; g < NSEG; ++g) {
+            if (join[g]) (void)hipEventDestroy(join[g]);
+            if (ss[g]) (void)hipStreamDestroy(ss[g]);
+        }
+        if (fork) (void)hipEventDestroy(fork);
+        if (cs) (void)hipStreamDestroy(cs);
         if (e != hipSuccess) return hip_fail(e, "graph capture");
     }
     if ((e = hipGraphLaunch(pl->exec, s))) return hip_fail(e, "graph launch");

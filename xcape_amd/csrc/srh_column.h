@@ -25,10 +25,13 @@ namespace xcape {
 
 #define XC_SPD(x) ((double)(float)(x))
 
-struct SrhOut {
+// TB: the storm-motion precision, float as in the reference (Bunkers is REAL
+// there) or double for the FP64 re-run
+template <typename TB> struct SrhOutT {
     double srh_rm, srh_lm;
-    float rm_u, rm_v, lm_u, lm_v, mean_u, mean_v;
+    TB rm_u, rm_v, lm_u, lm_v, mean_u, mean_v;
 };
+using SrhOut = SrhOutT<float>;
 
 // virtual temperature of one level (stdheight_2D_model_lev.f90:102-127), FP64
 __device__ __forceinline__ double std_tv(double T, double Td, double P)
@@ -53,35 +56,37 @@ __device__ __forceinline__ double std_dh(double tv, double tv_prev, double P, do
 
 // Column accessor C: P(k), Tc(k), Tdc(k), U(k), V(k) for k = 0..nk-1 with
 // k = 0 the surface (hPa, degC, m/s).  hs: surface height (2 m in calc_srh).
-template <class C>
-__device__ void srh_column(const C &col, int nk, double depth, double hs, SrhOut &out)
+// TB = float is the reference's arithmetic; TB = double evaluates the storm
+// motion (and the helicity's storm-relative winds) in FP64 as well.
+template <typename TB, class C>
+__device__ void srh_column(const C &col, int nk, double depth, double hs, SrhOutT<TB> &out)
 {
 #pragma clang fp contract(off)
-    const float lv[12] = {500.f, 1000.f, 1500.f, 2000.f, 2500.f, 3000.f,
-                          3500.f, 4000.f, 4500.f, 5000.f, 5500.f, 6000.f};
-    float iu[12], iv[12];
+    const TB lv[12] = {TB(500), TB(1000), TB(1500), TB(2000), TB(2500), TB(3000),
+                       TB(3500), TB(4000), TB(4500), TB(5000), TB(5500), TB(6000)};
+    TB iu[12], iv[12];
 #pragma unroll
-    for (int i = 0; i < 12; ++i) { iu[i] = -999999.0f; iv[i] = -999999.0f; }
+    for (int i = 0; i < 12; ++i) { iu[i] = TB(-999999.0); iv[i] = TB(-999999.0); }
 
     // ---- pass 1: heights and Bunkers brackets (increasing orientation) ----
     const double Ps = (double)col.P(0);
     double tv_prev = std_tv((double)col.Tc(0), (double)col.Tdc(0), Ps);
     double P_prev = Ps;
     double H = hs;
-    float z_prev = (float)hs;
-    float u_prev = col.U(0), v_prev = col.V(0);
-    const float zsurf = (float)hs;
+    TB z_prev = (TB)hs;
+    TB u_prev = col.U(0), v_prev = col.V(0);
+    const TB zsurf = (TB)hs;
     for (int k = 1; k < nk; ++k) {
         const double Pk = (double)col.P(k);
         const double tv = std_tv((double)col.Tc(k), (double)col.Tdc(k), Pk);
         H = H + std_dh(tv, tv_prev, Pk, P_prev);
-        const float z = (float)H;
-        const float uk = col.U(k), vk = col.V(k);
+        const TB z = (TB)H;
+        const TB uk = col.U(k), vk = col.V(k);
 #pragma unroll
         for (int i = 0; i < 12; ++i) {
             if (z_prev <= lv[i] && z > lv[i]) {
-                const float w2 = (lv[i] - z_prev) / (z - z_prev);
-                const float w1 = (float)(1.0 - (double)w2);
+                const TB w2 = (lv[i] - z_prev) / (z - z_prev);
+                const TB w1 = (TB)(1.0 - (double)w2);
                 iu[i] = w1 * u_prev + w2 * uk;
                 iv[i] = w1 * v_prev + w2 * vk;
             }
@@ -94,23 +99,23 @@ __device__ void srh_column(const C &col, int nk, double depth, double hs, SrhOut
         // bracket is z(k) <= h < z(k-1); the top-down search keeps the
         // highest k, i.e. the last bracket of this bottom-up walk
 #pragma unroll
-        for (int i = 0; i < 12; ++i) { iu[i] = -999999.0f; iv[i] = -999999.0f; }
+        for (int i = 0; i < 12; ++i) { iu[i] = TB(-999999.0); iv[i] = TB(-999999.0); }
         tv_prev = std_tv((double)col.Tc(0), (double)col.Tdc(0), Ps);
         P_prev = Ps;
         H = hs;
-        z_prev = (float)hs;
+        z_prev = (TB)hs;
         u_prev = col.U(0); v_prev = col.V(0);
         for (int k = 1; k < nk; ++k) {
             const double Pk = (double)col.P(k);
             const double tv = std_tv((double)col.Tc(k), (double)col.Tdc(k), Pk);
             H = H + std_dh(tv, tv_prev, Pk, P_prev);
-            const float z = (float)H;
-            const float uk = col.U(k), vk = col.V(k);
+            const TB z = (TB)H;
+            const TB uk = col.U(k), vk = col.V(k);
 #pragma unroll
             for (int i = 0; i < 12; ++i) {
                 if (z <= lv[i] && z_prev > lv[i]) {
-                    const float w2 = (lv[i] - z) / (z_prev - z);
-                    const float w1 = (float)(1.0 - (double)w2);
+                    const TB w2 = (lv[i] - z) / (z_prev - z);
+                    const TB w1 = (TB)(1.0 - (double)w2);
                     iu[i] = w1 * uk + w2 * u_prev;
                     iv[i] = w1 * vk + w2 * v_prev;
                 }
@@ -121,23 +126,25 @@ __device__ void srh_column(const C &col, int nk, double depth, double hs, SrhOut
     }
 
     // ---- Bunkers (Bunkers_model_lev.f90:123-177), accumulators from zero ----
-    const float u0 = col.U(0), v0 = col.V(0);
-    float mu = 0.0f + u0, mv = 0.0f + v0;
+    const TB u0 = col.U(0), v0 = col.V(0);
+    TB mu = TB(0) + u0, mv = TB(0) + v0;
 #pragma unroll
     for (int i = 0; i < 12; ++i) { mu = mu + iu[i]; mv = mv + iv[i]; }
-    mu = mu / 13.0f;
-    mv = mv / 13.0f;
-    const float ushr_u = ((0.0f + iu[10]) + iu[11]) / 2.0f;
-    const float vshr_u = ((0.0f + iv[10]) + iv[11]) / 2.0f;
-    const float ushr_d = ((0.0f + u0) + iu[0]) / 2.0f;
-    const float vshr_d = ((0.0f + v0) + iv[0]) / 2.0f;
-    const float ushr = ushr_u - ushr_d;
-    const float vshr = vshr_u - vshr_d;
-    const float mag = sqrtf(ushr * ushr + vshr * vshr);
-    out.rm_u = mu + 7.5f * vshr / mag;
-    out.rm_v = mv - 7.5f * ushr / mag;
-    out.lm_u = mu - 7.5f * vshr / mag;
-    out.lm_v = mv + 7.5f * ushr / mag;
+    mu = mu / TB(13);
+    mv = mv / TB(13);
+    const TB ushr_u = ((TB(0) + iu[10]) + iu[11]) / TB(2);
+    const TB vshr_u = ((TB(0) + iv[10]) + iv[11]) / TB(2);
+    const TB ushr_d = ((TB(0) + u0) + iu[0]) / TB(2);
+    const TB vshr_d = ((TB(0) + v0) + iv[0]) / TB(2);
+    const TB ushr = ushr_u - ushr_d;
+    const TB vshr = vshr_u - vshr_d;
+    TB mag;
+    if constexpr (sizeof(TB) == 4) mag = sqrtf(ushr * ushr + vshr * vshr);
+    else mag = sqrt(ushr * ushr + vshr * vshr);
+    out.rm_u = mu + TB(7.5) * vshr / mag;
+    out.rm_v = mv - TB(7.5) * ushr / mag;
+    out.lm_u = mu - TB(7.5) * vshr / mag;
+    out.lm_v = mv + TB(7.5) * ushr / mag;
     out.mean_u = mu;
     out.mean_v = mv;
 
