@@ -153,8 +153,8 @@ def accuracy(outs, inp, sample, fp64_outs):
 def pmc_summary(fetch_dir, write_dir, out, ncol, fast_math):
     """HBM bytes per launch of the ascent kernels from two rocprofv3 counter
     runs of --profile-pass (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch),
-    each scaled by what the same counter reads for the pass's 1 GiB device
-    copy.  Writes the JSON summary that the timed runs report as
+    each scaled by what the same counter reads for the pass's 1 GiB
+    device-to-device copy (HIP's copyBuffer kernel).  Writes the JSON summary that the timed runs report as
     roofline.traffic."""
     import csv
     import glob
@@ -171,18 +171,38 @@ def pmc_summary(fetch_dir, write_dir, out, ncol, fast_math):
             acc[k] = (name, acc.get(k, (name, 0.0))[1] + v)
         return acc
 
-    res = {'ncol': ncol, 'fast_math': fast_math}
+    def first(disp, pred):
+        """Counter value of the first dispatch whose kernel name matches."""
+        for k in sorted(disp):
+            if pred(disp[k][0]):
+                return disp[k][1]
+        return None
+
+    # the pass runs the plain CAPE-only call, then the warm one (setup pass +
+    # persistent ascent), then a combined step: the first dispatch of each
+    kernels = {
+        'plain': lambda n: 'cape_grid_kernel<float' in n,
+        'warm_setup': lambda n: 'warm_setup_kernel<true, true, true, false>' in n,
+        'warm_ascent': lambda n: 'warm_ascent_kernel<true, true, true, false>' in n,
+    }
+    res = {'ncol': ncol, 'fast_math': fast_math,
+           'kernels': {'plain': 'cape_grid_kernel<float, unsigned int, true, true, false>',
+                       'warm': 'warm_setup_kernel<true, true, true, false> + '
+                               'warm_ascent_kernel<true, true, true, false>'}}
     gib = float(1 << 30)
     for label, d, counter in (('fetch', fetch_dir, 'FETCH_SIZE'), ('write', write_dir, 'WRITE_SIZE')):
         disp = per_dispatch(d, counter)
-        copies = [v for n, v in disp.values() if 'direct_copy_kernel' in n]
+        copies = [v for n, v in disp.values() if 'copyBuffer' in n]
         cal = max(copies) * 1024.0 if copies else None
         factor = gib / cal if cal else 1.0
         res[f'{label}_calibration'] = {'copy_bytes': gib, 'counter_bytes': cal, 'factor': factor}
-        for mode, tag in (('plain', ', 0, false, false>'), ('warm', ', 1, true, false>')):
-            vals = [v for n, v in disp.values() if 'cape_grid_kernel' in n and tag in n]
-            if vals:
-                res[f'{mode}_{label}_bytes'] = vals[0] * 1024.0 * factor
+        for mode, pred in kernels.items():
+            v = first(disp, pred)
+            if v is not None:
+                res[f'{mode}_{label}_bytes'] = v * 1024.0 * factor
+        if f'warm_setup_{label}_bytes' in res and f'warm_ascent_{label}_bytes' in res:
+            res[f'warm_{label}_bytes'] = (res[f'warm_setup_{label}_bytes'] +
+                                          res[f'warm_ascent_{label}_bytes'])
     for mode in ('plain', 'warm'):
         if f'{mode}_fetch_bytes' in res and f'{mode}_write_bytes' in res:
             res[f'{mode}_hbm_bytes_per_launch'] = (res[f'{mode}_fetch_bytes'] +
@@ -304,6 +324,15 @@ def main():
 
     torch.cuda.set_device(0)
     ncol = args.ncol
+    if args.profile_pass:
+        # the counters' calibration: a 1 GiB device-to-device copy (HIP's
+        # copyBuffer blit kernel; the summary takes the largest such dispatch)
+        x = torch.ones(1 << 28, dtype=torch.float32, device='cuda')
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        del x, y
     log(f'[bench] generating {ncol} columns x {NLEV} levels on the GPU')
     inp = synthetic.sigma_columns_torch(ncol, NLEV, SEED, device='cuda', winds=True)
     torch.cuda.synchronize()
@@ -327,11 +356,6 @@ def main():
         return _hip.cape(*cape_lm, warm_start=warm, stats=stats, level_major=True, **kw)
 
     if args.profile_pass:
-        x = torch.ones(1 << 28, dtype=torch.float32, device='cuda')
-        y = torch.empty_like(x)
-        torch.cuda.synchronize()
-        y.copy_(x)
-        torch.cuda.synchronize()
         for warm in (False, True):
             cape_only(warm)
             torch.cuda.synchronize()
@@ -353,6 +377,9 @@ def main():
     for m, w in (('plain', False), ('cold_loop', 'cold'), ('extrapolation', 'extrapolation'),
                  ('warm', True)):
         _, ms_cape[m], _ = timed(lambda: cape_only(w), args.steps, args.warmup, stream)
+    # the helicity chain alone, on column-contiguous input as in the combined
+    # step (its staging transpose included)
+    _, ms_srh, _ = timed(lambda: _hip.srh(*a, depth=DEPTH), args.steps, args.warmup, stream)
 
     log('[bench] timing the combined step')
     outs_p, ms_plain, _ = timed(lambda: step(False), args.steps, args.warmup, stream)
@@ -415,7 +442,9 @@ def main():
                    'ncol': ncol, 'nlev': NLEV, 'warm_start': warm,
                    'fast_math': bool(args.fast_math), 'parallelism': 'single GPU'},
         'roofline': {'bound': 'mfma', 'pipe': 'VALU FP32 (the ascent has no matrix work)',
-                     'kernel': f'cape_grid_kernel ({dom}), CAPE-only launch',
+                     'kernel': ('warm_setup_kernel + warm_ascent_kernel (the warm-started '
+                                'CAPE-only call)' if warm else
+                                'cape_grid_kernel (the plain CAPE-only call)'),
                      'achieved': achieved, 'peak': FP32_PEAK_TFLOPS, 'unit': 'TFLOP/s',
                      'frac': achieved / FP32_PEAK_TFLOPS, 'traffic': traffic,
                      'ops_per_launch': ops, 'ms_per_launch': ms_cape[dom],
@@ -424,6 +453,7 @@ def main():
                          'accuracy': acc_plain},
         'speedup_vs_plain': ms_plain / ms,
         'attribution_cape_only_ms': ms_cape,
+        'srh_only_ms': ms_srh,
         'kernel_counts': counts,
         'surrogate': sur,
         'accuracy': acc,

@@ -115,17 +115,20 @@ def test_sigma_synthetic(hip, oracle_lib, source, adiabat):
 
 @pytest.mark.parametrize('adiabat', [1, 3])
 @pytest.mark.parametrize('source', [1, 2, 3])
-def test_pressure_synthetic(hip, oracle_lib, source, adiabat):
-    """37 fixed pressure levels with NaN below ground, device start levels."""
+@pytest.mark.parametrize('warm', [False, 'extrapolation'])
+def test_pressure_synthetic(hip, oracle_lib, source, adiabat, warm):
+    """37 fixed pressure levels with NaN below ground, device start levels
+    (columns of different level counts), plain and warm-started."""
     c = synthetic.pressure_columns(4096, 1001)
     st = oracle_lib.start_levels(c['p'], c['ps'])
     g = hip.cape(c['p'], c['t'], c['td'], c['ps'], c['ts'], c['tds'], pinc=500.,
-                 source=source, ml_depth=500., adiabat=adiabat, p_shared=True)
+                 source=source, ml_depth=500., adiabat=adiabat, p_shared=True,
+                 warm_start=warm)
     o = oracle_lib.cape(None, c['t'], c['td'], c['ps'], c['ts'], c['tds'], 500., source, 500.,
                         adiabat, p1d=c['p'], start_lev=st)
     assert np.all(np.isfinite(g[0])) and np.all(np.isfinite(g[1]))
-    assert not _report(f'pres s{source} a{adiabat} cape', g[0], o[0]).any()
-    assert not _report(f'pres s{source} a{adiabat} cin', g[1], o[1]).any()
+    assert not _report(f'pres s{source} a{adiabat} {warm} cape', g[0], o[0]).any()
+    assert not _report(f'pres s{source} a{adiabat} {warm} cin', g[1], o[1]).any()
 
 
 def test_fp64_matches_fp64_oracle(hip, oracle_lib):
@@ -198,6 +201,48 @@ def test_level_major_input_matches_column_contiguous(hip, warm):
     for x, y in zip(hip.cape(*pa, p_shared=True, **kw),
                     hip.cape(*pl, p_shared=True, level_major=True, **kw)):
         np.testing.assert_array_equal(x, y)
+
+
+@pytest.mark.parametrize('adiabat', [1, 3, 4])
+@pytest.mark.parametrize('source', [1, 2, 3])
+def test_warm_grid_matches_ragged_warm(hip, source, adiabat):
+    """The grid's warm-started ascent (setup pass, then lanes taking columns
+    from a counter until the grid is drained) and the ragged kernel's (one
+    column per lane start to end) run the same column code in the accurate
+    math mode: results and work counters must be identical, bit for bit."""
+    import torch
+
+    from xcape_amd.ragged import RaggedBatch, RaggedPlan
+    c = synthetic.sigma_columns(3000, 60, 31, winds=False)
+    g = hip.cape(c['p'], c['t'], c['td'], c['ps'], c['ts'], c['tds'], pinc=500.,
+                 source=source, adiabat=adiabat, warm_start='extrapolation', stats=True)
+    n = c['t'].shape[0]
+    d = dict(offsets=np.arange(n + 1, dtype=np.int64) * 60, p=c['p'].ravel(),
+             t=c['t'].ravel(), td=c['td'].ravel(), ps=c['ps'], ts=c['ts'], tds=c['tds'],
+             source=np.full(n, source, np.int32), adiabat=np.full(n, adiabat, np.int32))
+    b = RaggedBatch.from_dict(d)
+    plan = RaggedPlan(b, pinc=500., ml_depth=500., chunk=64, tile=0, waves=4, warm_start=True)
+    st = plan.run(stats=True)
+    torch.cuda.synchronize()
+    r = [x.cpu().numpy() for x in (b.cape, b.cin, b.mulev, b.zmulev)]
+    plan.close()
+    for x, y in zip(g[:4], r):
+        np.testing.assert_array_equal(x, y)
+    for key in ('levels', 'substeps', 'iterations', 'fallbacks', 'nonconverged'):
+        assert g[4][key] == st[key], key
+
+
+def test_deep_columns_take_the_tile_transpose(hip):
+    """Columns of more than 255 levels are staged by the tiled transpose
+    (the slab kernel holds up to 255 levels of 64 columns in LDS): results
+    equal the level-major call's."""
+    c = synthetic.sigma_columns(500, 300, 41)
+    lm = (c['p'].T.copy(), c['t'].T.copy(), c['td'].T.copy(), c['ps'], c['ts'], c['tds'])
+    args = (c['p'], c['t'], c['td'], c['ps'], c['ts'], c['tds'])
+    for warm in (False, 'extrapolation'):
+        kw = dict(pinc=500., source=2, adiabat=3, warm_start=warm)
+        for x, y in zip(hip.cape(*args, **kw), hip.cape(*lm, level_major=True, **kw)):
+            np.testing.assert_array_equal(x, y)
 
 
 def test_staged_chunks_match_level_major(hip, monkeypatch):

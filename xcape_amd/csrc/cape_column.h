@@ -184,9 +184,16 @@ template <typename T> struct Parcel {
     Env<T> e;       // environment at kmax
 };
 
-template <typename T, int MATH, class C>
+// An observer of the level environments the source scans evaluate:
+// sink(k, env) for each level k they compute (a caller that also needs the
+// environments keeps them instead of computing them again).
+struct NoEnvSink {
+    template <typename E> XC_HD __forceinline__ void operator()(int, const E &) const {}
+};
+
+template <typename T, int MATH, class C, class Sink = NoEnvSink>
 XC_HD __forceinline__ bool parcel_source(const C &col, int nk, int source, T ml_depth,
-                                              Parcel<T> &pc)
+                                              Parcel<T> &pc, const Sink &sink = Sink{})
 {
     using F = Fn<T, MATH>;
     using K = Phys<T>;
@@ -211,6 +218,7 @@ XC_HD __forceinline__ bool parcel_source(const C &col, int nk, int source, T ml_
                 T pk = T(100.0) * T(cur.p);
                 if (pk >= T(50000.0)) {
                     Env<T> e = env_from<T, MATH>(cur);
+                    sink(k, e);
                     if (k > 0 && k < kskip) zrun = zrun + layer_dz(e.thv, ep.thv, e.pi, ep.pi);
                     ep = e;
                     T the = thetae<T, MATH>(e.p, e.t, e.td, e.q);
@@ -238,6 +246,7 @@ XC_HD __forceinline__ bool parcel_source(const C &col, int nk, int source, T ml_
     } else if (source == SRC_ML) {
         // z(1), z(2) and z(nk) decide the branch (:287-301)
         Env<T> e0 = env_at<T, MATH>(col, 0);
+        sink(0, e0);
         Env<T> ep = e0;
         T z1 = T(0.0), ztop = T(0.0);
         Raw nx = nk > 1 ? load_raw(col, 1) : Raw{};
@@ -245,6 +254,7 @@ XC_HD __forceinline__ bool parcel_source(const C &col, int nk, int source, T ml_
             const Raw cur = nx;
             if (k + 1 < nk) nx = load_raw(col, k + 1);
             Env<T> e = env_from<T, MATH>(cur);
+            sink(k, e);
             ztop = ztop + layer_dz(e.thv, ep.thv, e.pi, ep.pi);
             if (k == 1) z1 = ztop;
             ep = e;
@@ -451,78 +461,153 @@ XC_HD void cape_column(const C &col, int nk, T pinc, int source, T ml_depth,
 //
 // Loop shape: with one or two iterations per sub-step the wavefront would
 // spend most of its time waiting for the lane with the most iterations if
-// sub-steps were lock-stepped.  Here every trip does exactly one iteration
-// for every running lane, and each lane moves to its next sub-step or level
-// on its own (the set-up blocks execute under the mask of the lanes that need
-// them).  The wave runs until its longest column is done.
-// `pc` comes from parcel_source (the caller may have used it to build the
-// first sub-step's surrogate inputs).  predict = false starts every sub-step
-// cold: the same loop without prediction, which separates the loop's share
-// of the speed-up from the prediction's.
-template <typename T, int MATH, bool ICE, bool PSEUDO, class C>
-XC_HD void cape_column_warm(const C &col, int nk, T pinc, const Parcel<T> &pc,
-                                 T first_guess, T fallback_tol, ColumnOut<T> &out,
-                                 LaneStats &st, bool predict = true)
+// sub-steps were lock-stepped.  Here every trip does up to XC_ITERS_PER_TRIP
+// iterations for every running lane, and each lane moves to its next sub-step
+// or level on its own (the set-up blocks execute under the mask of the lanes
+// that need them).
+//
+// The driver's state is a WarmAscent: init() loads a parcel, each trip()
+// runs one trip and reports when the column is done, and result() gives the
+// outputs.  cape_column_warm runs one column to completion; the grid kernel
+// instead hands a lane its next column when trip() reports done, so that a
+// wavefront's lanes stay busy until the whole grid is drained (a wave that
+// ran 64 fixed columns would idle every lane until its longest column ended).
+//
+// Level environments come from the accessor: env_from() of its raw values,
+// or, for an accessor that declares has_env, its env(k) (levels evaluated
+// by an earlier pass, with the same env_from), loaded one level ahead.
+template <class C, class = void> struct HasEnv {
+    static constexpr bool value = false;
+};
+template <class C> struct HasEnv<C, decltype((void)C::has_env)> {
+    static constexpr bool value = C::has_env;
+};
+
+
+// Shape of d(theta)/d(ln p) along a saturated pseudo-adiabat at (t, p, th),
+// up to a constant factor: th qs (a1 - t) / (t^2 + a2 qs), a1 = L kappa / Rv,
+// a2 = L^2 / (cp Rv), with qs and L of liquid water, or of ice (ICE_PHASE).
+// From dtheta / theta = -L dqs / (cp t) with Clausius-Clapeyron for dqs and
+// t = theta pi.  The predictor uses only ratios of it, to scale the last
+// measured rate from one sub-step to the next; its arithmetic is the
+// predictor's (hardware exp and reciprocal on the device).
+template <typename T, bool ICE_PHASE>
+XC_HD __forceinline__ T adiabat_shape(T t, T p, T th)
 {
-    using F = Fn<T, MATH>;
     using K = Phys<T>;
-    // the predictor's own arithmetic (slopes, rates, extrapolation weights)
-    // only shapes the first iterate, never the fixed point: it divides with
-    // the hardware reciprocal in every math mode
-    const auto pdiv = [](T a, T b) { return pred_div(a, b); };
-    const int mulvl = pc.mulvl;
-    T zout = T(-999999.0);
-    T th2 = pc.th2, pi2 = pc.e.pi, p2 = pc.e.p, t2 = pc.t2, qv2 = pc.qv2, b2 = pc.b2, zk = pc.zk;
-    T ql2 = T(0.0), qi2 = T(0.0), qt = qv2;
-    T cape = T(0.0), cin = T(0.0), narea = T(0.0);
-    // the previous level's environment: p in e.p until the next level
-    // replaces it, pi and thv here (the layer update needs both levels)
-    T pi_prev = pc.e.pi, thv_prev = pc.e.thv;
-    int k = pc.kmax;
-    if (!(k < nk - 1)) {
-        out.cape = cape; out.cin = cin; out.mulvl = mulvl; out.zout = zout;
-        return;
-    }
-    // level state (b2 keeps the previous level's buoyancy until the layer
-    // update, so it is also the reference's b1)
-    Env<T> e = pc.e;
-    T dp = T(0.0);
-    int nloop = 0, n = 0;
+    constexpr T a = ICE_PHASE ? T(21.8745584) : T(17.67);
+    constexpr T b = ICE_PHASE ? T(7.66) : T(29.65);
+    constexpr T lat = ICE_PHASE ? K::xls : K::xlv;
+    constexpr T a1 = lat * K::rddcp * K::cp / K::rv;     // L kappa / Rv
+    constexpr T a2 = lat * lat / (K::cp * K::rv);        // L^2 / (cp Rv)
+    const T es = T(611.2) * pred_exp(pred_div(a * (t - T(273.15)), t - b));
+    const T qs = K::eps * pred_div(es, p - es);
+    return pred_div(th * qs * (a1 - t), t * t + a2 * qs);
+}
+
+template <typename T, int MATH, bool ICE, bool PSEUDO>
+struct WarmAscent {
+    enum { M_LEVEL = 0, M_SUB = 1, M_ITER = 2, M_CONV = 3, M_LAYER = 4 };
+    int mode, k, nk, left, it, nh;   // left: sub-steps of the level not yet begun
+    bool warm, fell_back, first_sub;
+    bool layered;   // a layer is complete: the height output is zk (else -999999)
+    T th2, pi2, p2, t2, qv2, b2, zk, ql2, qi2, qt;
+    T cape, cin, narea;
+    // the previous level's pi and thv (the layer update needs both levels);
+    // p of the current level stays in e.p until the next level replaces it
+    T pi_prev, thv_prev;
+    Env<T> e;
+    // an accessor with evaluated environments: the next level's, loaded one
+    // level ahead so that the load is in flight while this level ascends
+    Env<T> nx;
+    T dp;
     // sub-step state
-    T p1 = T(0.0), t1 = T(0.0), th1 = T(0.0), qv1 = T(0.0), ql1 = T(0.0), qi1 = T(0.0);
-    T logp = T(0.0), thlast = T(0.0);
-    int it = 0;
-    bool warm = false, fell_back = false;
-    bool first_sub = true;
+    T p1, t1, th1, qv1, ql1, qi1, logp, thlast;
     // previous iterate (for the secant slope) and the last slope estimate
-    T x_prev = T(0.0), f_prev = T(0.0), fpr = T(0.0);
-    // extrapolation history: last two rates and their ln-p steps
-    T r1 = T(0.0), r2 = T(0.0), dl1 = T(0.0), dl2 = T(0.0);
-    int nh = 0;
-    // Trip state.  Every trip runs the same straight-line sequence (level
-    // set-up, sub-step set-up, one iteration, level completion), each part
-    // under the mask of the lanes whose mode asks for it; `mode` is made
-    // opaque at the top of the trip so that the compiler cannot thread the
-    // iterate-again path into an inner loop (which would make the lanes that
-    // converged wait for the slowest one again).
-    enum { M_LEVEL = 0, M_SUB = 1, M_ITER = 2, M_LAYER = 3 };
-    int mode = M_LEVEL;
-    for (;;) {
+    // the last relaxed iterate (the one before thlast) and its image, for the
+    // secant slope, and the last estimate of the map's slope f'
+    T x_prev, f_prev, fpr;
+    // extrapolation history: last two rates and their ln-p steps (while
+    // there is no history, r1 holds the caller's first guess, NaN for none)
+    T r1, r2, dl1, dl2;
+    // adiabat_shape at the current sub-step's start (NaN: not a pure liquid
+    // or pure ice regime), and whether the last sub-step condensed
+    T gs;
+    bool sat;
+
+    // Starts the ascent of parcel pc in a column of nk levels.  Returns false
+    // when there is no level above the parcel (result() is then final).
+    XC_HD __forceinline__ bool init(const Parcel<T> &pc, int nk_, T guess)
+    {
+        nk = nk_;
+        layered = false;
+        th2 = pc.th2; pi2 = pc.e.pi; p2 = pc.e.p; t2 = pc.t2; qv2 = pc.qv2; b2 = pc.b2;
+        zk = pc.zk;
+        ql2 = T(0.0); qi2 = T(0.0); qt = qv2;
+        cape = T(0.0); cin = T(0.0); narea = T(0.0);
+        pi_prev = pc.e.pi; thv_prev = pc.e.thv;
+        k = pc.kmax;
+        e = pc.e;
+        dp = T(0.0);
+        left = 0; it = 0; nh = 0;
+        p1 = T(0.0); t1 = T(0.0); th1 = T(0.0); qv1 = T(0.0); ql1 = T(0.0); qi1 = T(0.0);
+        logp = T(0.0); thlast = T(0.0);
+        warm = false; fell_back = false; first_sub = true;
+        x_prev = T(0.0); f_prev = T(0.0); fpr = T(0.0);
+        r1 = guess; r2 = T(0.0); dl1 = T(0.0); dl2 = T(0.0);
+        gs = T(NAN);
+        sat = false;
+        mode = M_LEVEL;
+        return k < nk - 1;
+    }
+
+    // The environment of the first level above the parcel, for accessors
+    // with evaluated environments (call after init()).
+    XC_HD __forceinline__ void set_next(const Env<T> &e1) { nx = e1; }
+
+    XC_HD __forceinline__ void result(ColumnOut<T> &out) const
+    {
+        out.cape = cape; out.cin = cin; out.zout = layered ? zk : T(-999999.0);
+    }
+
+    // One trip.  Returns true when the column is finished (terminated, top
+    // reached, or an iteration limit exceeded: then CAPE = CIN = 0 as in the
+    // reference, :464-474).
+    template <class C>
+    XC_HD __forceinline__ bool trip(const C &col, T pinc, T fallback_tol, LaneStats &st,
+                                    bool predict)
+    {
+        using F = Fn<T, MATH>;
+        using K = Phys<T>;
+        // the predictor's own arithmetic (slopes, rates, extrapolation
+        // weights) only shapes the first iterate, never the fixed point: it
+        // divides with the hardware reciprocal in every math mode
+        const auto pdiv = [](T a, T b) { return pred_div(a, b); };
+        // Every trip runs the same straight-line sequence (level set-up,
+        // sub-step set-up, the iterations, level completion), each part under
+        // the mask of the lanes whose mode asks for it; `mode` is made opaque
+        // at the top of the trip so that the compiler cannot thread the
+        // iterate-again path into an inner loop (which would make the lanes
+        // that converged wait for the slowest one again).
         xc_opaque(mode);
         if (mode == M_LEVEL) {
             k = k + 1;
             const T p_prev = e.p;
-            e = env_from<T, MATH>(col.raw_above(k));
+            if constexpr (HasEnv<C>::value) {
+                e = nx;
+                if (k + 1 < nk) nx = col.env(k + 1);
+            } else {
+                e = env_from<T, MATH>(col.raw_above(k));
+            }
             dp = p_prev - e.p;
-            nloop = substep_count<T, MATH>(dp, pinc);
-            n = 0;
+            left = substep_count<T, MATH>(dp, pinc);
             st.levels++;
             // zero sub-steps (NaN/overflow in the step count): the reference's
             // loop body does not run; straight to the buoyancy update
-            mode = nloop >= 1 ? M_SUB : M_LAYER;
+            mode = left >= 1 ? M_SUB : M_LAYER;
         }
         if (mode == M_SUB) {
-            n = n + 1;
+            left = left - 1;
             p1 = p2; t1 = t2; th1 = th2; qv1 = qv2;
             ql1 = PSEUDO ? T(0.0) : ql2;
             qi1 = PSEUDO ? T(0.0) : qi2;
@@ -537,9 +622,10 @@ XC_HD void cape_column_warm(const C &col, int nk, T pinc, const Parcel<T> &pc,
                 g = th1 + r * logp;
             } else if (nh == 1) {
                 g = th1 + r1 * logp;
-            } else if (first_guess == first_guess) {
-                g = first_guess;
-                first_guess = T(NAN);
+            } else if (r1 == r1) {
+                // the caller's guess, used once
+                g = r1;
+                r1 = T(NAN);
             }
             // a guess that is not finite or absurdly far is no guess
             const T dg = g - th1;
@@ -567,8 +653,9 @@ XC_HD void cape_column_warm(const C &col, int nk, T pinc, const Parcel<T> &pc,
             const T ad = d < T(0.0) ? -d : d;
             if (it > 100) {
                 st.nonconverged++;
-                out.cape = T(0.0); out.cin = T(0.0); out.mulvl = mulvl; out.zout = zout;
-                return;
+                cape = T(0.0);
+                cin = T(0.0);
+                return true;
             }
             if (warm && it == 1 && ad > fallback_tol) {
                 // fallback: the reference's cold start, full iteration budget
@@ -582,37 +669,73 @@ XC_HD void cape_column_warm(const C &col, int nk, T pinc, const Parcel<T> &pc,
                 x_prev = x;
                 f_prev = th2;
             } else {
-                // converged: record the rate, finish the sub-step (:485-497).
-                // The slope f' of the map from this sub-step's last two
-                // iterates (kept from an earlier sub-step when it converged
-                // at its first)
-                first_sub = false;
-                if (it >= 2) {
-                    const T dx = x - x_prev;
-                    if (dx != T(0.0)) {
-                        const T sl = pdiv(th2 - f_prev, dx);
-                        fpr = sl < T(-10.0) ? T(-10.0) : (sl > T(0.9) ? T(0.9) : sl);
-                    }
-                }
-                if (logp != T(0.0)) {
-                    const T xs = x + pdiv(th2 - x, T(1.0) - fpr);
-                    r2 = r1; dl2 = dl1;
-                    r1 = pdiv(xs - th1, logp);
-                    dl1 = logp;
-                    // a second-order guess that failed marks a change of
-                    // regime (saturation, ice onset): the older rate belongs
-                    // to the other side, so restart at first order
-                    nh = fell_back && nh >= 2 ? 1 : (nh < 2 ? nh + 1 : 2);
-                } else {
-                    nh = 0;
-                }
-                if (PSEUDO) {
-                    qt = qv2;
-                    ql2 = T(0.0);
-                    qi2 = T(0.0);
-                }
-                mode = n < nloop ? M_SUB : M_LAYER;
+                // converged (:475-479): finished below, once per trip
+                mode = M_CONV;
             }
+        }
+        if (mode == M_CONV) {
+            // record the rate, finish the sub-step (:485-497).  x = thlast is
+            // the iterate that converged.  The slope f' of the map from this
+            // sub-step's last two iterates (kept from an earlier sub-step
+            // when it converged at its first)
+            const T x = thlast;
+            first_sub = false;
+            if (it >= 2) {
+                const T dx = x - x_prev;
+                if (dx != T(0.0)) {
+                    const T sl = pdiv(th2 - f_prev, dx);
+                    fpr = sl < T(-10.0) ? T(-10.0) : (sl > T(0.9) ? T(0.9) : sl);
+                }
+            }
+            // the rate's shape at the next sub-step's start, for the
+            // pseudo-adiabats (the shape assumes the condensate falls out),
+            // where the condensate is all liquid (above 0 degC; everywhere
+            // for the liquid adiabat) and, in the fast math mode, where it is
+            // all ice (below -40 degC; in the accurate mode the second set of
+            // constants would cost the ascent kernel its eighth wave per
+            // SIMD); none in the ice adiabat's mixed range
+            const bool sat_now = qv2 < qt;
+            T ge = T(NAN);
+            if (PSEUDO && predict) {
+                constexpr bool ICE_SHAPE = ICE && MATH == MATH_FAST;
+                const bool ice = ICE_SHAPE && t2 < T(233.15);
+                if (ice)
+                    ge = adiabat_shape<T, true>(t2, p2, th2);
+                else if (!ICE || t2 > T(273.15))
+                    ge = adiabat_shape<T, false>(t2, p2, th2);
+            }
+            if (logp != T(0.0)) {
+                const T xs = x + pdiv(th2 - x, T(1.0) - fpr);
+                const T rn = pdiv(xs - th1, logp);
+                // saturated ascent within one phase: a history whose
+                // second-order extrapolation is this rate scaled by the
+                // shape's change between the sub-steps' midpoints (log-linear
+                // in ln p, linearised):
+                // r = rn (1 + (ge / gs - 1) (logp' + dl1) / (2 dl1)).
+                // Otherwise the measured history; a second-order guess that
+                // failed marks a change of regime (saturation, ice onset):
+                // the older rate belongs to the other side, so restart at
+                // first order
+                const bool shaped = sat && sat_now && gs == gs && ge == ge;
+                const T r2n = shaped ? rn * (T(2.0) - pdiv(ge, gs)) : r1;
+                const T dl2n = shaped ? logp : dl1;
+                nh = shaped ? 2 : (fell_back && nh >= 2 ? 1 : (nh < 2 ? nh + 1 : 2));
+                r2 = r2n;
+                dl2 = dl2n;
+                r1 = rn;
+                dl1 = logp;
+            } else {
+                nh = 0;
+                r1 = T(NAN);
+            }
+            gs = ge;
+            sat = sat_now;
+            if (PSEUDO) {
+                qt = qv2;
+                ql2 = T(0.0);
+                qi2 = T(0.0);
+            }
+            mode = left > 0 ? M_SUB : M_LAYER;
         }
         if (mode == M_LAYER) {
             // level complete (:501-558)
@@ -621,13 +744,31 @@ XC_HD void cape_column_warm(const C &col, int nk, T pinc, const Parcel<T> &pc,
             layer_update<T, MATH>(th2, qv2, ql2, qi2, e, thv_prev, pi_prev, b1, b2, dz, cape,
                                   cin, narea);
             zk = zk + dz;
-            zout = zk;
+            layered = true;
             pi_prev = e.pi; thv_prev = e.thv;
-            if ((e.p <= T(10000.0) && b2 < T(0.0)) || !(k < nk - 1)) break;
+            if ((e.p <= T(10000.0) && b2 < T(0.0)) || !(k < nk - 1)) return true;
             mode = M_LEVEL;
         }
+        return false;
     }
-    out.cape = cape; out.cin = cin; out.mulvl = mulvl; out.zout = zout;
+};
+
+// One column to completion.  `pc` comes from parcel_source (the caller may
+// have used it to build the first sub-step's surrogate inputs).  predict =
+// false starts every sub-step cold: the same loop without prediction, which
+// separates the loop's share of the speed-up from the prediction's.
+template <typename T, int MATH, bool ICE, bool PSEUDO, class C>
+XC_HD void cape_column_warm(const C &col, int nk, T pinc, const Parcel<T> &pc,
+                                 T first_guess, T fallback_tol, ColumnOut<T> &out,
+                                 LaneStats &st, bool predict = true)
+{
+    WarmAscent<T, MATH, ICE, PSEUDO> w;
+    out.mulvl = pc.mulvl;
+    if (w.init(pc, nk, first_guess)) {
+        for (;;)
+            if (w.trip(col, pinc, fallback_tol, st, predict)) break;
+    }
+    w.result(out);
 }
 
 }  // namespace xcape

@@ -147,78 +147,297 @@ __device__ __forceinline__ void surrogate_inputs(const C &c, int nk, const Parce
     }
 }
 
-// CAPE/CIN, one column per lane.  MODE: 0 the reference's loops with the
-// cold start (cape_column); 1 the warm-started ascent (cape_column_warm);
-// 2 the same loop with every sub-step cold (to separate the loop's share of
-// the speed-up from the prediction's).  SUR (with MODE 1): a surrogate is
-// loaded; every wave first evaluates it for its 64 columns' first sub-steps
-// (one wave-wide BF16 MFMA evaluation, a point every lane of the block
-// reaches), and each lane starts its ascent from that iterate.  STATS:
-// accumulate the work counters into a.stats (a separate instantiation, so
-// that the counters do not occupy registers in the timed kernels).
-template <typename T, typename IDX, bool ICE, bool PSEUDO, int MODE, bool SUR, bool STATS>
+// CAPE/CIN with the reference's loops and cold start (cape_column), one
+// column per lane.  STATS: accumulate the work counters into a.stats (a
+// separate instantiation, so that the counters do not occupy registers in
+// the timed kernels).
+template <typename T, typename IDX, bool ICE, bool PSEUDO, bool STATS>
 __global__ void __launch_bounds__(256) cape_grid_kernel(CapeGridArgs a)
 {
-    extern __shared__ uint32_t dyn_lds[];
     const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     SigmaCol<IDX> c{};
     int nk = 0;
     const bool in_grid = column_at(a, col, c, nk);
     T *cape = static_cast<T *>(a.cape), *cin = static_cast<T *>(a.cin),
       *zout = static_cast<T *>(a.zout);
-    if constexpr (MODE == 0) {
-        LaneStats st;
-        bool ran = false;
-        if (in_grid) {
-            ColumnOut<T> o;
-            if (c.ts > 0.0f) {
-                ran = true;
-                cape_column<T, XC_MATH, ICE, PSEUDO>(c, nk, T(a.pinc), a.source, T(a.ml_depth),
-                                                     o, st);
-            } else {
-                o.cape = T(0); o.cin = T(0); o.zout = T(0); o.mulvl = 0;
-            }
-            cape[col] = o.cape;
-            cin[col] = o.cin;
-            a.mulvl[col] = o.mulvl;
-            zout[col] = o.zout;
+    LaneStats st;
+    bool ran = false;
+    if (in_grid) {
+        ColumnOut<T> o;
+        if (c.ts > 0.0f) {
+            ran = true;
+            cape_column<T, XC_MATH, ICE, PSEUDO>(c, nk, T(a.pinc), a.source, T(a.ml_depth), o,
+                                                 st);
+        } else {
+            o.cape = T(0); o.cin = T(0); o.zout = T(0); o.mulvl = 0;
         }
-        if constexpr (STATS) block_stats_add(st, ran, a.stats);
-    } else {
-        const bool ran = in_grid && c.ts > 0.0f;
-        Parcel<T> pc;
-        bool ok = false;
-        T lp1 = T(0.0);
-        float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
-        if (ran) {
-            ok = parcel_source<T, XC_MATH>(c, nk, a.source, T(a.ml_depth), pc);
-            // stored before the ascent: one register less through it
-            a.mulvl[col] = pc.mulvl;
-            if (SUR && ok && pc.kmax < nk - 1) surrogate_inputs(c, nk, pc, a.pinc, f0, f1, f2, lp1);
-        }
-        float r = 0.0f;
-        if constexpr (SUR)
-            r = surrogate_eval_wave(static_cast<const SurrogateDev *>(a.surrogate), f0, f1, f2,
-                                    dyn_lds + (threadIdx.x >> 6) * MLP_LDS_WORDS_PER_WAVE);
-        LaneStats st;
-        if (in_grid) {
-            ColumnOut<T> o{};
-            if (ok) {
-                const T g = lp1 != T(0.0) ? pc.th2 + T(r) * lp1 : T(NAN);
-                cape_column_warm<T, XC_MATH, ICE, PSEUDO>(c, nk, T(a.pinc), pc, g,
-                                                          T(a.fallback_tol), o, st, MODE == 1);
-            } else if (ran) {
-                // unknown source (unreachable through the C-ABI's checks)
-                o.zout = T(-999999.0);
-            } else {
-                a.mulvl[col] = 0;
-            }
-            cape[col] = o.cape;
-            cin[col] = o.cin;
-            zout[col] = o.zout;
-        }
-        if constexpr (STATS) block_stats_add(st, ran, a.stats);
+        cape[col] = o.cape;
+        cin[col] = o.cin;
+        a.mulvl[col] = o.mulvl;
+        zout[col] = o.zout;
     }
+    if constexpr (STATS) block_stats_add(st, ran, a.stats);
+}
+
+// ---------------------------------------------------------------------------
+// Warm-started CAPE/CIN: two launches.
+//
+// warm_setup_kernel, one column per lane, every lane of a wave on the same
+// level: the parcel source (parcel_source), the surrogate's inputs for the
+// first sub-step (SUR), and the environment (p, pi, thv) of every level from
+// the parcel's up, written
+// level-major to a.env as 16-byte records (level k of column j at record
+// k * ncol + j, k = 0 the surface), so that a lane's level is one load of
+// one cache line.  Columns with nothing to ascend get their final outputs
+// here.
+//
+// surrogate_guess_kernel (SUR): the first sub-step's guesses, one wave-wide
+// BF16 MFMA evaluation of the surrogate per 64 columns.  A launch of its own,
+// so that the evaluation's registers do not lower the setup pass's
+// occupancy.
+//
+// warm_ascent_kernel, persistent: each lane takes a column from the a.work
+// counters, runs the WarmAscent state machine on it trip by trip, writes its
+// outputs when it ends and takes the next column, until the counter passes
+// ncol.  A level's set-up is then one load, issued a level ahead: the
+// environment's pow, exp and divisions ran in the setup pass with every lane
+// on the same level, instead of under the mask of the few lanes that reach a
+// new level in a given trip.
+// ---------------------------------------------------------------------------
+
+// Level environments of one column evaluated by warm_setup_kernel.
+struct EnvRec {
+    float p, pi, thv;   // of a 16-byte record (the fourth word unused)
+};
+
+// A column's parcel as the setup pass leaves it for the ascent: 64 bytes, four
+// 16-byte loads with no dependence between them.  kmax = -1: nothing to
+// ascend.  guess: the surrogate's first iterate (NaN: none).
+struct ParcelRec {
+    int32_t kmax, nk;
+    float th2, t2;
+    float qv2, b2, zk, guess;
+    float e_p, e_pi, e_thv, pad0;     // the environment at kmax
+    float n_p, n_pi, n_thv, pad1;     // and at kmax + 1
+};
+static_assert(sizeof(ParcelRec) == 64, "parcel record layout");
+
+struct EnvCol {
+    static constexpr bool has_env = true;
+    const float *rec;
+    uint32_t col, ls;
+    __device__ __forceinline__ Env<float> env(int k) const
+    {
+        const EnvRec r =
+            *reinterpret_cast<const EnvRec *>(rec + 4 * (size_t)((uint32_t)k * ls + col));
+        Env<float> e{};
+        e.p = r.p;
+        e.pi = r.pi;
+        e.thv = r.thv;
+        return e;
+    }
+};
+
+template <bool ICE, bool PSEUDO, bool SUR, bool STATS>
+__global__ void __launch_bounds__(256) warm_setup_kernel(CapeGridArgs a)
+{
+    using T = float;
+    const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    SigmaCol<uint64_t> c{};
+    int nk = 0;
+    const bool in_grid = column_at(a, col, c, nk);
+    const bool ran = in_grid && c.ts > 0.0f;
+    Parcel<T> pc;
+    bool ok = false;
+    T lp1 = T(0.0);
+    float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
+    const uint32_t ls = (uint32_t)a.ncol, j = (uint32_t)col;
+    float4 *env = reinterpret_cast<float4 *>(a.env);
+    // the source scan's environments are stored as it computes them; `cover`:
+    // the levels 0 .. cover-1 it has stored
+    int cover = 0;
+    if (ran) {
+        const auto sink = [&](int k, const Env<T> &e) {
+            env[(uint32_t)k * ls + j] = make_float4(e.p, e.pi, e.thv, 0.0f);
+            if (k == cover) cover = k + 1;
+        };
+        ok = parcel_source<T, XC_MATH>(c, nk, a.source, T(a.ml_depth), pc, sink);
+        a.mulvl[col] = pc.mulvl;
+        if (SUR && ok && pc.kmax < nk - 1) surrogate_inputs(c, nk, pc, a.pinc, f0, f1, f2, lp1);
+    }
+    const bool go = ok && pc.kmax < nk - 1;
+    if (go) {
+        // the levels from the parcel's up that the scan did not store, by
+        // absolute level so that the wave's lanes write the same row
+        const int k0 = pc.kmax > cover ? pc.kmax : cover;
+        Env<T> e1{};
+        for (int k = 0; k < nk; ++k) {
+            if (k < k0) continue;
+            const Env<T> e = env_from<T, XC_MATH>(load_raw(c, k));
+            env[(uint32_t)k * ls + j] = make_float4(e.p, e.pi, e.thv, 0.0f);
+            if (k == pc.kmax + 1) e1 = e;
+        }
+        if (cover > pc.kmax + 1) {
+            // the scan stored level kmax + 1 already
+            const float4 r = env[(uint32_t)(pc.kmax + 1) * ls + j];
+            e1.p = r.x; e1.pi = r.y; e1.thv = r.z;
+        }
+        float4 *rec = reinterpret_cast<float4 *>(a.par) + 4 * col;
+        rec[0] = make_float4(__int_as_float(pc.kmax), __int_as_float(nk), pc.th2, pc.t2);
+        rec[1] = make_float4(pc.qv2, pc.b2, pc.zk, T(NAN));
+        rec[2] = make_float4(pc.e.p, pc.e.pi, pc.e.thv, 0.0f);
+        rec[3] = make_float4(e1.p, e1.pi, e1.thv, 0.0f);
+        if constexpr (SUR) {
+            const int64_t n = a.ncol;
+            a.sur_in[col] = f0;
+            a.sur_in[n + col] = f1;
+            a.sur_in[2 * n + col] = f2;
+            a.sur_in[3 * n + col] = lp1;
+        }
+    } else if (in_grid) {
+        // ts <= 0 degC: zeros; no level above the parcel (or an unknown
+        // source): CAPE = CIN = 0 and the height's initial -999999
+        reinterpret_cast<ParcelRec *>(a.par)[col].kmax = -1;
+        if constexpr (SUR) a.sur_in[3 * a.ncol + col] = T(0.0);
+        T *cape = static_cast<T *>(a.cape), *cin = static_cast<T *>(a.cin),
+          *zout = static_cast<T *>(a.zout);
+        cape[col] = T(0.0);
+        cin[col] = T(0.0);
+        zout[col] = ran ? T(-999999.0) : T(0.0);
+        if (!ran) a.mulvl[col] = 0;
+    }
+    if constexpr (STATS) block_stats_add(LaneStats{}, ran, a.stats);
+}
+
+__global__ void __launch_bounds__(256) surrogate_guess_kernel(CapeGridArgs a)
+{
+    extern __shared__ uint32_t dyn_lds[];
+    const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = a.ncol;
+    const bool in_grid = col < n;
+    // every lane takes part in its wave's evaluation; lanes without a first
+    // sub-step have lp1 = 0 and zero inputs
+    ParcelRec *rec = reinterpret_cast<ParcelRec *>(a.par);
+    const bool go = in_grid && rec[col].kmax >= 0;
+    float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f, lp1 = 0.0f;
+    if (go) {
+        lp1 = a.sur_in[3 * n + col];
+        if (lp1 != 0.0f) {
+            f0 = a.sur_in[col];
+            f1 = a.sur_in[n + col];
+            f2 = a.sur_in[2 * n + col];
+        }
+    }
+    const float r = surrogate_eval_wave(static_cast<const SurrogateDev *>(a.surrogate), f0, f1,
+                                        f2, dyn_lds + (threadIdx.x >> 6) * MLP_LDS_WORDS_PER_WAVE);
+    if (go) rec[col].guess = lp1 != 0.0f ? rec[col].th2 + r * lp1 : NAN;
+}
+
+// The ascent's columns are split into WORK_PARTS contiguous parts with a
+// counter each.  Workgroups are dispatched to the eight XCDs in turn, so the
+// waves of part blockIdx % 8 all run on one XCD.  (Every part holds an eighth
+// of the columns and every XCD as many waves, so the parts drain together.)
+// A wave takes its part's columns 64 at a time (one atomic per 64 columns);
+// its lanes draw from that block as they finish their columns.
+constexpr uint32_t WORK_PARTS = 8;
+constexpr uint32_t WORK_BLOCK = 64;
+
+__device__ __forceinline__ uint32_t work_part_begin(uint32_t ncol, uint32_t part)
+{
+    return (uint32_t)(((uint64_t)ncol * part) / WORK_PARTS);
+}
+
+// PREDICT: the warm start (false: the same loop with every sub-step cold)
+template <bool ICE, bool PSEUDO, bool PREDICT, bool STATS>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
+warm_ascent_kernel(CapeGridArgs a)
+{
+    using T = float;
+    const uint32_t ncol = (uint32_t)a.ncol;
+    T *cape = static_cast<T *>(a.cape), *cin = static_cast<T *>(a.cin),
+      *zout = static_cast<T *>(a.zout);
+    const float4 *recs = reinterpret_cast<const float4 *>(a.par);
+    const int lane = threadIdx.x & 63;
+    const uint64_t below = (1ull << lane) - 1ull;
+    WarmAscent<T, XC_MATH, ICE, PSEUDO> w;
+    LaneStats st;
+    uint32_t col = 0;
+    bool have = false, need = true;
+    // wave-uniform: the wave's part (its XCD's), its current block of columns
+    // [blk, blk + 64) and the block's slots handed out
+    const uint32_t part = blockIdx.x % WORK_PARTS;
+    const uint32_t part_end = work_part_begin(ncol, part + 1);
+    uint32_t blk = 0, used = WORK_BLOCK;
+    bool drained = false;
+    for (;;) {
+        const uint64_t want = __ballot(need);
+        if (want != 0 && !drained) {
+            if (used >= WORK_BLOCK) {
+                uint32_t b = 0;
+                if (lane == 0) b = atomicAdd(a.work + part, WORK_BLOCK);
+                blk = __builtin_amdgcn_readfirstlane(b);
+                used = 0;
+                drained = blk >= part_end;
+            }
+            if (!drained) {
+                uint32_t avail = WORK_BLOCK - used;
+                if (blk + used + avail > part_end) avail = part_end - blk - used;
+                const uint32_t rank = (uint32_t)__popcll(want & below);
+                if (need && rank < avail) {
+                    col = blk + used + rank;
+                    need = false;
+                    const float4 r0 = recs[4 * col], r1 = recs[4 * col + 1],
+                                 r2 = recs[4 * col + 2], r3 = recs[4 * col + 3];
+                    const int kmax = __float_as_int(r0.x);
+                    if (kmax >= 0) {
+                        Parcel<T> pc;
+                        pc.kmax = kmax;
+                        pc.mulvl = 0;
+                        pc.th2 = r0.z;
+                        pc.t2 = r0.w;
+                        pc.qv2 = r1.x;
+                        pc.b2 = r1.y;
+                        pc.zk = r1.z;
+                        pc.e.p = r2.x;
+                        pc.e.pi = r2.y;
+                        pc.e.thv = r2.z;
+                        // the setup pass queued only columns with a level
+                        // above the parcel: init() succeeds
+                        have = w.init(pc, __float_as_int(r0.y), r1.w);
+                        Env<T> e1{};
+                        e1.p = r3.x;
+                        e1.pi = r3.y;
+                        e1.thv = r3.z;
+                        w.set_next(e1);
+                    }
+                    need = !have;
+                }
+                const uint32_t got = (uint32_t)__popcll(want);
+                used = avail == 0 ? WORK_BLOCK : used + (got < avail ? got : avail);
+            }
+        }
+        // the wave ends when every lane has run out of columns
+        if (drained) need = false;
+        if (__ballot(have || need) == 0) break;
+        if (have) {
+            const EnvCol ec{a.env, col, ncol};
+            if (w.trip(ec, T(a.pinc), T(a.fallback_tol), st, PREDICT)) {
+                ColumnOut<T> o;
+                w.result(o);
+                cape[col] = o.cape;
+                cin[col] = o.cin;
+                zout[col] = o.zout;
+                have = false;
+                need = true;
+            }
+        }
+    }
+    if constexpr (STATS) block_stats_add(st, false, a.stats);
+}
+
+__global__ void warm_reset_kernel(unsigned int *work, uint32_t ncol)
+{
+    if (threadIdx.x < WORK_PARTS) work[threadIdx.x] = work_part_begin(ncol, threadIdx.x);
 }
 
 // Storm-relative helicity chain, one column per lane.  TB: the storm-motion
@@ -248,16 +467,15 @@ __global__ void __launch_bounds__(256) srh_grid_kernel(CapeGridArgs a)
     }
 }
 
-template <typename T, typename IDX, int MODE, bool SUR, bool STATS>
+template <typename T, typename IDX, bool STATS>
 hipError_t launch_cape_s(const CapeGridArgs &a, int block, hipStream_t s)
 {
     dim3 grid((unsigned)((a.ncol + block - 1) / block));
-    const size_t lds = SUR ? sizeof(uint32_t) * MLP_LDS_WORDS_PER_WAVE * (size_t)(block / 64) : 0;
     switch (a.adiabat) {
-    case 1: cape_grid_kernel<T, IDX, false, true, MODE, SUR, STATS><<<grid, block, lds, s>>>(a); break;
-    case 2: cape_grid_kernel<T, IDX, false, false, MODE, SUR, STATS><<<grid, block, lds, s>>>(a); break;
-    case 3: cape_grid_kernel<T, IDX, true, true, MODE, SUR, STATS><<<grid, block, lds, s>>>(a); break;
-    case 4: cape_grid_kernel<T, IDX, true, false, MODE, SUR, STATS><<<grid, block, lds, s>>>(a); break;
+    case 1: cape_grid_kernel<T, IDX, false, true, STATS><<<grid, block, 0, s>>>(a); break;
+    case 2: cape_grid_kernel<T, IDX, false, false, STATS><<<grid, block, 0, s>>>(a); break;
+    case 3: cape_grid_kernel<T, IDX, true, true, STATS><<<grid, block, 0, s>>>(a); break;
+    case 4: cape_grid_kernel<T, IDX, true, false, STATS><<<grid, block, 0, s>>>(a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -265,37 +483,106 @@ hipError_t launch_cape_s(const CapeGridArgs &a, int block, hipStream_t s)
 
 // 32-bit element offsets when every index of the profile arrays fits (the
 // stats launches, which only count work, always use 64-bit ones)
-template <typename T, int MODE, bool SUR>
+template <typename T>
 hipError_t launch_cape(const CapeGridArgs &a, int block, hipStream_t s)
 {
-    if (a.stats) return launch_cape_s<T, uint64_t, MODE, SUR, true>(a, block, s);
+    if (a.stats) return launch_cape_s<T, uint64_t, true>(a, block, s);
     const bool fits = (uint64_t)a.ncol * (uint64_t)a.nk_in < ((uint64_t)1 << 32);
-    return fits ? launch_cape_s<T, uint32_t, MODE, SUR, false>(a, block, s)
-                : launch_cape_s<T, uint64_t, MODE, SUR, false>(a, block, s);
+    return fits ? launch_cape_s<T, uint32_t, false>(a, block, s)
+                : launch_cape_s<T, uint64_t, false>(a, block, s);
+}
+
+// Blocks of the persistent ascent: as many as the device holds at once, never
+// more than the columns need (occupancy queried once per instantiation and
+// block size)
+template <bool ICE, bool PSEUDO, bool PREDICT, bool STATS>
+int ascent_grid(int block, int64_t ncol)
+{
+    static int per_cu[5] = {-1, -1, -1, -1, -1};   // by block / 64
+    static int ncu = 0;
+    const int slot = block / 64;
+    if (slot < 1 || slot > 4) return 1;
+    if (per_cu[slot] < 0) {
+        int dev = 0, pc = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &pc, reinterpret_cast<const void *>(
+                         warm_ascent_kernel<ICE, PSEUDO, PREDICT, STATS>),
+                block, 0) != hipSuccess ||
+            pc < 1)
+            pc = 1;
+        per_cu[slot] = pc;
+    }
+    const int64_t need = (ncol + block - 1) / block;
+    const int64_t full = (int64_t)(ncu > 0 ? ncu : 1) * per_cu[slot];
+    return (int)(need < full ? need : full);
+}
+
+template <bool ICE, bool PSEUDO, bool SUR, bool PREDICT, bool STATS>
+hipError_t launch_warm_s(const CapeGridArgs &a, int block, hipStream_t s)
+{
+    const dim3 sgrid((unsigned)((a.ncol + 255) / 256));
+    warm_setup_kernel<ICE, PSEUDO, SUR, STATS><<<sgrid, 256, 0, s>>>(a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if constexpr (SUR) {
+        const size_t lds = sizeof(uint32_t) * MLP_LDS_WORDS_PER_WAVE * 4;
+        surrogate_guess_kernel<<<sgrid, 256, lds, s>>>(a);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    warm_reset_kernel<<<1, 64, 0, s>>>(a.work, (uint32_t)a.ncol);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    warm_ascent_kernel<ICE, PSEUDO, PREDICT, STATS>
+        <<<ascent_grid<ICE, PSEUDO, PREDICT, STATS>(block, a.ncol), block, 0, s>>>(a);
+    return hipGetLastError();
+}
+
+template <bool ICE, bool PSEUDO>
+hipError_t launch_warm_a(const CapeGridArgs &a, int block, hipStream_t s)
+{
+    const bool sur = a.warm == 1 && a.surrogate;
+    const bool predict = a.warm == 1;
+    if (a.stats) {
+        if (sur) return launch_warm_s<ICE, PSEUDO, true, true, true>(a, block, s);
+        return predict ? launch_warm_s<ICE, PSEUDO, false, true, true>(a, block, s)
+                       : launch_warm_s<ICE, PSEUDO, false, false, true>(a, block, s);
+    }
+    if (sur) return launch_warm_s<ICE, PSEUDO, true, true, false>(a, block, s);
+    return predict ? launch_warm_s<ICE, PSEUDO, false, true, false>(a, block, s)
+                   : launch_warm_s<ICE, PSEUDO, false, false, false>(a, block, s);
+}
+
+inline hipError_t launch_warm(const CapeGridArgs &a, int block, hipStream_t s)
+{
+    if (!a.work || !a.env || !a.par) return hipErrorInvalidValue;
+    switch (a.adiabat) {
+    case 1: return launch_warm_a<false, true>(a, block, s);
+    case 2: return launch_warm_a<false, false>(a, block, s);
+    case 3: return launch_warm_a<true, true>(a, block, s);
+    case 4: return launch_warm_a<true, false>(a, block, s);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 template <typename T>
 hipError_t launch_t(const CapeGridArgs &a, int kind, int block, hipStream_t s)
 {
     if (a.ncol <= 0) return hipSuccess;
+    dim3 grid((unsigned)((a.ncol + block - 1) / block));
     if constexpr (sizeof(T) == 8) {
         // the FP64 yardstick: CAPE with the reference's cold start, and the
         // helicity chain with FP64 storm motion
         if (kind == GRID_SRH) {
-            dim3 grid((unsigned)((a.ncol + block - 1) / block));
             srh_grid_kernel<double><<<grid, block, 0, s>>>(a);
             return hipGetLastError();
         }
         if (kind != GRID_CAPE || a.warm) return hipErrorInvalidValue;
-        return launch_cape<T, 0, false>(a, block, s);
+        return launch_cape<T>(a, block, s);
     } else {
-        dim3 grid((unsigned)((a.ncol + block - 1) / block));
         switch (kind) {
         case GRID_CAPE:
-            if (a.warm == 0) return launch_cape<T, 0, false>(a, block, s);
-            if (a.warm == 2) return launch_cape<T, 2, false>(a, block, s);
-            return a.surrogate ? launch_cape<T, 1, true>(a, block, s)
-                               : launch_cape<T, 1, false>(a, block, s);
+            return a.warm == 0 ? launch_cape<T>(a, block, s) : launch_warm(a, block, s);
         case GRID_SRH:
             srh_grid_kernel<float><<<grid, block, 0, s>>>(a);
             return hipGetLastError();

@@ -35,7 +35,19 @@ struct CapeGridArgs {
                                  // loop; 2: the same loop, every sub-step cold
     float fallback_tol;          // first-iteration residual that rejects a guess [K]
     const void *surrogate;       // device MLP weights (or nullptr)
+    // scratch of the warm-started ascent (warm != 0): the environment of
+    // every level from each parcel's up as (p, pi, thv, unused) records,
+    // (nk_in + 1, ncol) level-major (level 0 the surface; (nk_in + 1) * ncol
+    // < 2^32); one 64-byte parcel record per column (ParcelRec); the
+    // surrogate's inputs, (4, ncol); and the ascent's column counters
+    float *env;                  // 4 floats per record
+    float *par;                  // 16 floats per column
+    float *sur_in;               // f0, f1, f2 and ln(p2/p1) of the first sub-step
+    unsigned int *work;          // 8 counters
 };
+
+// floats of warm-start scratch per column (env, par and sur_in)
+inline int64_t warm_scratch_floats(int64_t nk_in) { return 4 * (nk_in + 1) + 20; }
 
 // kind of work a grid launch does
 enum { GRID_CAPE = 1, GRID_SRH = 2, GRID_CAPE_SRH = 3 };

@@ -139,10 +139,11 @@ int cape_block(bool warm)
 // for the ~32 levels each line serves (measured on 2048x1024x137: 142 GB
 // fetched per ascent launch for 3.4 GB of profiles).  Level-major profiles go
 // straight to the kernels; column-contiguous ones are staged level-major
-// first through the LDS-tiled transpose, in chunks of columns whose copies
-// fit in about 8 GiB (each chunk: one transpose launch for every profile
-// array the call needs, shared by the CAPE and helicity launches, then those
-// launches on the copies).
+// first through the LDS-tiled transpose.  run_grid works in chunks of
+// columns whose scratch fits in about 16 GiB: per chunk, one transpose launch
+// for every profile array the call needs (shared by the CAPE and helicity
+// launches; none for level-major input), the warm-started ascent's scratch
+// (level environments and parcels), then the launches.
 hipError_t launch_grid(const CapeGridArgs &a, bool cape, bool srh, int math, hipStream_t s)
 {
     hipError_t e = hipSuccess;
@@ -189,60 +190,88 @@ hipError_t launch_grid(const CapeGridArgs &a, bool cape, bool srh, int math, hip
 
 hipError_t run_grid(const CapeGridArgs &a0, bool cape, bool srh, int math, hipStream_t s)
 {
-    if (a0.ls != 1 || a0.nk_in == 1) return launch_grid(a0, cape, srh, math, s);
+    const bool stage = a0.ls == 1 && a0.nk_in > 1;
+    const bool warm = cape && a0.warm != 0 && math != 2;
+    if (!stage && !warm) return launch_grid(a0, cape, srh, math, s);
     const bool sigma = a0.p != nullptr;
     const float *src0[5];
     int narr = 0;
     int it = -1, itd = -1, iu = -1, iv = -1, ip = -1;
-    it = narr; src0[narr++] = a0.t;
-    itd = narr; src0[narr++] = a0.td;
-    if (sigma) { ip = narr; src0[narr++] = a0.p; }
-    if (srh) { iu = narr; src0[narr++] = a0.u; iv = narr; src0[narr++] = a0.v; }
+    if (stage) {
+        it = narr; src0[narr++] = a0.t;
+        itd = narr; src0[narr++] = a0.td;
+        if (sigma) { ip = narr; src0[narr++] = a0.p; }
+        if (srh) { iu = narr; src0[narr++] = a0.u; iv = narr; src0[narr++] = a0.v; }
+    }
     const int64_t nk = a0.nk_in;
-    const int64_t per_col = (int64_t)narr * nk * (int64_t)sizeof(float);
-    int64_t chunk = ((int64_t)8 << 30) / per_col;
+    const int64_t wfl = warm ? warm_scratch_floats(nk) : 0;
+    const int64_t per_col = ((int64_t)narr * nk + wfl) * (int64_t)sizeof(float);
+    int64_t chunk = ((int64_t)16 << 30) / per_col;
     if (chunk < ((int64_t)1 << 20)) chunk = (int64_t)1 << 20;
-    // XCAPE_STAGE_COLUMNS: columns per staged chunk, overriding the 8 GiB
-    // sizing (read on every call; for tests of the chunked path)
+    // XCAPE_STAGE_COLUMNS: columns per chunk, overriding the 16 GiB sizing
+    // (read on every call; for tests of the chunked path)
     if (const char *env = getenv("XCAPE_STAGE_COLUMNS")) {
         const long long v = atoll(env);
         if (v > 0) chunk = v;
     }
+    // the warm ascent indexes its level environments with 32-bit offsets
+    if (warm) {
+        const int64_t cap = (((int64_t)1 << 32) - 1) / (nk + 1);
+        if (chunk > cap) chunk = cap;
+    }
     if (chunk > a0.ncol) chunk = a0.ncol;
     DevBuf<float> scratch;
-    hipError_t e = scratch.alloc((size_t)narr * (size_t)nk * (size_t)chunk, s);
+    DevBuf<unsigned int> work;
+    hipError_t e = scratch.alloc((size_t)(narr * nk + wfl) * (size_t)chunk, s);
     if (e != hipSuccess) return e;
+    if (warm && (e = work.alloc(8, s)) != hipSuccess) return e;   // WORK_PARTS counters
+    const int64_t esz = math == 2 ? 8 : 4;
     for (int64_t c0 = 0; c0 < a0.ncol; c0 += chunk) {
         const int64_t n = chunk < a0.ncol - c0 ? chunk : a0.ncol - c0;
-        const int64_t o = c0 * nk;
-        const float *src[5];
-        float *dst[5];
-        for (int i = 0; i < narr; ++i) {
-            src[i] = src0[i] + o;
-            dst[i] = scratch.ptr + (size_t)i * (size_t)nk * (size_t)n;
-        }
-        if ((e = launch_transpose_levels(src, dst, narr, n, nk, s)) != hipSuccess) return e;
         CapeGridArgs a = a0;
         a.ncol = n;
-        a.ls = n;
-        a.t = dst[it];
-        a.td = dst[itd];
-        if (sigma) a.p = dst[ip];
+        if (stage) {
+            const int64_t o = c0 * nk;
+            const float *src[5];
+            float *dst[5];
+            for (int i = 0; i < narr; ++i) {
+                src[i] = src0[i] + o;
+                dst[i] = scratch.ptr + (size_t)i * (size_t)nk * (size_t)n;
+            }
+            if ((e = launch_transpose_levels(src, dst, narr, n, nk, s)) != hipSuccess) return e;
+            a.ls = n;
+            a.t = dst[it];
+            a.td = dst[itd];
+            if (sigma) a.p = dst[ip];
+            if (srh) { a.u = dst[iu]; a.v = dst[iv]; }
+        } else {
+            // level-major input (or a single level): the chunk's columns
+            // start c0 into each level's row
+            a.t += c0; a.td += c0;
+            if (sigma) a.p += c0;
+            if (srh) { a.u += c0; a.v += c0; }
+        }
         a.ps += c0; a.ts += c0; a.tds += c0;
         if (a.start) a.start += c0;
+        if (warm) {
+            float *w = scratch.ptr + (size_t)narr * (size_t)nk * (size_t)n;
+            const size_t lv = (size_t)(nk + 1) * (size_t)n;
+            a.env = w;
+            a.par = w + 4 * lv;
+            a.sur_in = a.par + 16 * (size_t)n;
+            a.work = work.ptr;
+        }
         if (cape) {
-            a.cape = static_cast<char *>(a.cape) + c0 * (math == 2 ? 8 : 4);
-            a.cin = static_cast<char *>(a.cin) + c0 * (math == 2 ? 8 : 4);
-            a.zout = static_cast<char *>(a.zout) + c0 * (math == 2 ? 8 : 4);
+            a.cape = static_cast<char *>(a.cape) + c0 * esz;
+            a.cin = static_cast<char *>(a.cin) + c0 * esz;
+            a.zout = static_cast<char *>(a.zout) + c0 * esz;
             a.mulvl += c0;
         }
         if (srh) {
-            a.u = dst[iu];
-            a.v = dst[iv];
             a.us += c0; a.vs += c0;
             a.srh_rm += c0; a.srh_lm += c0;
             if (a.rm) {
-                const int64_t b = 2 * c0 * (math == 2 ? 8 : 4);
+                const int64_t b = 2 * c0 * esz;
                 a.rm = static_cast<char *>(a.rm) + b;
                 a.lm = static_cast<char *>(a.lm) + b;
                 a.mean6 = static_cast<char *>(a.mean6) + b;

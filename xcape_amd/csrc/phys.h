@@ -115,6 +115,16 @@ XC_HD __forceinline__ T pred_div(T a, T b)
     return a / b;
 }
 
+// exp for the same kind of arithmetic: the hardware exp2 on the device.
+template <typename T>
+XC_HD __forceinline__ T pred_exp(T x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (sizeof(T) == 4) return Fn<float, MATH_FAST>::exp(x);
+#endif
+    return ::exp(x);
+}
+
 template <int MATH> struct Fn<double, MATH> {
     static XC_HD __forceinline__ double exp(double x) { return ::exp(x); }
     static XC_HD __forceinline__ double log(double x) { return ::log(x); }

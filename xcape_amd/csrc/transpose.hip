@@ -1,9 +1,17 @@
 // Column-contiguous (ncol, nk) -> level-major (nk, ncol) copies of up to five
-// profile arrays, staged through LDS in 64 x 64 tiles: a wave reads 64
-// consecutive levels of one column (256 contiguous bytes) and writes 64
-// consecutive columns of one level, so both directions are coalesced.  The
-// tile rows are padded to 65 words so that the transposed LDS reads hit 64
-// distinct banks.
+// profile arrays, staged through LDS.
+//
+// Slab kernel (nk <= 255): a workgroup copies 64 whole columns of one array.
+// Their nk * 64 values are one contiguous run of the source, read with
+// consecutive lanes on consecutive words, into LDS as 64 rows of an odd
+// pitch; each level is then written as 64 consecutive columns (256 bytes).
+// Every load and store is coalesced and no thread idles on a partial tile
+// (137 levels in 64-level tiles leave 55 of every 192 tile rows empty).
+//
+// Tile kernel (any nk): 64 x 64 tiles, a wave reading 64 consecutive levels
+// of one column and writing 64 consecutive columns of one level; the tile
+// rows are padded to 65 words so that the transposed LDS reads hit distinct
+// banks.
 //
 // HBM traffic: 8 bytes per element (one read, one write).
 #include <hip/hip_runtime.h>
@@ -46,6 +54,42 @@ __global__ void __launch_bounds__(256) transpose_levels_kernel(TransposeArgs a)
     }
 }
 
+constexpr int SLAB = 64;          // columns per workgroup of the slab kernel
+constexpr int SLAB_MAX_PITCH = 255;
+
+// blockIdx.x: 64-column slab, blockIdx.y: array; LDS: SLAB * pitch floats
+__global__ void __launch_bounds__(256) transpose_slab_kernel(TransposeArgs a, int pitch)
+{
+    extern __shared__ float slab[];
+    const float *src = a.src[blockIdx.y];
+    float *dst = a.dst[blockIdx.y];
+    const int64_t c0 = (int64_t)blockIdx.x * SLAB;
+    const int nk = (int)a.nk;
+    const int nc = (int)(a.ncol - c0 < SLAB ? a.ncol - c0 : SLAB);
+    const int total = nc * nk;
+    const float *s = src + c0 * a.nk;
+    // element e of the run is level k of column c: walk (c, k) along with e
+    int c = threadIdx.x / nk, k = threadIdx.x - c * nk;
+    const int dc = 256 / nk, dk = 256 - dc * nk;
+#pragma unroll 4
+    for (int e = threadIdx.x; e < total; e += 256) {
+        slab[c * pitch + k] = s[e];
+        k += dk;
+        c += dc;
+        if (k >= nk) {
+            k -= nk;
+            ++c;
+        }
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane < nc) {
+        float *d = dst + c0 + lane;
+#pragma unroll 4
+        for (int kk = w; kk < nk; kk += 4) d[(int64_t)kk * a.ncol] = slab[lane * pitch + kk];
+    }
+}
+
 }  // namespace
 
 hipError_t launch_transpose_levels(const float *const *src, float *const *dst, int narr,
@@ -62,7 +106,16 @@ hipError_t launch_transpose_levels(const float *const *src, float *const *dst, i
     }
     a.ncol = ncol;
     a.nk = nk;
-    transpose_levels_kernel<<<dim3((unsigned)gx, (unsigned)gy, (unsigned)narr), 256, 0, s>>>(a);
+    const int64_t pitch = nk | 1;   // odd: the column-wise LDS reads hit distinct banks
+    if (pitch <= SLAB_MAX_PITCH) {
+        const int64_t sx = (ncol + SLAB - 1) / SLAB;
+        if (sx > 0x7fffffff) return hipErrorInvalidValue;
+        transpose_slab_kernel<<<dim3((unsigned)sx, (unsigned)narr), 256,
+                                sizeof(float) * SLAB * (size_t)pitch, s>>>(a, (int)pitch);
+    } else {
+        transpose_levels_kernel<<<dim3((unsigned)gx, (unsigned)gy, (unsigned)narr), 256, 0, s>>>(
+            a);
+    }
     return hipGetLastError();
 }
 
