@@ -183,12 +183,14 @@ def pmc_summary(fetch_dir, write_dir, out, ncol, fast_math):
     kernels = {
         'plain': lambda n: 'cape_grid_kernel<float' in n,
         'warm_setup': lambda n: 'warm_setup_kernel<true, true, true, false>' in n,
-        'warm_ascent': lambda n: 'warm_ascent_kernel<true, true, true, false>' in n,
+        'warm_surrogate': lambda n: 'surrogate_guess_kernel' in n,
+        'warm_ascent': lambda n: 'warm_ascent_kernel<true, true, true, false, false>' in n,
     }
     res = {'ncol': ncol, 'fast_math': fast_math,
-           'kernels': {'plain': 'cape_grid_kernel<float, unsigned int, true, true, false>',
+           'kernels': {'plain': 'cape_grid_kernel<float, unsigned int, true, true, false, false>',
                        'warm': 'warm_setup_kernel<true, true, true, false> + '
-                               'warm_ascent_kernel<true, true, true, false>'}}
+                               'surrogate_guess_kernel + '
+                               'warm_ascent_kernel<true, true, true, false, false>'}}
     gib = float(1 << 30)
     for label, d, counter in (('fetch', fetch_dir, 'FETCH_SIZE'), ('write', write_dir, 'WRITE_SIZE')):
         disp = per_dispatch(d, counter)
@@ -200,9 +202,9 @@ def pmc_summary(fetch_dir, write_dir, out, ncol, fast_math):
             v = first(disp, pred)
             if v is not None:
                 res[f'{mode}_{label}_bytes'] = v * 1024.0 * factor
-        if f'warm_setup_{label}_bytes' in res and f'warm_ascent_{label}_bytes' in res:
-            res[f'warm_{label}_bytes'] = (res[f'warm_setup_{label}_bytes'] +
-                                          res[f'warm_ascent_{label}_bytes'])
+        parts = [f'warm_{x}_{label}_bytes' for x in ('setup', 'surrogate', 'ascent')]
+        if all(x in res for x in parts):
+            res[f'warm_{label}_bytes'] = sum(res[x] for x in parts)
     for mode in ('plain', 'warm'):
         if f'{mode}_fetch_bytes' in res and f'{mode}_write_bytes' in res:
             res[f'{mode}_hbm_bytes_per_launch'] = (res[f'{mode}_fetch_bytes'] +
@@ -400,6 +402,7 @@ def main():
     # the dominant kernel: the ascent (CAPE-only launch of the reported method)
     dom = 'warm' if warm else 'plain'
     achieved = ops / (ms_cape[dom] / 1e3) / 1e12
+    ops_exec = algorithmic_ops(counts[dom], NLEV, mu_levels)
     traffic = None
     if os.path.exists(PMC_FILE):
         with open(PMC_FILE) as f:
@@ -442,12 +445,16 @@ def main():
                    'ncol': ncol, 'nlev': NLEV, 'warm_start': warm,
                    'fast_math': bool(args.fast_math), 'parallelism': 'single GPU'},
         'roofline': {'bound': 'mfma', 'pipe': 'VALU FP32 (the ascent has no matrix work)',
-                     'kernel': ('warm_setup_kernel + warm_ascent_kernel (the warm-started '
-                                'CAPE-only call)' if warm else
+                     'kernel': ('warm_setup_kernel + surrogate_guess_kernel + '
+                                'warm_ascent_kernel (the warm-started CAPE-only call)' if warm else
                                 'cape_grid_kernel (the plain CAPE-only call)'),
                      'achieved': achieved, 'peak': FP32_PEAK_TFLOPS, 'unit': 'TFLOP/s',
                      'frac': achieved / FP32_PEAK_TFLOPS, 'traffic': traffic,
                      'ops_per_launch': ops, 'ms_per_launch': ms_cape[dom],
+                     # the same unit weights over the work the kernel executed
+                     # (its own iteration counts rather than the plain method's)
+                     'ops_executed_per_launch': ops_exec,
+                     'achieved_executed': ops_exec / (ms_cape[dom] / 1e3) / 1e12,
                      'hbm_gbs_algorithmic': bytes_per_column(NLEV) * value / 1e9},
         'plain_kernel': {'value': ncol / (ms_plain / 1e3), 'ms_per_step': ms_plain,
                          'accuracy': acc_plain},

@@ -108,6 +108,22 @@ int xcape_cape_f32(const float *p, int p_shared, const int32_t *start_lev,
                    int adiabat, float *cape, float *cin, int32_t *mulev, float *zmulev,
                    uint32_t flags, xcape_stats *stats, void *stream);
 
+/* As xcape_cape_f32, plus each column's level of free convection and
+ * equilibrium level, p_lfc and p_el [hPa] (NaN for columns without positive
+ * area).  No reference counterpart: getcape computes the crossing pressure
+ * ps (CAPE_CODE_model_lev.f90:511,526) and drops it.  Over the layers whose
+ * trapezoid adds positive area, the LFC is where the first one's positive
+ * area begins (ps where the parcel enters positive buoyancy in that layer,
+ * else the layer's bottom) and the EL where the last one's ends (ps where it
+ * leaves positive buoyancy, else the layer's top).  Same flags as
+ * xcape_cape_f32 except that stats are not collected. */
+int xcape_cape_levels_f32(const float *p, int p_shared, const int32_t *start_lev,
+                          const float *t, const float *td,
+                          const float *ps, const float *ts, const float *tds,
+                          int64_t nk, int64_t ncol, float pinc, int source, float ml_depth,
+                          int adiabat, float *cape, float *cin, int32_t *mulev, float *zmulev,
+                          float *p_lfc, float *p_el, uint32_t flags, void *stream);
+
 /* Same computation evaluated in FP64 on the device from the same FP32 inputs
  * (the accuracy yardstick for the FP32 kernel).  No reference counterpart:
  * the reference computes CAPE in FP32 only. */

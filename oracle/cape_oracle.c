@@ -70,11 +70,12 @@ float xo_getqvi_f32(float p, float t) { return qvi_f32(p, t); }
 float xo_getthe_f32(float p, float t, float td, float q) { return the_f32(p, t, td, q); }
 
 #define DEFINE_LOOP(SUFX, R)                                                         \
-void xo_cape##SUFX(const R *p, const R *p1d, const R *t, const R *td,                \
-                   const R *ps, const R *ts, const R *tds,                           \
-                   const int32_t *start_lev, int64_t nk_in, int64_t ncol,            \
-                   R pinc, int source, R ml_depth, int adiabat,                      \
-                   R *cape, R *cin, int32_t *mulvl, R *zout, xo_stats *st)           \
+void xo_cape_levels##SUFX(const R *p, const R *p1d, const R *t, const R *td,         \
+                          const R *ps, const R *ts, const R *tds,                    \
+                          const int32_t *start_lev, int64_t nk_in, int64_t ncol,     \
+                          R pinc, int source, R ml_depth, int adiabat,               \
+                          R *cape, R *cin, int32_t *mulvl, R *zout, R *plfc, R *pel, \
+                          xo_stats *st)                                              \
 {                                                                                    \
     R *work = (R *)malloc(sizeof(R) * 8 * (size_t)(nk_in + 1));                      \
     for (int64_t i = 0; i < ncol; ++i) {                                             \
@@ -90,12 +91,24 @@ void xo_cape##SUFX(const R *p, const R *p1d, const R *t, const R *td,           
             if (st) st->columns++;                                                   \
             getcape##SUFX((int)(nk_in - s0), pc, tc + s0, tdc + s0, ps[i], ts[i],    \
                           tds[i], pinc, source, ml_depth, adiabat, &cape[i], &cin[i], \
-                          &mulvl[i], &zout[i], work, st);                            \
+                          &mulvl[i], &zout[i], plfc ? &plfc[i] : NULL,               \
+                          pel ? &pel[i] : NULL, work, st);                           \
         } else {                                                                     \
             cape[i] = 0; cin[i] = 0; zout[i] = 0; mulvl[i] = 0;                      \
+            if (plfc) { plfc[i] = (R)NAN; pel[i] = (R)NAN; }                         \
         }                                                                            \
     }                                                                                \
     free(work);                                                                      \
+}                                                                                    \
+void xo_cape##SUFX(const R *p, const R *p1d, const R *t, const R *td,                \
+                   const R *ps, const R *ts, const R *tds,                           \
+                   const int32_t *start_lev, int64_t nk_in, int64_t ncol,            \
+                   R pinc, int source, R ml_depth, int adiabat,                      \
+                   R *cape, R *cin, int32_t *mulvl, R *zout, xo_stats *st)           \
+{                                                                                    \
+    xo_cape_levels##SUFX(p, p1d, t, td, ps, ts, tds, start_lev, nk_in, ncol, pinc,   \
+                         source, ml_depth, adiabat, cape, cin, mulvl, zout, NULL,    \
+                         NULL, st);                                                  \
 }
 
 DEFINE_LOOP(_f32, float)
@@ -115,7 +128,7 @@ int64_t xo_cape_log_f32(const float *p, const float *t, const float *td,
         int32_t m;
         getcape_log32((int)nk_in, p + i * nk_in, t + i * nk_in, td + i * nk_in, ps[i],
                       ts[i], tds[i], pinc, source, ml_depth, adiabat, &c, &ci, &m, &z,
-                      work, NULL, rec, &nrec, cap);
+                      NULL, NULL, work, NULL, rec, &nrec, cap);
     }
     free(work);
     return nrec;

@@ -48,6 +48,22 @@ void xo_cape_f32(const float *p, const float *p1d, const float *t, const float *
                  float *cape, float *cin, int32_t *mulvl, float *zout,
                  xo_stats *stats);
 
+/* The same with the LFC and EL pressures [hPa] of each column (NaN without
+ * positive area; no reference counterpart: defined from the reference's own
+ * crossing pressure, CAPE_CODE_model_lev.f90:511,526, see cape_column.inc). */
+void xo_cape_levels_f32(const float *p, const float *p1d, const float *t, const float *td,
+                        const float *ps, const float *ts, const float *tds,
+                        const int32_t *start_lev, int64_t nk_in, int64_t ncol,
+                        float pinc, int source, float ml_depth, int adiabat,
+                        float *cape, float *cin, int32_t *mulvl, float *zout,
+                        float *plfc, float *pel, xo_stats *stats);
+void xo_cape_levels_f64(const double *p, const double *p1d, const double *t, const double *td,
+                        const double *ps, const double *ts, const double *tds,
+                        const int32_t *start_lev, int64_t nk_in, int64_t ncol,
+                        double pinc, int source, double ml_depth, int adiabat,
+                        double *cape, double *cin, int32_t *mulvl, double *zout,
+                        double *plfc, double *pel, xo_stats *stats);
+
 void xo_cape_f64(const double *p, const double *p1d, const double *t, const double *td,
                  const double *ps, const double *ts, const double *tds,
                  const int32_t *start_lev, int64_t nk_in, int64_t ncol,

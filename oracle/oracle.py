@@ -62,9 +62,11 @@ def start_levels(p1d, ps):
 
 
 def cape(p, t, td, ps, ts, tds, pinc=500., source=1, ml_depth=500., adiabat=1,
-         p1d=None, start_lev=None, fp64=False, stats=False):
+         p1d=None, start_lev=None, fp64=False, stats=False, levels=False):
     """Column CAPE/CIN/MUlvl/z_out.  ``p`` is (ncol, nk) (ignored when ``p1d``
-    gives shared pressure levels, in which case ``start_lev`` is required)."""
+    gives shared pressure levels, in which case ``start_lev`` is required).
+    With ``levels`` the LFC and EL pressures [hPa] follow z_out (NaN without
+    positive area)."""
     dt = np.float64 if fp64 else np.float32
     t = _c(t, dt)
     ncol, nk = t.shape
@@ -81,16 +83,19 @@ def cape(p, t, td, ps, ts, tds, pinc=500., source=1, ml_depth=500., adiabat=1,
     mul = np.zeros(ncol, np.int32)
     zo = np.zeros(ncol, dt)
     st = Stats()
-    fn = lib().xo_cape_f64 if fp64 else lib().xo_cape_f32
+    lfc = np.zeros(ncol, dt)
+    el = np.zeros(ncol, dt)
+    fn = lib().xo_cape_levels_f64 if fp64 else lib().xo_cape_levels_f32
     R = ctypes.c_double if fp64 else ctypes.c_float
     fn.restype = None
     fn(_p(p), _p(p1d), _p(t), _p(td), _p(ps), _p(ts), _p(tds), _p(start_lev),
        ctypes.c_int64(nk), ctypes.c_int64(ncol), R(pinc), ctypes.c_int(source),
        R(ml_depth), ctypes.c_int(adiabat), _p(cape_o), _p(cin_o), _p(mul), _p(zo),
-       ctypes.byref(st))
+       _p(lfc) if levels else None, _p(el) if levels else None, ctypes.byref(st))
+    out = (cape_o, cin_o, mul, zo) + ((lfc, el) if levels else ())
     if stats:
-        return cape_o, cin_o, mul, zo, st.as_dict()
-    return cape_o, cin_o, mul, zo
+        return out + (st.as_dict(),)
+    return out
 
 
 def cape_log(p, t, td, ps, ts, tds, pinc=500., source=1, ml_depth=500., adiabat=1,

@@ -28,26 +28,29 @@ void run(const HostCol &c, int nk, float pinc, int source, float ml_depth, int w
     if (warm) {
         Parcel<float> pc;
         if (parcel_source<float, MATH_ACCURATE>(c, nk, source, ml_depth, pc))
-            cape_column_warm<float, MATH_ACCURATE, ICE, PSEUDO>(c, nk, pinc, pc, NAN, 1e-3f, o,
-                                                                st);
+            cape_column_warm<float, MATH_ACCURATE, ICE, PSEUDO, true>(c, nk, pinc, pc, NAN,
+                                                                      1e-3f, o, st);
         else {
             o.cape = 0.0f; o.cin = 0.0f; o.mulvl = pc.mulvl; o.zout = -999999.0f;
+            o.p_lfc = NAN; o.p_el = NAN;
         }
     } else {
-        cape_column<float, MATH_ACCURATE, ICE, PSEUDO>(c, nk, pinc, source, ml_depth, o, st);
+        cape_column<float, MATH_ACCURATE, ICE, PSEUDO, HostCol, NoLog, true>(
+            c, nk, pinc, source, ml_depth, o, st);
     }
 }
 
 }  // namespace
 
 // p1d != NULL: shared pressure levels with 1-based start levels; level_major:
-// (nk, ncol) profile arrays.  stats: levels, substeps, iterations,
-// nonconverged, fallbacks, first-sub-step iterations.
+// (nk, ncol) profile arrays.  lfc, el: the LFC/EL pressures (hPa), or NULL.
+// stats: levels, substeps, iterations, nonconverged, fallbacks,
+// first-sub-step iterations.
 extern "C" void host_cape(const float *p, const float *p1d, const int *start, const float *t,
                           const float *td, const float *ps, const float *ts, const float *tds,
                           int nk_in, int ncol, int level_major, float pinc, int source,
                           float ml_depth, int adiabat, int warm, float *cape, float *cin,
-                          int *mul, float *z, long long *stats)
+                          int *mul, float *z, float *lfc, float *el, long long *stats)
 {
     LaneStats st;
     for (int i = 0; i < ncol; ++i) {
@@ -73,6 +76,7 @@ extern "C" void host_cape(const float *p, const float *p1d, const int *start, co
         if (!(ts[i] > 0.0f)) {
             cape[i] = cin[i] = z[i] = 0.0f;
             mul[i] = 0;
+            if (lfc) { lfc[i] = NAN; el[i] = NAN; }
             continue;
         }
         switch (adiabat) {
@@ -82,6 +86,7 @@ extern "C" void host_cape(const float *p, const float *p1d, const int *start, co
         default: run<true, false>(c, nk, pinc, source, ml_depth, warm, o, st); break;
         }
         cape[i] = o.cape; cin[i] = o.cin; mul[i] = o.mulvl; z[i] = o.zout;
+        if (lfc) { lfc[i] = o.p_lfc; el[i] = o.p_el; }
     }
     stats[0] = st.levels; stats[1] = st.substeps; stats[2] = st.iterations;
     stats[3] = st.nonconverged; stats[4] = st.fallbacks; stats[5] = st.first;

@@ -134,3 +134,18 @@ def test_invalid_arguments_rejected_without_device():
     assert call(pinc=float('nan')) == -1
     assert call(nk=0) == -1
     assert L.xcape_version().startswith(b'xcape-hip')
+
+
+def test_levels_keyword_appends_two_outputs(p_t_td_3d, p_t_td_surface):
+    """levels=True (not in the reference) appends p_lfc, p_el shaped like the
+    other outputs; the fake backend gives NaN levels."""
+    p, t, td = p_t_td_3d
+    ps, ts, tds = p_t_td_surface
+    res = core.calc_cape(p, t, td, ps, ts, tds, source='most-unstable', vertical_lev='sigma',
+                         method='dummy', levels=True)
+    assert len(res) == 6
+    assert all(r.shape == p.shape[:-1] for r in res)
+    assert np.all(np.isnan(res[4])) and np.all(np.isnan(res[5]))
+    res = core.calc_cape(p, t, td, ps, ts, tds, source='surface', vertical_lev='sigma',
+                         method='dummy', levels=True)
+    assert len(res) == 4

@@ -206,10 +206,9 @@ def test_level_major_input_matches_column_contiguous(hip, warm):
 @pytest.mark.parametrize('adiabat', [1, 3, 4])
 @pytest.mark.parametrize('source', [1, 2, 3])
 def test_warm_grid_matches_ragged_warm(hip, source, adiabat):
-    """The grid's warm-started ascent (setup pass, then lanes taking columns
-    from a counter until the grid is drained) and the ragged kernel's (one
-    column per lane start to end) run the same column code in the accurate
-    math mode: results and work counters must be identical, bit for bit."""
+    """A uniform batch through a ragged plan (staged in sorted order, each
+    column's level count, source and output slot passed to the warm kernels)
+    gives the grid call's warm results and work counters bit for bit."""
     import torch
 
     from xcape_amd.ragged import RaggedBatch, RaggedPlan
@@ -246,34 +245,54 @@ def test_deep_columns_take_the_tile_transpose(hip):
 
 
 def test_staged_chunks_match_level_major(hip, monkeypatch):
-    """Column-contiguous grids larger than one staging chunk are staged and
-    computed chunk by chunk (outputs offset per chunk, 4- or 8-byte
-    elements): forced here to 700 columns a chunk, 3000 columns give five
-    chunks, the last one partial.  Every output equals the unstaged
-    level-major call's."""
+    """Grids larger than one scratch chunk are computed chunk by chunk
+    (outputs offset per chunk, 4- or 8-byte elements): forced here to 700
+    columns a chunk, 3000 columns give five chunks, the last one partial.
+    Column-contiguous (staged) and level-major input, chunked, both equal the
+    unchunked level-major call's outputs."""
     c = synthetic.sigma_columns(3000, 60, 23, winds=True)
     lmj = {k: (c[k].T.copy() if c[k].ndim == 2 else c[k]) for k in c}
-    monkeypatch.setenv('XCAPE_STAGE_COLUMNS', '700')
     cols = ('p', 't', 'td', 'ps', 'ts', 'tds')
-    for kw in (dict(), dict(warm_start='extrapolation'), dict(fp64=True),
-               dict(fast_math=True)):
-        kw.update(pinc=500., source=2, adiabat=3)
-        x = hip.cape(*(c[k] for k in cols), **kw)
-        y = hip.cape(*(lmj[k] for k in cols), level_major=True, **kw)
-        for a, b in zip(x, y):
-            np.testing.assert_array_equal(a, b)
     wcols = ('p', 't', 'td', 'u', 'v', 'ps', 'ts', 'tds', 'us', 'vs')
-    for fp64 in (False, True):
+    cases = [dict(), dict(warm_start='extrapolation'), dict(fp64=True), dict(fast_math=True),
+             dict(warm_start='extrapolation', fast_math=True)]
+    monkeypatch.delenv('XCAPE_STAGE_COLUMNS', raising=False)
+    ref = [hip.cape(*(lmj[k] for k in cols), level_major=True, pinc=500., source=2,
+                    adiabat=3, **kw) for kw in cases]
+    ref_srh = [hip.srh(*(lmj[k] for k in wcols), level_major=True, fp64=f)
+               for f in (False, True)]
+    ref_fused = hip.cape_srh(*(lmj[k] for k in wcols), pinc=500., source=2, adiabat=3,
+                             warm_start='extrapolation', level_major=True)
+    monkeypatch.setenv('XCAPE_STAGE_COLUMNS', '700')
+    for kw, r in zip(cases, ref):
+        for lm in (False, True):
+            src = lmj if lm else c
+            x = hip.cape(*(src[k] for k in cols), level_major=lm, pinc=500., source=2,
+                         adiabat=3, **kw)
+            for a, b in zip(x, r):
+                np.testing.assert_array_equal(a, b)
+    for fp64, r in zip((False, True), ref_srh):
         x = hip.srh(*(c[k] for k in wcols), fp64=fp64)
-        y = hip.srh(*(lmj[k] for k in wcols), level_major=True, fp64=fp64)
-        for a, b in zip(x, y):
+        for a, b in zip(x, r):
             np.testing.assert_array_equal(a, b)
     x = hip.cape_srh(*(c[k] for k in wcols), pinc=500., source=2, adiabat=3,
                      warm_start='extrapolation')
-    y = hip.cape_srh(*(lmj[k] for k in wcols), pinc=500., source=2, adiabat=3,
-                     warm_start='extrapolation', level_major=True)
-    for a, b in zip(x, y):
+    for a, b in zip(x, ref_fused):
         np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize('ncol', [1, 63, 700, 2047, 2049])
+def test_warm_small_grids(hip, oracle_lib, ncol):
+    """Grids smaller than the device: the persistent ascent's grid shrinks to
+    the columns (and its work parts with it); every column is computed."""
+    c = synthetic.sigma_columns(ncol, 60, 900 + ncol)
+    args = (c['p'], c['t'], c['td'], c['ps'], c['ts'], c['tds'])
+    o = oracle_lib.cape(*args, pinc=500., source=2, adiabat=3)
+    for warm in ('extrapolation', 'cold'):
+        g = hip.cape(*args, pinc=500., source=2, adiabat=3, warm_start=warm)
+        assert not _report(f'small {ncol} {warm} cape', g[0], o[0]).any()
+        assert not _report(f'small {ncol} {warm} cin', g[1], o[1]).any()
+        np.testing.assert_array_equal(g[2], o[2])
 
 
 @pytest.mark.parametrize('adiabat', [1, 2, 3, 4])
@@ -303,3 +322,58 @@ def test_warm_start_fast_math(hip, oracle_lib):
     o = oracle_lib.cape(*args, pinc=500., source=2, adiabat=3)
     assert not _report('warm fast cape', w[0], o[0]).any()
     assert not _report('warm fast cin', w[1], o[1]).any()
+
+
+@pytest.mark.parametrize('source', [1, 2, 3])
+@pytest.mark.parametrize('adiabat', [1, 3, 4])
+def test_lfc_el_against_oracle(hip, oracle_lib, source, adiabat):
+    """LFC and EL (xcape_cape_levels_f32; no reference output, the oracle
+    restates their definition from the reference's crossing pressure): the
+    plain method within 0.05 hPa of the oracle's for 99.5 % of the columns
+    (the device's exp/log/pow differ from the host's by an ulp, which moves a
+    crossing where the layer's buoyancy barely changes), the warm start within
+    1 hPa for 99 % (its fixed points move within the solver tolerance); both
+    present exactly where CAPE is positive; asking for them leaves the other
+    outputs unchanged."""
+    c = synthetic.sigma_columns(3000, 80, 61)
+    args = (c['p'], c['t'], c['td'], c['ps'], c['ts'], c['tds'])
+    kw = dict(pinc=500., source=source, adiabat=adiabat)
+    o = oracle_lib.cape(*args, levels=True, **kw)
+    for warm, tol, share in ((False, 0.05, 0.995), ('extrapolation', 1.0, 0.99)):
+        g = hip.cape(*args, warm_start=warm, levels=True, **kw)
+        base = hip.cape(*args, warm_start=warm, **kw)
+        for x, y in zip(g[:4], base):
+            np.testing.assert_array_equal(x, y)
+        pos = g[0] > 0
+        np.testing.assert_array_equal(np.isfinite(g[4]), pos)
+        np.testing.assert_array_equal(np.isfinite(g[5]), pos)
+        assert np.all(g[4][pos] >= g[5][pos])
+        both = pos & (o[0] > 0)
+        assert np.mean(pos == (o[0] > 0)) >= 0.999
+        for name, x, y in (('lfc', g[4], o[4]), ('el', g[5], o[5])):
+            d = np.abs(x[both] - y[both])
+            print(f'{name} s{source} a{adiabat} {warm}: max {d.max():.3g} '
+                  f'within {np.mean(d <= tol):.4f}', flush=True)
+            assert np.mean(d <= tol) >= share
+
+
+def test_lfc_el_pressure_grid_and_small(hip, oracle_lib):
+    """Shared pressure levels with start levels, and a grid of 37 columns,
+    warm-started (the persistent ascent on a small grid)."""
+    c = synthetic.pressure_columns(2000, 1003)
+    st = oracle_lib.start_levels(c['p'], c['ps'])
+    o = oracle_lib.cape(None, c['t'], c['td'], c['ps'], c['ts'], c['tds'], 500., 2, 500., 3,
+                        p1d=c['p'], start_lev=st, levels=True)
+    g = hip.cape(c['p'], c['t'], c['td'], c['ps'], c['ts'], c['tds'], pinc=500., source=2,
+                 adiabat=3, p_shared=True, levels=True)
+    both = (g[0] > 0) & (o[0] > 0)
+    assert both.sum() > 100
+    assert np.mean(np.abs(g[4][both] - o[4][both]) <= 0.05) >= 0.995
+    assert np.mean(np.abs(g[5][both] - o[5][both]) <= 0.05) >= 0.995
+    s = synthetic.sigma_columns(37, 60, 62)
+    a = (s['p'], s['t'], s['td'], s['ps'], s['ts'], s['tds'])
+    w = hip.cape(*a, pinc=500., source=1, adiabat=1, warm_start='extrapolation', levels=True)
+    r = oracle_lib.cape(*a, pinc=500., source=1, adiabat=1, levels=True)
+    np.testing.assert_array_equal(np.isfinite(w[4]), w[0] > 0)
+    pos = (w[0] > 0) & (r[0] > 0)
+    assert np.all(np.abs(w[4][pos] - r[4][pos]) <= 1.0)

@@ -54,12 +54,16 @@ def test_ragged_matches_oracle(batch_np, oracle_lib, cfg):
     plan.close()
 
 
-def test_graph_replays_match_direct_launch(batch_np):
+@pytest.mark.parametrize('warm', [False, True])
+def test_graph_replays_match_direct_launch(batch_np, warm):
+    """Replays of the plan's graph (staging, the segments' launches as
+    parallel branches; warm: setup, surrogate and persistent ascent per
+    segment) equal a direct launch, with eager work between replays."""
     import torch
 
     from xcape_amd.ragged import RaggedBatch, RaggedPlan
     b = RaggedBatch.from_dict(batch_np)
-    plan = RaggedPlan(b, chunk=64, tile=16, waves=4)
+    plan = RaggedPlan(b, chunk=64, tile=16, waves=4, warm_start=warm)
     plan.run(stats=True)              # direct launch
     torch.cuda.synchronize()
     ref = [x.clone() for x in (b.cape, b.cin, b.mulev, b.zmulev)]
@@ -95,9 +99,10 @@ def test_ragged_warm_start(batch_np, oracle_lib):
     plan = RaggedPlan(b, chunk=64, tile=16, waves=4, warm_start=True)
     st = plan.run(stats=True)
     torch.cuda.synchronize()
-    got = [x.cpu().numpy() for x in (b.cape, b.cin)]
+    got = [x.cpu().numpy() for x in (b.cape, b.cin, b.mulev)]
     ref = _oracle_ragged(oracle_lib, batch_np)
     print('ragged warm', st, flush=True)
     assert np.all(cape_ok(got[0], ref[0])) and np.all(cape_ok(got[1], ref[1]))
+    np.testing.assert_array_equal(got[2], ref[2])
     assert st['iterations'] < 0.45 * 9.5 * st['substeps']
     plan.close()

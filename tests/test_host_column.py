@@ -40,7 +40,7 @@ def _p(a):
 
 
 def host_cape(L, p, t, td, ps, ts, tds, source, adiabat, warm, p1d=None, start=None,
-              level_major=False, pinc=500.):
+              level_major=False, pinc=500., levels=False):
     ncol, nk = t.shape
     if level_major:
         t, td = np.ascontiguousarray(t.T), np.ascontiguousarray(td.T)
@@ -48,11 +48,15 @@ def host_cape(L, p, t, td, ps, ts, tds, source, adiabat, warm, p1d=None, start=N
             p = np.ascontiguousarray(p.T)
     out = [np.zeros(ncol, np.float32), np.zeros(ncol, np.float32), np.zeros(ncol, np.int32),
            np.zeros(ncol, np.float32)]
+    lv = [np.zeros(ncol, np.float32), np.zeros(ncol, np.float32)] if levels else [None, None]
     st = np.zeros(6, np.int64)
     L.host_cape(_p(p), _p(p1d), _p(start), _p(t), _p(td), _p(ps), _p(ts), _p(tds),
                 ctypes.c_int(nk), ctypes.c_int(ncol), ctypes.c_int(int(level_major)),
                 ctypes.c_float(pinc), ctypes.c_int(source), ctypes.c_float(500.),
-                ctypes.c_int(adiabat), ctypes.c_int(int(warm)), *[_p(o) for o in out], _p(st))
+                ctypes.c_int(adiabat), ctypes.c_int(int(warm)), *[_p(o) for o in out + lv],
+                _p(st))
+    if levels:
+        out = out + lv
     return out, dict(zip(('levels', 'substeps', 'iterations', 'nonconverged', 'fallbacks',
                           'first'), st.tolist()))
 
@@ -89,3 +93,25 @@ def test_pressure_levels_plain_bitexact(host_lib, source):
                            level_major=lm)
         for g, r in zip(got, ref):
             np.testing.assert_array_equal(g, r)
+
+
+@pytest.mark.parametrize('source', [1, 2, 3])
+@pytest.mark.parametrize('adiabat', [1, 2, 3, 4])
+def test_lfc_el_plain_bitexact_warm_close(host_lib, source, adiabat):
+    """The device column code's LFC/EL (host build): the plain method equals
+    the oracle's bit for bit; the warm start, whose fixed points differ within
+    the solver tolerance, moves them by little (crossings interpolate between
+    levels, so a buoyancy change of ~1e-5 m/s^2 shifts them by a fraction of
+    the layer where the layer's buoyancy change is small)."""
+    c = S.sigma_columns(400, 60, 51)
+    arr = [c[k] for k in ('p', 't', 'td', 'ps', 'ts', 'tds')]
+    ref = O.cape(*arr, 500., source, 500., adiabat, levels=True)
+    plain, _ = host_cape(host_lib, *arr, source, adiabat, False, levels=True)
+    for g, r in zip(plain, ref):
+        np.testing.assert_array_equal(g, r)
+    warm, _ = host_cape(host_lib, *arr, source, adiabat, True, levels=True)
+    pos = ref[0] > 0
+    np.testing.assert_array_equal(np.isfinite(warm[4]), np.isfinite(ref[4]))
+    for w, r in ((warm[4], ref[4]), (warm[5], ref[5])):
+        d = np.abs(w[pos] - r[pos])
+        assert np.mean(d <= 1.0) >= 0.99, np.sort(d)[-5:]

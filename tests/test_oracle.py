@@ -326,3 +326,89 @@ def test_bunkers_closed_form(oracle_lib):
     np.testing.assert_allclose(rm[:, 1], -7.5, rtol=1e-6)
     np.testing.assert_allclose(lm[:, 1], 7.5, rtol=1e-6)
     np.testing.assert_allclose(rm[:, 0], mean_u, rtol=1e-6)
+
+
+# ------------------------------------------------------------- LFC / EL --
+def _dry_column_levels(nk=80, ncol=40, seed=5):
+    """Bone-dry soundings whose buoyancy changes sign several times: the
+    environment's theta oscillates around the surface parcel's.  Returns the
+    inputs and, computed independently in float64 from the closed-form dry
+    ascent (the sub-steps telescope to theta_p = theta_s (p/p_s)^e with
+    e = R_m/c_pm - R_d/c_p), the expected LFC and EL [hPa]."""
+    rng = np.random.default_rng(seed)
+    rd, rv, cp, cpv, g = 287.04, 461.5, 1005.7, 1875.0, 9.81
+    kap, eps = rd / cp, rd / rv
+    ps = rng.uniform(980., 1020., ncol)
+    # top at 0.3 ps: with a dewpoint 100 K below, the parcel never saturates
+    p = ps[:, None] * np.linspace(0.99, 0.3, nk)[None, :]
+    ts = rng.uniform(295., 305., ncol)
+    amp = rng.uniform(1.0, 3.0, ncol)
+    per = rng.uniform(15000., 30000., ncol)
+    th_s = ts * (1000. / ps) ** kap
+    th_env = th_s[:, None] + amp[:, None] * np.cos(2 * np.pi * (ps[:, None] - p) * 100. / per[:, None])
+    t = th_env * (p / 1000.) ** kap
+    td = t - 100.
+    tds = ts - 100.
+    # float32 inputs, as the oracle will see them
+    P, T, TD = (np.float32(x) for x in (p, t - 273.15, td - 273.15))
+    PS, TS, TDS = (np.float32(x) for x in (ps, ts - 273.15, tds - 273.15))
+
+    def qsat(pp, tt):   # Bolton over liquid, as getqvs
+        es = 611.2 * np.exp(17.67 * (tt - 273.15) / (tt - 29.65))
+        return eps * es / (pp - es)
+    lfc = np.full(ncol, np.nan)
+    el = np.full(ncol, np.nan)
+    for i in range(ncol):
+        pa = 100. * np.concatenate([[PS[i]], P[i]]).astype(np.float64)
+        ta = 273.15 + np.concatenate([[TS[i]], T[i]]).astype(np.float64)
+        tda = 273.15 + np.concatenate([[TDS[i]], TD[i]]).astype(np.float64)
+        q = qsat(pa, tda)
+        th = ta * (1e5 / pa) ** kap
+        thv = th * (1 + q / eps) / (1 + q)
+        e = (rd + rv * q[0]) / (cp + cpv * q[0]) - rd / cp
+        thp = th[0] * (pa / pa[0]) ** e
+        b = g * (thp * (1 + q[0] / eps) / (1 + q[0]) - thv) / thv
+        b[0] = 0.0
+        for k in range(1, len(pa)):
+            b1, b2 = b[k - 1], b[k]
+            enter, leave = b2 >= 0 > b1, b2 < 0 < b1
+            parea = b2 if enter else (b1 if leave else (0.0 if b2 < 0 else b1 + b2))
+            if parea > 0:
+                ps_ = pa[k - 1] + (pa[k] - pa[k - 1]) * (0 - b1) / (b2 - b1)
+                if np.isnan(lfc[i]):
+                    lfc[i] = ps_ if enter else pa[k - 1]
+                el[i] = ps_ if leave else pa[k]
+            if pa[k] <= 10000. and b2 < 0:
+                break
+    return (P, T, TD, PS, TS, TDS), lfc / 100., el / 100.
+
+
+def test_lfc_el_dry_closed_form(oracle_lib):
+    """LFC and EL (no reference output) against the same crossings computed
+    independently from the closed-form dry ascent in float64."""
+    args, lfc, el = _dry_column_levels()
+    r = oracle_lib.cape(*args, pinc=500., source=1, adiabat=1, levels=True)
+    got_lfc, got_el = r[4], r[5]
+    assert np.all(np.isfinite(lfc)) and np.all(np.isfinite(el))
+    np.testing.assert_allclose(got_lfc, lfc, atol=0.01)
+    np.testing.assert_allclose(got_el, el, atol=0.01)
+
+
+@pytest.mark.parametrize('source', [1, 2, 3])
+@pytest.mark.parametrize('adiabat', [1, 3])
+def test_lfc_el_invariants(oracle_lib, source, adiabat):
+    """LFC and EL exist exactly when CAPE is positive, the LFC lies at or
+    below the EL (higher pressure) and at or above the surface, and asking for
+    them changes no other output."""
+    c = synthetic.sigma_columns(1500, 60, 33)
+    a = [c[k] for k in ('p', 't', 'td', 'ps', 'ts', 'tds')]
+    r = oracle_lib.cape(*a, pinc=500., source=source, adiabat=adiabat, levels=True)
+    cape, lfc, el = r[0], r[4], r[5]
+    pos = cape > 0
+    assert pos.sum() > 100
+    np.testing.assert_array_equal(np.isfinite(lfc), pos)
+    np.testing.assert_array_equal(np.isfinite(el), pos)
+    assert np.all(lfc[pos] >= el[pos])
+    assert np.all(lfc[pos] <= c['ps'][pos] + 1e-3)
+    for x, y in zip(r[:4], oracle_lib.cape(*a, pinc=500., source=source, adiabat=adiabat)):
+        np.testing.assert_array_equal(x, y)

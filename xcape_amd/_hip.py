@@ -58,6 +58,8 @@ SIGNATURES = {
     'xcape_device_count': (_I, []),
     'xcape_cape_f32': (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _I64, _I64, _F, _I, _F, _I,
                             _P, _P, _P, _P, _U32, _P, _P]),
+    'xcape_cape_levels_f32': (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _I64, _I64, _F, _I, _F, _I,
+                                   _P, _P, _P, _P, _P, _P, _U32, _P]),
     'xcape_cape_f64': (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _I64, _I64, _D, _I, _D, _I,
                             _P, _P, _P, _P, _U32, _P, _P]),
     'xcape_srh_f32': (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I64, _I64, _D,
@@ -208,10 +210,14 @@ def _warm_flags(warm_start):
 
 def cape(p, t, td, ps, ts, tds, *, pinc=500., source=1, ml_depth=500., adiabat=1,
          p_shared=False, start_lev=None, fast_math=False, warm_start=False, fp64=False,
-         stats=False, level_major=False):
-    """Run xcape_cape_f32 (or _f64).  t, td: (ncol, nk); p: (ncol, nk) or, with
-    p_shared, (nk,); with ``level_major`` the profile arrays are (nk, ncol).
-    Returns cape, cin, mulev, zmulev (and a stats dict)."""
+         stats=False, level_major=False, levels=False):
+    """Run xcape_cape_f32 (or _f64, or with ``levels`` xcape_cape_levels_f32).
+    t, td: (ncol, nk); p: (ncol, nk) or, with p_shared, (nk,); with
+    ``level_major`` the profile arrays are (nk, ncol).  Returns cape, cin,
+    mulev, zmulev (then p_lfc, p_el [hPa] with ``levels``; then a stats dict
+    with ``stats``)."""
+    if levels and (fp64 or stats):
+        raise ValueError('levels are computed by the float32 method without stats')
     A = _Args([p, t, td, ps, ts, tds, start_lev])
     p, t, td, ps, ts, tds = (A.f32(a) for a in (p, t, td, ps, ts, tds))
     start_lev = A.i32(start_lev)
@@ -222,6 +228,14 @@ def cape(p, t, td, ps, ts, tds, *, pinc=500., source=1, ml_depth=500., adiabat=1
     flags = (XCAPE_DEVICE_PTRS if A.device else 0) | (XCAPE_FAST_MATH if fast_math else 0) \
         | _warm_flags(warm_start) | (XCAPE_LEVEL_MAJOR if level_major else 0)
     st = Stats()
+    if levels:
+        lfc_o, el_o = (A.empty(ncol, np.float32, t) for _ in range(2))
+        rc = lib().xcape_cape_levels_f32(
+            _ptr(p), 1 if p_shared else 0, _ptr(start_lev), _ptr(t), _ptr(td), _ptr(ps),
+            _ptr(ts), _ptr(tds), nk, ncol, pinc, source, ml_depth, adiabat, _ptr(cape_o),
+            _ptr(cin_o), _ptr(mul_o), _ptr(z_o), _ptr(lfc_o), _ptr(el_o), flags, A.stream)
+        check(rc)
+        return cape_o, cin_o, mul_o, z_o, lfc_o, el_o
     fn = lib().xcape_cape_f64 if fp64 else lib().xcape_cape_f32
     rc = fn(_ptr(p), 1 if p_shared else 0, _ptr(start_lev), _ptr(t), _ptr(td), _ptr(ps),
             _ptr(ts), _ptr(tds), nk, ncol, pinc, source, ml_depth, adiabat,

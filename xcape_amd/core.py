@@ -115,8 +115,10 @@ def calc_cape(*args, **kwargs):
     Returns ``(cape, cin)``, or ``(cape, cin, mulev, zmulev)`` for
     ``source='most-unstable'``, shaped like the inputs without the level axis.
 
-    Extra keyword (not in the reference): ``fast_math`` selects the hardware
-    exp2/log2/reciprocal kernel variant; ``warm_start`` the learned warm start.
+    Extra keywords (not in the reference): ``fast_math`` selects the hardware
+    exp2/log2/reciprocal kernel variant; ``warm_start`` the learned warm start;
+    ``levels=True`` appends the level of free convection and the equilibrium
+    level, ``p_lfc, p_el`` [hPa] (NaN without positive area), to the outputs.
     """
     if len(args) < 6:
         raise ValueError("Too few arguments.")
@@ -148,18 +150,24 @@ def _calc_cape_gufunc(*args, **kwargs):
         if kwargs.get('source') == 'most-unstable':
             signature += ",(),()"
             output_dtypes = output_dtypes + ('i4', 'f4')
+        if kwargs.get('levels'):
+            signature += ",(),()"
+            output_dtypes = output_dtypes + ('f4', 'f4')
         return da.apply_gufunc(_fn, signature, *rest, output_dtypes=output_dtypes, axis=-1,
                                vectorize=False, **kwargs)
     if kwargs.get('source') == 'most-unstable':
         signature += ",(),()"
         output_dtypes = output_dtypes + ('i4', 'f4')
+    if kwargs.get('levels'):
+        signature += ",(),()"
+        output_dtypes = output_dtypes + ('f4', 'f4')
     return da.apply_gufunc(_calc_cape_numpy, signature, *args, output_dtypes=output_dtypes,
                            axis=-1, vectorize=False, **kwargs)
 
 
 def _calc_cape_numpy(*args, source='surface', ml_depth=500., adiabat='pseudo-liquid',
                      pinc=500., method='gpu', vertical_lev='sigma', fast_math=False,
-                     warm_start=False):
+                     warm_start=False, levels=False):
     """numpy path (core.py:261-332)."""
     p, t, td, ps, ts, tds = args
     original_shape = t.shape
@@ -187,19 +195,25 @@ def _calc_cape_numpy(*args, source='surface', ml_depth=500., adiabat='pseudo-liq
     adi = _ADIABAT[adiabat]
     _VLEV[vertical_lev]
 
+    lv = ()
     if method in _GPU_METHODS:
-        cape_2d, cin_2d, mulev, zmulev = _hip.cape(
+        out = _hip.cape(
             p_2d, t_2d, td_2d, p_s1d, t_s1d, td_s1d, pinc=float(pinc), source=src,
             ml_depth=float(ml_depth), adiabat=adi, p_shared=p_shared, start_lev=start,
-            fast_math=fast_math, warm_start=warm_start)
+            fast_math=fast_math, warm_start=warm_start, levels=levels)
+        cape_2d, cin_2d, mulev, zmulev = out[:4]
+        lv = tuple(out[4:])
     elif method == 'dummy':
         cape_2d, cin_2d, mulev, zmulev = _cape_dummy(p_2d, t_2d, td_2d, p_s1d, t_s1d, td_s1d)
+        if levels:
+            lv = (np.full_like(cape_2d, np.nan), np.full_like(cape_2d, np.nan))
     else:
         raise ValueError('invalid method')
 
     if src == 2:
-        return tuple(_reshape_outputs(cape_2d, cin_2d, mulev, zmulev, shape=original_shape))
-    return tuple(_reshape_outputs(cape_2d, cin_2d, shape=original_shape))
+        return tuple(_reshape_outputs(cape_2d, cin_2d, mulev, zmulev, *lv,
+                                      shape=original_shape))
+    return tuple(_reshape_outputs(cape_2d, cin_2d, *lv, shape=original_shape))
 
 
 def calc_srh(*args, **kwargs):

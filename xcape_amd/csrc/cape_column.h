@@ -54,6 +54,7 @@ XC_HD __forceinline__ void xc_opaque(int &v)
 template <typename T> struct ColumnOut {
     T cape, cin, zout;
     int mulvl;
+    T p_lfc, p_el;   // [hPa]; only from the drivers' LEVELS variants
 };
 
 struct LaneStats {
@@ -315,12 +316,42 @@ XC_HD __forceinline__ bool parcel_source(const C &col, int nk, int source, T ml_
     return true;
 }
 
+// The level of free convection and the equilibrium level, from the
+// reference's own crossing pressure ps (:511, :526), which it computes and
+// drops (the reference has no such output).  Over the layers whose trapezoid
+// adds positive area: the LFC is where the first one's positive area begins
+// (ps when the parcel enters positive buoyancy in that layer, else the
+// layer's bottom), the EL where the last one's ends (ps when it leaves
+// positive buoyancy, else the layer's top).  Pressures in Pa; NaN while there
+// is no positive area.
+template <typename T, int MATH> struct LevelTrack {
+    T lfc = T(NAN), el = T(NAN);
+    XC_HD __forceinline__ void add(bool enter, bool leave, T parea, T b1, T b2, T p_below,
+                                   T p_k)
+    {
+        if (!(parea > T(0.0))) return;
+        const T ps = (enter || leave)
+                         ? p_below + Fn<T, MATH>::div((p_k - p_below) * (T(0.0) - b1), b2 - b1)
+                         : T(0.0);
+        if (lfc != lfc) lfc = enter ? ps : p_below;
+        el = leave ? ps : p_k;
+    }
+};
+
+// The default: no levels tracked.
+struct NoTrack {
+    template <typename T>
+    XC_HD __forceinline__ void add(bool, bool, T, T, T, T, T) {}
+};
+
 // Buoyancy of the parcel at level k and the CAPE/CIN trapezoid of the layer
 // below it (:501-547).  CIN is committed only on entering positive buoyancy.
-template <typename T, int MATH>
+// p_below: the pressure of level k - 1, for the tracker.
+template <typename T, int MATH, class TRK = NoTrack>
 XC_HD __forceinline__ void layer_update(T th2, T qv2, T ql2, T qi2, const Env<T> &e,
                                              T thv_prev, T pi_prev, T b1, T &b2, T &dz,
-                                             T &cape, T &cin, T &narea)
+                                             T &cape, T &cin, T &narea, T p_below = T(0.0),
+                                             TRK *trk = nullptr)
 {
     using F = Fn<T, MATH>;
     using K = Phys<T>;
@@ -351,6 +382,7 @@ XC_HD __forceinline__ void layer_update(T th2, T qv2, T ql2, T qi2, const Env<T>
         narea = T(0.0);
     }
     cape = cape + fmx(T(0.0), parea);
+    if (trk) trk->add(enter, leave, parea, b1, b2, p_below, e.p);
 }
 
 // sub-step count and size for the layer p_prev -> p_k (:408-415)
@@ -373,17 +405,22 @@ struct NoLog {
     XC_HD __forceinline__ void operator()(T, T, T, T, T, T, T) const {}
 };
 
-// Reference driver: nested loops, cold start.
-template <typename T, int MATH, bool ICE, bool PSEUDO, class C, class LOG = NoLog>
+// Reference driver: nested loops, cold start.  LEVELS: also the LFC and EL
+// (LevelTrack).
+template <typename T, int MATH, bool ICE, bool PSEUDO, class C, class LOG = NoLog,
+          bool LEVELS = false>
 XC_HD void cape_column(const C &col, int nk, T pinc, int source, T ml_depth,
                        ColumnOut<T> &out, LaneStats &st, const LOG &log = LOG())
 {
     using K = Phys<T>;
     Parcel<T> pc;
+    out.p_lfc = T(NAN);
+    out.p_el = T(NAN);
     if (!parcel_source<T, MATH>(col, nk, source, ml_depth, pc)) {
         out.cape = T(0.0); out.cin = T(0.0); out.mulvl = pc.mulvl; out.zout = T(-999999.0);
         return;
     }
+    LevelTrack<T, MATH> trk;
     const int mulvl = pc.mulvl;
     T zout = T(-999999.0);
     T th2 = pc.th2, pi2 = pc.e.pi, p2 = pc.e.p, t2 = pc.t2, qv2 = pc.qv2, b2 = pc.b2, zk = pc.zk;
@@ -428,14 +465,22 @@ XC_HD void cape_column(const C &col, int nk, T pinc, int source, T ml_depth,
             }
         }
         T dz;
-        layer_update<T, MATH>(th2, qv2, ql2, qi2, e, thv_prev, pi_prev, b1, b2, dz, cape, cin,
-                              narea);
+        if constexpr (LEVELS)
+            layer_update<T, MATH>(th2, qv2, ql2, qi2, e, thv_prev, pi_prev, b1, b2, dz, cape,
+                                  cin, narea, p_prev, &trk);
+        else
+            layer_update<T, MATH>(th2, qv2, ql2, qi2, e, thv_prev, pi_prev, b1, b2, dz, cape,
+                                  cin, narea);
         if (e.p <= T(10000.0) && b2 < T(0.0)) doit = false;
         zk = zk + dz;
         zout = zk;
         p_prev = e.p; pi_prev = e.pi; thv_prev = e.thv;
     }
     out.cape = cape; out.cin = cin; out.mulvl = mulvl; out.zout = zout;
+    if constexpr (LEVELS) {
+        out.p_lfc = Fn<T, MATH>::div(trk.lfc, T(100.0));
+        out.p_el = Fn<T, MATH>::div(trk.el, T(100.0));
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -505,7 +550,7 @@ XC_HD __forceinline__ T adiabat_shape(T t, T p, T th)
     return pred_div(th * qs * (a1 - t), t * t + a2 * qs);
 }
 
-template <typename T, int MATH, bool ICE, bool PSEUDO>
+template <typename T, int MATH, bool ICE, bool PSEUDO, bool LEVELS = false>
 struct WarmAscent {
     enum { M_LEVEL = 0, M_SUB = 1, M_ITER = 2, M_CONV = 3, M_LAYER = 4 };
     int mode, k, nk, left, it, nh;   // left: sub-steps of the level not yet begun
@@ -534,6 +579,9 @@ struct WarmAscent {
     // or pure ice regime), and whether the last sub-step condensed
     T gs;
     bool sat;
+    // LEVELS: the LFC/EL tracker and the previous level's pressure
+    LevelTrack<T, MATH> trk;
+    T pb;
 
     // Starts the ascent of parcel pc in a column of nk levels.  Returns false
     // when there is no level above the parcel (result() is then final).
@@ -557,6 +605,10 @@ struct WarmAscent {
         r1 = guess; r2 = T(0.0); dl1 = T(0.0); dl2 = T(0.0);
         gs = T(NAN);
         sat = false;
+        if constexpr (LEVELS) {
+            trk = LevelTrack<T, MATH>{};
+            pb = pc.e.p;
+        }
         mode = M_LEVEL;
         return k < nk - 1;
     }
@@ -568,6 +620,10 @@ struct WarmAscent {
     XC_HD __forceinline__ void result(ColumnOut<T> &out) const
     {
         out.cape = cape; out.cin = cin; out.zout = layered ? zk : T(-999999.0);
+        if constexpr (LEVELS) {
+            out.p_lfc = Fn<T, MATH>::div(trk.lfc, T(100.0));
+            out.p_el = Fn<T, MATH>::div(trk.el, T(100.0));
+        }
     }
 
     // One trip.  Returns true when the column is finished (terminated, top
@@ -593,6 +649,7 @@ struct WarmAscent {
         if (mode == M_LEVEL) {
             k = k + 1;
             const T p_prev = e.p;
+            if constexpr (LEVELS) pb = p_prev;
             if constexpr (HasEnv<C>::value) {
                 e = nx;
                 if (k + 1 < nk) nx = col.env(k + 1);
@@ -655,6 +712,8 @@ struct WarmAscent {
                 st.nonconverged++;
                 cape = T(0.0);
                 cin = T(0.0);
+                // a column stopped by the iteration limit reports no levels
+                if constexpr (LEVELS) trk = LevelTrack<T, MATH>{};
                 return true;
             }
             if (warm && it == 1 && ad > fallback_tol) {
@@ -741,8 +800,12 @@ struct WarmAscent {
             // level complete (:501-558)
             T dz;
             const T b1 = b2;
-            layer_update<T, MATH>(th2, qv2, ql2, qi2, e, thv_prev, pi_prev, b1, b2, dz, cape,
-                                  cin, narea);
+            if constexpr (LEVELS)
+                layer_update<T, MATH>(th2, qv2, ql2, qi2, e, thv_prev, pi_prev, b1, b2, dz,
+                                      cape, cin, narea, pb, &trk);
+            else
+                layer_update<T, MATH>(th2, qv2, ql2, qi2, e, thv_prev, pi_prev, b1, b2, dz,
+                                      cape, cin, narea);
             zk = zk + dz;
             layered = true;
             pi_prev = e.pi; thv_prev = e.thv;
@@ -757,12 +820,12 @@ struct WarmAscent {
 // have used it to build the first sub-step's surrogate inputs).  predict =
 // false starts every sub-step cold: the same loop without prediction, which
 // separates the loop's share of the speed-up from the prediction's.
-template <typename T, int MATH, bool ICE, bool PSEUDO, class C>
+template <typename T, int MATH, bool ICE, bool PSEUDO, bool LEVELS = false, class C>
 XC_HD void cape_column_warm(const C &col, int nk, T pinc, const Parcel<T> &pc,
                                  T first_guess, T fallback_tol, ColumnOut<T> &out,
                                  LaneStats &st, bool predict = true)
 {
-    WarmAscent<T, MATH, ICE, PSEUDO> w;
+    WarmAscent<T, MATH, ICE, PSEUDO, LEVELS> w;
     out.mulvl = pc.mulvl;
     if (w.init(pc, nk, first_guess)) {
         for (;;)

@@ -112,7 +112,7 @@ __device__ __forceinline__ bool column_at(const CapeGridArgs &a, int64_t col, Si
         c.p = a.p;
         c.poff = off;
         c.pls = c.ls;
-        nk = (int)a.nk_in + 1;
+        nk = (a.nk_col ? (int)a.nk_col[col] : (int)a.nk_in) + 1;
     }
     c.off = off;
     c.t = a.t;
@@ -123,6 +123,13 @@ __device__ __forceinline__ bool column_at(const CapeGridArgs &a, int64_t col, Si
     c.ts = a.ts[col];
     c.tds = a.tds[col];
     return true;
+}
+
+// The output slot of column `col` (a.out_index maps a batch's work order back
+// to its callers' order).
+__device__ __forceinline__ int64_t out_col(const CapeGridArgs &a, int64_t col)
+{
+    return a.out_index ? (int64_t)a.out_index[col] : col;
 }
 
 // First-sub-step inputs of the surrogate for a parcel: ln p2, the parcel's
@@ -150,8 +157,8 @@ __device__ __forceinline__ void surrogate_inputs(const C &c, int nk, const Parce
 // CAPE/CIN with the reference's loops and cold start (cape_column), one
 // column per lane.  STATS: accumulate the work counters into a.stats (a
 // separate instantiation, so that the counters do not occupy registers in
-// the timed kernels).
-template <typename T, typename IDX, bool ICE, bool PSEUDO, bool STATS>
+// the timed kernels).  LEVELS: also the LFC and EL (a.p_lfc, a.p_el).
+template <typename T, typename IDX, bool ICE, bool PSEUDO, bool STATS, bool LEVELS = false>
 __global__ void __launch_bounds__(256) cape_grid_kernel(CapeGridArgs a)
 {
     const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -166,15 +173,20 @@ __global__ void __launch_bounds__(256) cape_grid_kernel(CapeGridArgs a)
         ColumnOut<T> o;
         if (c.ts > 0.0f) {
             ran = true;
-            cape_column<T, XC_MATH, ICE, PSEUDO>(c, nk, T(a.pinc), a.source, T(a.ml_depth), o,
-                                                 st);
+            cape_column<T, XC_MATH, ICE, PSEUDO, SigmaCol<IDX>, NoLog, LEVELS>(
+                c, nk, T(a.pinc), a.source, T(a.ml_depth), o, st);
         } else {
             o.cape = T(0); o.cin = T(0); o.zout = T(0); o.mulvl = 0;
+            o.p_lfc = T(NAN); o.p_el = T(NAN);
         }
         cape[col] = o.cape;
         cin[col] = o.cin;
         a.mulvl[col] = o.mulvl;
         zout[col] = o.zout;
+        if constexpr (LEVELS) {
+            a.p_lfc[col] = (float)o.p_lfc;
+            a.p_el[col] = (float)o.p_el;
+        }
     }
     if constexpr (STATS) block_stats_add(st, ran, a.stats);
 }
@@ -261,8 +273,9 @@ __global__ void __launch_bounds__(256) warm_setup_kernel(CapeGridArgs a)
             env[(uint32_t)k * ls + j] = make_float4(e.p, e.pi, e.thv, 0.0f);
             if (k == cover) cover = k + 1;
         };
-        ok = parcel_source<T, XC_MATH>(c, nk, a.source, T(a.ml_depth), pc, sink);
-        a.mulvl[col] = pc.mulvl;
+        const int source = a.src_col ? a.src_col[col] : a.source;
+        ok = parcel_source<T, XC_MATH>(c, nk, source, T(a.ml_depth), pc, sink);
+        a.mulvl[out_col(a, col)] = pc.mulvl;
         if (SUR && ok && pc.kmax < nk - 1) surrogate_inputs(c, nk, pc, a.pinc, f0, f1, f2, lp1);
     }
     const bool go = ok && pc.kmax < nk - 1;
@@ -301,10 +314,15 @@ __global__ void __launch_bounds__(256) warm_setup_kernel(CapeGridArgs a)
         if constexpr (SUR) a.sur_in[3 * a.ncol + col] = T(0.0);
         T *cape = static_cast<T *>(a.cape), *cin = static_cast<T *>(a.cin),
           *zout = static_cast<T *>(a.zout);
-        cape[col] = T(0.0);
-        cin[col] = T(0.0);
-        zout[col] = ran ? T(-999999.0) : T(0.0);
-        if (!ran) a.mulvl[col] = 0;
+        const int64_t oc = out_col(a, col);
+        cape[oc] = T(0.0);
+        cin[oc] = T(0.0);
+        zout[oc] = ran ? T(-999999.0) : T(0.0);
+        if (!ran) a.mulvl[oc] = 0;
+        if (a.p_lfc) {
+            a.p_lfc[oc] = T(NAN);
+            a.p_el[oc] = T(NAN);
+        }
     }
     if constexpr (STATS) block_stats_add(LaneStats{}, ran, a.stats);
 }
@@ -333,22 +351,30 @@ __global__ void __launch_bounds__(256) surrogate_guess_kernel(CapeGridArgs a)
     if (go) rec[col].guess = lp1 != 0.0f ? rec[col].th2 + r * lp1 : NAN;
 }
 
-// The ascent's columns are split into WORK_PARTS contiguous parts with a
-// counter each.  Workgroups are dispatched to the eight XCDs in turn, so the
-// waves of part blockIdx % 8 all run on one XCD.  (Every part holds an eighth
-// of the columns and every XCD as many waves, so the parts drain together.)
-// A wave takes its part's columns 64 at a time (one atomic per 64 columns);
-// its lanes draw from that block as they finish their columns.
+// The ascent's columns are split into `parts` contiguous parts with a counter
+// each, parts = min(8, grid size), and the grid is a multiple of it.
+// Workgroups are dispatched to the eight XCDs in turn, so the waves of part
+// blockIdx % 8 all run on one XCD.  (Every part holds an equal share of the
+// columns and as many waves, so the parts drain together.)  A wave takes its
+// part's columns 64 at a time (one atomic per 64 columns); its lanes draw
+// from that block as they finish their columns.
 constexpr uint32_t WORK_PARTS = 8;
 constexpr uint32_t WORK_BLOCK = 64;
 
-__device__ __forceinline__ uint32_t work_part_begin(uint32_t ncol, uint32_t part)
+__device__ __forceinline__ uint32_t work_part_begin(uint32_t ncol, uint32_t part,
+                                                    uint32_t parts)
 {
-    return (uint32_t)(((uint64_t)ncol * part) / WORK_PARTS);
+    return (uint32_t)(((uint64_t)ncol * part) / parts);
 }
 
-// PREDICT: the warm start (false: the same loop with every sub-step cold)
-template <bool ICE, bool PSEUDO, bool PREDICT, bool STATS>
+__host__ __device__ __forceinline__ uint32_t work_parts(uint32_t grid)
+{
+    return grid < WORK_PARTS ? grid : WORK_PARTS;
+}
+
+// PREDICT: the warm start (false: the same loop with every sub-step cold).
+// LEVELS: also the LFC and EL.
+template <bool ICE, bool PSEUDO, bool PREDICT, bool STATS, bool LEVELS = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
 warm_ascent_kernel(CapeGridArgs a)
 {
@@ -359,14 +385,15 @@ warm_ascent_kernel(CapeGridArgs a)
     const float4 *recs = reinterpret_cast<const float4 *>(a.par);
     const int lane = threadIdx.x & 63;
     const uint64_t below = (1ull << lane) - 1ull;
-    WarmAscent<T, XC_MATH, ICE, PSEUDO> w;
+    WarmAscent<T, XC_MATH, ICE, PSEUDO, LEVELS> w;
     LaneStats st;
     uint32_t col = 0;
     bool have = false, need = true;
     // wave-uniform: the wave's part (its XCD's), its current block of columns
     // [blk, blk + 64) and the block's slots handed out
-    const uint32_t part = blockIdx.x % WORK_PARTS;
-    const uint32_t part_end = work_part_begin(ncol, part + 1);
+    const uint32_t parts = work_parts(gridDim.x);
+    const uint32_t part = blockIdx.x % parts;
+    const uint32_t part_end = work_part_begin(ncol, part + 1, parts);
     uint32_t blk = 0, used = WORK_BLOCK;
     bool drained = false;
     for (;;) {
@@ -424,9 +451,14 @@ warm_ascent_kernel(CapeGridArgs a)
             if (w.trip(ec, T(a.pinc), T(a.fallback_tol), st, PREDICT)) {
                 ColumnOut<T> o;
                 w.result(o);
-                cape[col] = o.cape;
-                cin[col] = o.cin;
-                zout[col] = o.zout;
+                const int64_t oc = out_col(a, col);
+                cape[oc] = o.cape;
+                cin[oc] = o.cin;
+                zout[oc] = o.zout;
+                if constexpr (LEVELS) {
+                    a.p_lfc[oc] = o.p_lfc;
+                    a.p_el[oc] = o.p_el;
+                }
                 have = false;
                 need = true;
             }
@@ -435,9 +467,10 @@ warm_ascent_kernel(CapeGridArgs a)
     if constexpr (STATS) block_stats_add(st, false, a.stats);
 }
 
-__global__ void warm_reset_kernel(unsigned int *work, uint32_t ncol)
+__global__ void warm_reset_kernel(unsigned int *work, uint32_t ncol, uint32_t grid)
 {
-    if (threadIdx.x < WORK_PARTS) work[threadIdx.x] = work_part_begin(ncol, threadIdx.x);
+    const uint32_t parts = work_parts(grid);
+    if (threadIdx.x < parts) work[threadIdx.x] = work_part_begin(ncol, threadIdx.x, parts);
 }
 
 // Storm-relative helicity chain, one column per lane.  TB: the storm-motion
@@ -467,35 +500,42 @@ __global__ void __launch_bounds__(256) srh_grid_kernel(CapeGridArgs a)
     }
 }
 
-template <typename T, typename IDX, bool STATS>
+template <typename T, typename IDX, bool STATS, bool LEVELS>
 hipError_t launch_cape_s(const CapeGridArgs &a, int block, hipStream_t s)
 {
     dim3 grid((unsigned)((a.ncol + block - 1) / block));
     switch (a.adiabat) {
-    case 1: cape_grid_kernel<T, IDX, false, true, STATS><<<grid, block, 0, s>>>(a); break;
-    case 2: cape_grid_kernel<T, IDX, false, false, STATS><<<grid, block, 0, s>>>(a); break;
-    case 3: cape_grid_kernel<T, IDX, true, true, STATS><<<grid, block, 0, s>>>(a); break;
-    case 4: cape_grid_kernel<T, IDX, true, false, STATS><<<grid, block, 0, s>>>(a); break;
+    case 1: cape_grid_kernel<T, IDX, false, true, STATS, LEVELS><<<grid, block, 0, s>>>(a); break;
+    case 2: cape_grid_kernel<T, IDX, false, false, STATS, LEVELS><<<grid, block, 0, s>>>(a); break;
+    case 3: cape_grid_kernel<T, IDX, true, true, STATS, LEVELS><<<grid, block, 0, s>>>(a); break;
+    case 4: cape_grid_kernel<T, IDX, true, false, STATS, LEVELS><<<grid, block, 0, s>>>(a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
 // 32-bit element offsets when every index of the profile arrays fits (the
-// stats launches, which only count work, always use 64-bit ones)
+// stats launches, which only count work, always use 64-bit ones).  The LFC/EL
+// outputs come from float, uncounted launches.
 template <typename T>
 hipError_t launch_cape(const CapeGridArgs &a, int block, hipStream_t s)
 {
-    if (a.stats) return launch_cape_s<T, uint64_t, true>(a, block, s);
+    if (a.p_lfc && (sizeof(T) != 4 || a.stats)) return hipErrorInvalidValue;
+    if (a.stats) return launch_cape_s<T, uint64_t, true, false>(a, block, s);
     const bool fits = (uint64_t)a.ncol * (uint64_t)a.nk_in < ((uint64_t)1 << 32);
-    return fits ? launch_cape_s<T, uint32_t, false>(a, block, s)
-                : launch_cape_s<T, uint64_t, false>(a, block, s);
+    if constexpr (sizeof(T) == 4) {
+        if (a.p_lfc)
+            return fits ? launch_cape_s<T, uint32_t, false, true>(a, block, s)
+                        : launch_cape_s<T, uint64_t, false, true>(a, block, s);
+    }
+    return fits ? launch_cape_s<T, uint32_t, false, false>(a, block, s)
+                : launch_cape_s<T, uint64_t, false, false>(a, block, s);
 }
 
 // Blocks of the persistent ascent: as many as the device holds at once, never
 // more than the columns need (occupancy queried once per instantiation and
 // block size)
-template <bool ICE, bool PSEUDO, bool PREDICT, bool STATS>
+template <bool ICE, bool PSEUDO, bool PREDICT, bool STATS, bool LEVELS>
 int ascent_grid(int block, int64_t ncol)
 {
     static int per_cu[5] = {-1, -1, -1, -1, -1};   // by block / 64
@@ -508,7 +548,7 @@ int ascent_grid(int block, int64_t ncol)
         (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
                 &pc, reinterpret_cast<const void *>(
-                         warm_ascent_kernel<ICE, PSEUDO, PREDICT, STATS>),
+                         warm_ascent_kernel<ICE, PSEUDO, PREDICT, STATS, LEVELS>),
                 block, 0) != hipSuccess ||
             pc < 1)
             pc = 1;
@@ -516,10 +556,13 @@ int ascent_grid(int block, int64_t ncol)
     }
     const int64_t need = (ncol + block - 1) / block;
     const int64_t full = (int64_t)(ncu > 0 ? ncu : 1) * per_cu[slot];
-    return (int)(need < full ? need : full);
+    int64_t g = need < full ? need : full;
+    // a multiple of the work parts, so that every part has as many waves
+    if (g >= (int64_t)WORK_PARTS) g -= g % WORK_PARTS;
+    return (int)g;
 }
 
-template <bool ICE, bool PSEUDO, bool SUR, bool PREDICT, bool STATS>
+template <bool ICE, bool PSEUDO, bool SUR, bool PREDICT, bool STATS, bool LEVELS>
 hipError_t launch_warm_s(const CapeGridArgs &a, int block, hipStream_t s)
 {
     const dim3 sgrid((unsigned)((a.ncol + 255) / 256));
@@ -531,10 +574,11 @@ hipError_t launch_warm_s(const CapeGridArgs &a, int block, hipStream_t s)
         surrogate_guess_kernel<<<sgrid, 256, lds, s>>>(a);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
-    warm_reset_kernel<<<1, 64, 0, s>>>(a.work, (uint32_t)a.ncol);
+    const int grid = ascent_grid<ICE, PSEUDO, PREDICT, STATS, LEVELS>(block, a.ncol);
+    // the counters start at their parts' first columns, for the grid's parts
+    warm_reset_kernel<<<1, 64, 0, s>>>(a.work, (uint32_t)a.ncol, (uint32_t)grid);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    warm_ascent_kernel<ICE, PSEUDO, PREDICT, STATS>
-        <<<ascent_grid<ICE, PSEUDO, PREDICT, STATS>(block, a.ncol), block, 0, s>>>(a);
+    warm_ascent_kernel<ICE, PSEUDO, PREDICT, STATS, LEVELS><<<grid, block, 0, s>>>(a);
     return hipGetLastError();
 }
 
@@ -544,13 +588,19 @@ hipError_t launch_warm_a(const CapeGridArgs &a, int block, hipStream_t s)
     const bool sur = a.warm == 1 && a.surrogate;
     const bool predict = a.warm == 1;
     if (a.stats) {
-        if (sur) return launch_warm_s<ICE, PSEUDO, true, true, true>(a, block, s);
-        return predict ? launch_warm_s<ICE, PSEUDO, false, true, true>(a, block, s)
-                       : launch_warm_s<ICE, PSEUDO, false, false, true>(a, block, s);
+        if (a.p_lfc) return hipErrorInvalidValue;
+        if (sur) return launch_warm_s<ICE, PSEUDO, true, true, true, false>(a, block, s);
+        return predict ? launch_warm_s<ICE, PSEUDO, false, true, true, false>(a, block, s)
+                       : launch_warm_s<ICE, PSEUDO, false, false, true, false>(a, block, s);
     }
-    if (sur) return launch_warm_s<ICE, PSEUDO, true, true, false>(a, block, s);
-    return predict ? launch_warm_s<ICE, PSEUDO, false, true, false>(a, block, s)
-                   : launch_warm_s<ICE, PSEUDO, false, false, false>(a, block, s);
+    if (a.p_lfc) {
+        if (sur) return launch_warm_s<ICE, PSEUDO, true, true, false, true>(a, block, s);
+        return predict ? launch_warm_s<ICE, PSEUDO, false, true, false, true>(a, block, s)
+                       : launch_warm_s<ICE, PSEUDO, false, false, false, true>(a, block, s);
+    }
+    if (sur) return launch_warm_s<ICE, PSEUDO, true, true, false, false>(a, block, s);
+    return predict ? launch_warm_s<ICE, PSEUDO, false, true, false, false>(a, block, s)
+                   : launch_warm_s<ICE, PSEUDO, false, false, false, false>(a, block, s);
 }
 
 inline hipError_t launch_warm(const CapeGridArgs &a, int block, hipStream_t s)

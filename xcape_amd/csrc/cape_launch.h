@@ -17,11 +17,16 @@ struct CapeGridArgs {
     const float *t, *td;   // (ncol, nk_in)
     const float *ps, *ts, *tds;  // (ncol,)
     int64_t nk_in, ncol;
+    // ragged batches staged as a grid (the warm-started kernels only): each
+    // column's level count (<= nk_in) and source, and the output slot of each
+    // column (its caller's order); nullptr: uniform / identity
+    const int32_t *nk_col, *src_col, *out_index;
     int64_t ls;                  // level stride of the (nk_in, ncol) arrays: 1 or ncol
     float pinc, ml_depth;
     int source, adiabat;
     void *cape, *cin, *zout;     // float or double (ncol,)
     int32_t *mulvl;              // (ncol,)
+    float *p_lfc, *p_el;         // (ncol,) LFC and EL [hPa], or nullptr (float only)
     unsigned long long *stats;   // ST_COUNT counters or nullptr
     // storm-relative helicity in the same pass (fused kernel) or alone
     const float *u, *v;          // (ncol, nk_in)

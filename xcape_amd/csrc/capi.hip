@@ -266,6 +266,7 @@ hipError_t run_grid(const CapeGridArgs &a0, bool cape, bool srh, int math, hipSt
             a.cin = static_cast<char *>(a.cin) + c0 * esz;
             a.zout = static_cast<char *>(a.zout) + c0 * esz;
             a.mulvl += c0;
+            if (a.p_lfc) { a.p_lfc += c0; a.p_el += c0; }
         }
         if (srh) {
             a.us += c0; a.vs += c0;
@@ -296,6 +297,7 @@ struct GridCall {
     int source = 1, adiabat = 1;
     void *cape = nullptr, *cin = nullptr, *zout = nullptr;
     int32_t *mulev = nullptr;
+    float *p_lfc = nullptr, *p_el = nullptr;   // LFC/EL outputs (float CAPE calls only)
     double *srh_rm = nullptr, *srh_lm = nullptr;
     void *rm = nullptr, *lm = nullptr, *mean6 = nullptr;  // float, or double when fp64
     uint32_t flags = 0;
@@ -315,6 +317,8 @@ int grid_entry(const GridCall &c)
             return fail(XCAPE_EINVAL, "NULL input array");
         if (want_cape && (!c.cape || !c.cin || !c.mulev || !c.zout))
             return fail(XCAPE_EINVAL, "NULL CAPE output array");
+        if ((c.p_lfc != nullptr) != (c.p_el != nullptr))
+            return fail(XCAPE_EINVAL, "p_lfc and p_el must be both NULL or both set");
         if (want_srh && (!c.u || !c.v || !c.us || !c.vs || !c.srh_rm || !c.srh_lm))
             return fail(XCAPE_EINVAL, "NULL helicity array");
         if (want_srh && ((c.rm != nullptr) != (c.lm != nullptr) ||
@@ -348,6 +352,7 @@ int grid_entry(const GridCall &c)
     DevBuf<float> dp_, dt_, dtd_, dps_, dts_, dtds_, du_, dv_, dus_, dvs_;
     DevBuf<int32_t> dst_, dmul_;
     DevBuf<unsigned char> dcape_, dcin_, dz_, drm_, dlm_, dmw_;
+    DevBuf<float> dlfc_, del_;
     DevBuf<double> dsrm_, dslm_;
     DevBuf<unsigned long long> dstats_;
     CapeGridArgs a{};
@@ -369,6 +374,7 @@ int grid_entry(const GridCall &c)
         a.start = c.p_shared ? c.start : nullptr;
         a.t = c.t; a.td = c.td; a.ps = c.ps; a.ts = c.ts; a.tds = c.tds;
         a.cape = c.cape; a.cin = c.cin; a.zout = c.zout; a.mulvl = c.mulev;
+        a.p_lfc = c.p_lfc; a.p_el = c.p_el;
         a.u = c.u; a.v = c.v; a.us = c.us; a.vs = c.vs;
         a.srh_rm = c.srh_rm; a.srh_lm = c.srh_lm; a.rm = c.rm; a.lm = c.lm; a.mean6 = c.mean6;
     } else {
@@ -390,6 +396,10 @@ int grid_entry(const GridCall &c)
             XC_TRY(dcape_.alloc(ncol * osz, s)); XC_TRY(dcin_.alloc(ncol * osz, s));
             XC_TRY(dz_.alloc(ncol * osz, s)); XC_TRY(dmul_.alloc(ncol, s));
             a.cape = dcape_.ptr; a.cin = dcin_.ptr; a.zout = dz_.ptr; a.mulvl = dmul_.ptr;
+            if (c.p_lfc) {
+                XC_TRY(dlfc_.alloc(ncol, s)); XC_TRY(del_.alloc(ncol, s));
+                a.p_lfc = dlfc_.ptr; a.p_el = del_.ptr;
+            }
         }
         if (want_srh) {
             XC_TRY(du_.alloc(n2, s)); XC_TRY(du_.upload(c.u, n2, s));
@@ -420,6 +430,10 @@ int grid_entry(const GridCall &c)
             XC_TRY(hipMemcpyAsync(c.cin, dcin_.ptr, ncol * osz, hipMemcpyDeviceToHost, s));
             XC_TRY(hipMemcpyAsync(c.zout, dz_.ptr, ncol * osz, hipMemcpyDeviceToHost, s));
             XC_TRY(dmul_.download(c.mulev, ncol, s));
+            if (c.p_lfc) {
+                XC_TRY(dlfc_.download(c.p_lfc, ncol, s));
+                XC_TRY(del_.download(c.p_el, ncol, s));
+            }
         }
         if (want_srh) {
             XC_TRY(dsrm_.download(c.srh_rm, ncol, s));
@@ -473,6 +487,26 @@ int xcape_cape_f32(const float *p, int p_shared, const int32_t *start_lev, const
     c.adiabat = adiabat;
     c.cape = cape; c.cin = cin; c.mulev = mulev; c.zout = zmulev;
     c.flags = flags; c.stats = stats; c.stream = stream;
+    return grid_entry(c);
+}
+
+int xcape_cape_levels_f32(const float *p, int p_shared, const int32_t *start_lev,
+                          const float *t, const float *td, const float *ps, const float *ts,
+                          const float *tds, int64_t nk, int64_t ncol, float pinc, int source,
+                          float ml_depth, int adiabat, float *cape, float *cin, int32_t *mulev,
+                          float *zmulev, float *p_lfc, float *p_el, uint32_t flags,
+                          void *stream)
+{
+    GridCall c;
+    c.kind = GRID_CAPE;
+    c.p = p; c.p_shared = p_shared; c.start = start_lev;
+    c.t = t; c.td = td; c.ps = ps; c.ts = ts; c.tds = tds;
+    c.nk = nk; c.ncol = ncol; c.pinc = pinc; c.source = source; c.ml_depth = ml_depth;
+    c.adiabat = adiabat;
+    c.cape = cape; c.cin = cin; c.mulev = mulev; c.zout = zmulev;
+    c.p_lfc = p_lfc; c.p_el = p_el;
+    if (ncol > 0 && (!p_lfc || !p_el)) return fail(XCAPE_EINVAL, "NULL LFC/EL output array");
+    c.flags = flags; c.stream = stream;
     return grid_entry(c);
 }
 

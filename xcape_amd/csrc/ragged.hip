@@ -22,15 +22,19 @@
 //     A wavefront's 64 lanes then read 64 consecutive words per level (the
 //     CSR layout read one column per lane makes every lane fetch its own
 //     cache lines, which the L2 cannot keep for large batches);
-//  3. per segment, a persistent ascent kernel: a grid sized to fill the
-//     device once, every wavefront taking the next `chunk` columns of its
-//     segment from the counter until the segment is drained (every wave
-//     reaches the exit test, so the grid always drains).  Optionally the
-//     first `tile` levels of a wave's columns are copied to LDS first (the
-//     autotuner decides whether that pays).  The segments' launches are
-//     parallel branches of the graph (forked after the staging, joined
-//     before the graph ends): one segment alone is often too small to fill
-//     the device, and a segment's tail then overlaps the others' work.
+//  3. per segment, the ascent, the segments' launches as parallel branches
+//     of the graph (forked after the staging, joined before the graph ends):
+//     one segment alone is often too small to fill the device, and a
+//     segment's tail then overlaps the others' work.
+//     * The reference's method: a persistent ascent kernel, a grid sized to
+//       fill the device once, every wavefront taking the next `chunk`
+//       columns of its segment from the counter until the segment is drained
+//       (every wave reaches the exit test, so the grid always drains).
+//       Optionally the first `tile` levels of a wave's columns are copied to
+//       LDS first (the autotuner decides whether that pays).
+//     * The warm start: the segment is a grid of its own to the grid path's
+//       warm kernels (setup pass, surrogate, persistent per-lane ascent),
+//       with each column's level count, source and output slot.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -156,10 +160,9 @@ __global__ void reset_counters(unsigned int *c)
     if (threadIdx.x < NSEG) c[threadIdx.x] = 0u;
 }
 
-// MODE 0: the reference's loops (cape_column); 1: warm-started
-// (cape_column_warm).  TILED: stage the first a.tile levels in LDS.  STATS:
-// accumulate the work counters.
-template <int MATH, bool ICE, bool PSEUDO, int MODE, bool TILED, bool STATS>
+// The reference's loops (cape_column).  TILED: stage the first a.tile levels
+// in LDS.  STATS: accumulate the work counters.
+template <int MATH, bool ICE, bool PSEUDO, bool TILED, bool STATS>
 __global__ void __launch_bounds__(256) ragged_kernel(RaggedArgs a)
 {
     // LDS: per wave, 3 arrays x tile levels x 64 lanes, level-major so the
@@ -217,19 +220,8 @@ __global__ void __launch_bounds__(256) ragged_kernel(RaggedArgs a)
             ColumnOut<float> o{};
             const int nk = nk_in + 1;
             const int src = a.src_sorted[j];
-            if constexpr (MODE == 0) {
-                cape_column<float, MATH, ICE, PSEUDO>(c, nk, a.pinc, src, a.ml_depth, o, st);
-                a.mulvl[col] = o.mulvl;
-            } else {
-                Parcel<float> pc;
-                const bool ok = parcel_source<float, MATH>(c, nk, src, a.ml_depth, pc);
-                a.mulvl[col] = pc.mulvl;
-                if (ok)
-                    cape_column_warm<float, MATH, ICE, PSEUDO>(c, nk, a.pinc, pc, NAN,
-                                                               a.fallback_tol, o, st, true);
-                else
-                    o.zout = -999999.0f;
-            }
+            cape_column<float, MATH, ICE, PSEUDO>(c, nk, a.pinc, src, a.ml_depth, o, st);
+            a.mulvl[col] = o.mulvl;
             a.cape[col] = o.cape;
             a.cin[col] = o.cin;
             a.zout[col] = o.zout;
@@ -249,32 +241,31 @@ __global__ void __launch_bounds__(256) ragged_kernel(RaggedArgs a)
 
 using Kernel = void (*)(RaggedArgs);
 
-template <int MATH, int MODE, bool TILED, bool STATS>
+template <int MATH, bool TILED, bool STATS>
 Kernel pick_adiabat(int adiabat)
 {
     switch (adiabat) {
-    case 1: return ragged_kernel<MATH, false, true, MODE, TILED, STATS>;
-    case 2: return ragged_kernel<MATH, false, false, MODE, TILED, STATS>;
-    case 3: return ragged_kernel<MATH, true, true, MODE, TILED, STATS>;
-    default: return ragged_kernel<MATH, true, false, MODE, TILED, STATS>;
+    case 1: return ragged_kernel<MATH, false, true, TILED, STATS>;
+    case 2: return ragged_kernel<MATH, false, false, TILED, STATS>;
+    case 3: return ragged_kernel<MATH, true, true, TILED, STATS>;
+    default: return ragged_kernel<MATH, true, false, TILED, STATS>;
     }
 }
 
-template <int MATH, int MODE>
-Kernel pick_mode(int adiabat, bool tiled, bool stats)
+template <int MATH>
+Kernel pick_math(int adiabat, bool tiled, bool stats)
 {
-    if (tiled) return stats ? pick_adiabat<MATH, MODE, true, true>(adiabat)
-                            : pick_adiabat<MATH, MODE, true, false>(adiabat);
-    return stats ? pick_adiabat<MATH, MODE, false, true>(adiabat)
-                 : pick_adiabat<MATH, MODE, false, false>(adiabat);
+    if (tiled) return stats ? pick_adiabat<MATH, true, true>(adiabat)
+                            : pick_adiabat<MATH, true, false>(adiabat);
+    return stats ? pick_adiabat<MATH, false, true>(adiabat)
+                 : pick_adiabat<MATH, false, false>(adiabat);
 }
 
-Kernel pick(bool fast, int adiabat, bool warm, bool tiled, bool stats)
+// the plain method's kernel for one segment
+Kernel pick(bool fast, int adiabat, bool tiled, bool stats)
 {
-    if (fast) return warm ? pick_mode<MATH_FAST, 1>(adiabat, tiled, stats)
-                          : pick_mode<MATH_FAST, 0>(adiabat, tiled, stats);
-    return warm ? pick_mode<MATH_ACCURATE, 1>(adiabat, tiled, stats)
-                : pick_mode<MATH_ACCURATE, 0>(adiabat, tiled, stats);
+    return fast ? pick_math<MATH_FAST>(adiabat, tiled, stats)
+                : pick_math<MATH_ACCURATE>(adiabat, tiled, stats);
 }
 
 }  // namespace
@@ -285,6 +276,9 @@ using namespace xcape;
 struct xcape_ragged_plan {
     StageArgs stage{};
     RaggedArgs args{};                 // common part of every segment's launch
+    CapeGridArgs warm_args[NSEG] = {}; // warm plans: each segment as a grid
+    float *d_warm = nullptr;           // warm plans: the segments' scratch
+    unsigned int *d_work = nullptr;    // warm plans: 8 counters per segment
     int64_t seg_begin[NSEG] = {}, seg_end[NSEG] = {};
     int seg_grid[NSEG] = {};
     int32_t *d_order = nullptr, *d_nk = nullptr, *d_src = nullptr;
@@ -323,15 +317,22 @@ hipError_t enqueue(const xcape_ragged_plan *pl, bool stats, hipStream_t s,
             sg = seg_streams[g];
             if ((e = hipStreamWaitEvent(sg, fork, 0)) != hipSuccess) return e;
         }
-        RaggedArgs a = pl->args;
-        a.seg_begin = pl->seg_begin[g];
-        a.seg_end = pl->seg_end[g];
-        a.counter = pl->d_counter + g;
-        a.stats = stats ? pl->d_stats : nullptr;
-        const Kernel k = pick(pl->fast, g + 1, pl->warm, pl->args.tile > 0, stats);
-        void *kargs[] = {&a};
-        e = hipLaunchKernel(reinterpret_cast<const void *>(k), dim3(pl->seg_grid[g]),
-                            dim3(pl->block), kargs, pl->lds_bytes, sg);
+        if (pl->warm) {
+            CapeGridArgs w = pl->warm_args[g];
+            w.stats = stats ? pl->d_stats : nullptr;
+            e = pl->fast ? launch_cape_grid_fast(w, GRID_CAPE, 256, sg)
+                         : launch_cape_grid_accurate(w, false, GRID_CAPE, 256, sg);
+        } else {
+            RaggedArgs a = pl->args;
+            a.seg_begin = pl->seg_begin[g];
+            a.seg_end = pl->seg_end[g];
+            a.counter = pl->d_counter + g;
+            a.stats = stats ? pl->d_stats : nullptr;
+            const Kernel k = pick(pl->fast, g + 1, pl->args.tile > 0, stats);
+            void *kargs[] = {&a};
+            e = hipLaunchKernel(reinterpret_cast<const void *>(k), dim3(pl->seg_grid[g]),
+                                dim3(pl->block), kargs, pl->lds_bytes, sg);
+        }
         if (e != hipSuccess) return e;
         if (seg_streams && ((e = hipEventRecord(join[g], sg)) != hipSuccess ||
                             (e = hipStreamWaitEvent(s, join[g], 0)) != hipSuccess))
@@ -351,6 +352,8 @@ void destroy(xcape_ragged_plan *pl)
     if (pl->d_stage) (void)hipFree(pl->d_stage);
     if (pl->d_counter) (void)hipFree(pl->d_counter);
     if (pl->d_stats) (void)hipFree(pl->d_stats);
+    if (pl->d_warm) (void)hipFree(pl->d_warm);
+    if (pl->d_work) (void)hipFree(pl->d_work);
     delete pl;
 }
 
@@ -382,7 +385,8 @@ int xcape_ragged_plan_create(const int64_t *offsets_host, const int32_t *source_
         return fail(XCAPE_EINVAL, "tile=%d outside 0..%d", tile, MAX_TILE);
     if (waves_per_block < 1 || waves_per_block > 4)
         return fail(XCAPE_EINVAL, "waves_per_block=%d outside 1..4", waves_per_block);
-    if (flags & ~(uint32_t)(XCAPE_FAST_MATH | XCAPE_DEVICE_PTRS | XCAPE_WARM_START))
+    if (flags & ~(uint32_t)(XCAPE_FAST_MATH | XCAPE_DEVICE_PTRS | XCAPE_WARM_START |
+                            XCAPE_NO_SURROGATE))
         return fail(XCAPE_EINVAL, "unsupported flags 0x%x", flags);
     // validate the layout on the host copies and build the work order
     if (offsets_host[0] != 0) return fail(XCAPE_EINVAL, "offsets[0] must be 0");
@@ -458,6 +462,48 @@ int xcape_ragged_plan_create(const int64_t *offsets_host, const int32_t *source_
     a.ncol = ncol; a.pinc = pinc; a.ml_depth = ml_depth;
     a.cape = cape; a.cin = cin; a.zout = zmulev; a.mulvl = mulev;
     a.chunk = chunk; a.tile = tile; a.fallback_tol = 1.0e-3f;
+    if (pl->warm) {
+        // each segment as a grid of its own for the warm kernels: its columns
+        // start seg_begin into each level row of the staged copies
+        size_t per_seg_floats[NSEG] = {}, total = 0;
+        for (int g = 0; g < NSEG; ++g) {
+            const int64_t n = pl->seg_end[g] - pl->seg_begin[g];
+            per_seg_floats[g] = n > 0 ? (size_t)warm_scratch_floats(maxnk) * (size_t)n : 0;
+            total += per_seg_floats[g];
+        }
+        if ((e = hipMalloc((void **)&pl->d_warm, sizeof(float) * (total ? total : 1))) ||
+            (e = hipMalloc((void **)&pl->d_work, sizeof(unsigned int) * 8 * NSEG))) {
+            destroy(pl);
+            return hip_fail(e, "plan allocation");
+        }
+        size_t off = 0;
+        for (int g = 0; g < NSEG; ++g) {
+            const int64_t b = pl->seg_begin[g], n = pl->seg_end[g] - b;
+            if (n <= 0) continue;
+            // the warm kernels index level environments with 32-bit offsets
+            if ((uint64_t)(maxnk + 1) * (uint64_t)n >= ((uint64_t)1 << 32)) {
+                destroy(pl);
+                return fail(XCAPE_EINVAL, "batch too large for the warm start (split it)");
+            }
+            CapeGridArgs &w = pl->warm_args[g];
+            w.p = sa.lp + b; w.t = sa.lt + b; w.td = sa.ltd + b;
+            w.ps = sa.sps + b; w.ts = sa.sts + b; w.tds = sa.stds + b;
+            w.nk_in = maxnk; w.ncol = n; w.ls = ncol;
+            w.nk_col = pl->d_nk + b; w.src_col = pl->d_src + b; w.out_index = pl->d_order + b;
+            w.pinc = pinc; w.ml_depth = ml_depth; w.source = 1; w.adiabat = g + 1;
+            w.cape = cape; w.cin = cin; w.zout = zmulev; w.mulvl = mulev;
+            w.warm = 1;
+            w.fallback_tol = 1.0e-3f;
+            w.surrogate = (flags & XCAPE_NO_SURROGATE) ? nullptr : surrogate_device();
+            float *ws = pl->d_warm + off;
+            const size_t lv = (size_t)(maxnk + 1) * (size_t)n;
+            w.env = ws;
+            w.par = ws + 4 * lv;
+            w.sur_in = w.par + 16 * (size_t)n;
+            w.work = pl->d_work + 8 * g;
+            off += per_seg_floats[g];
+        }
+    }
     int dev = 0, ncu = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
@@ -467,7 +513,7 @@ int xcape_ragged_plan_create(const int64_t *offsets_host, const int32_t *source_
         int per_cu = 0;
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &per_cu,
-            reinterpret_cast<const void *>(pick(pl->fast, g + 1, pl->warm, tile > 0, false)),
+            reinterpret_cast<const void *>(pick(pl->fast, g + 1, tile > 0, false)),
             pl->block, pl->lds_bytes);
         if (e != hipSuccess || per_cu < 1) per_cu = 1;
         // enough blocks to fill every CU once, never more than the work needs

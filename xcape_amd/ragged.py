@@ -7,6 +7,11 @@ mixed-layer) and adiabat (1-4, as ``calc_cape``).  ``RaggedPlan`` wraps the
 C-ABI plan (work order, launch geometry, captured hipGraph); ``autotune``
 times the plan over wavefront chunk sizes and LDS level-tile depths and keeps
 the fastest, cached per device and batch signature in a JSON file.
+
+Warm-started plans run each adiabat's columns through the grid path's warm
+kernels (a setup pass, then a persistent ascent whose lanes take columns one
+at a time), which have no chunk or tile to tune: their autotune times the one
+plan.
 """
 import ctypes
 import hashlib
@@ -148,6 +153,9 @@ def autotune(batch, pinc=500., ml_depth=500., fast_math=False, chunks=CHUNKS, ti
         return ent['best'], ent['times']
     times = {}
     best, best_ms = None, float('inf')
+    if warm_start:
+        # the warm kernels take neither a chunk nor a tile
+        chunks, tiles, waves = (DEFAULT['chunk'],), (0,), (DEFAULT['waves'],)
     for ch in chunks:
         for tl in tiles:
             for wv in waves:
