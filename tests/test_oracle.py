@@ -412,3 +412,23 @@ def test_lfc_el_invariants(oracle_lib, source, adiabat):
     assert np.all(lfc[pos] <= c['ps'][pos] + 1e-3)
     for x, y in zip(r[:4], oracle_lib.cape(*a, pinc=500., source=source, adiabat=adiabat)):
         np.testing.assert_array_equal(x, y)
+
+
+def test_dewpoint_from_specific_humidity_round_trip(oracle_lib):
+    """The humidity adaptor (xcape_amd.core.dewpoint_from_specific_humidity)
+    inverts the reference's own saturation mixing ratio: getqvs(p, Td(q, p))
+    gives back r = q / (1 - q), over the range of tropospheric humidities."""
+    from xcape_amd.core import dewpoint_from_specific_humidity as td_of
+    qvs = _f32fn(oracle_lib.lib(), 'xo_getqvs_f32', 2)
+    rng = np.random.default_rng(7)
+    p = rng.uniform(100., 1050., 400)
+    q = 10 ** rng.uniform(-5.5, -1.6, 400)
+    td = td_of(q, p)
+    assert td.dtype == np.float64 or td.dtype == np.float32
+    r = q / (1 - q)
+    got = np.array([qvs(100. * pp, 273.15 + tt) for pp, tt in zip(p, td)])
+    np.testing.assert_allclose(got, r, rtol=2e-5)
+    # saturated air at 0 degC: e = 6.112 hPa gives Td = 0
+    eps = 287.04 / 461.5
+    r0 = eps * 6.112 / (850. - 6.112)
+    assert abs(float(td_of(r0 / (1 + r0), 850.))) < 1e-9

@@ -216,6 +216,28 @@ def _calc_cape_numpy(*args, source='surface', ml_depth=500., adiabat='pseudo-liq
     return tuple(_reshape_outputs(cape_2d, cin_2d, *lv, shape=original_shape))
 
 
+def dewpoint_from_specific_humidity(q, p):
+    """Dewpoint [degC] of air with specific humidity ``q`` [kg/kg] at
+    pressure ``p`` [hPa], for inputs that carry humidity instead of dewpoint.
+
+    Not in the reference (its tutorial expects q inputs, doc/tutorial.rst:
+    18-23, but xcape has no converter).  It inverts the saturation formula the
+    ascent itself uses for the environment's mixing ratio (getqvs, Bolton
+    1980, CAPE_CODE_model_lev.f90:570-581): with r = q / (1 - q) the vapour
+    pressure is e = p r / (eps + r), eps = Rd/Rv, and the dewpoint solves
+    e = 6.112 exp(17.67 Td / (Td + 243.5)), so that the kernels recover r from
+    the returned Td.  Broadcasts like numpy; float64 arithmetic, the result in
+    the promoted input dtype (at least float32)."""
+    q = np.asarray(q)
+    p = np.asarray(p)
+    dt = np.result_type(q, p, np.float32)
+    eps = 287.04 / 461.5
+    r = np.asarray(q, np.float64) / (1.0 - np.asarray(q, np.float64))
+    e = np.asarray(p, np.float64) * r / (eps + r)
+    x = np.log(e / 6.112)
+    return (243.5 * x / (17.67 - x)).astype(dt)
+
+
 def calc_srh(*args, **kwargs):
     """Storm-relative helicity with Bunkers storm motion.
 

@@ -1,17 +1,15 @@
 // Rectangular-grid kernels (sigma or shared pressure levels), included by
 // one translation unit per math mode with XC_MATH defined.
 //
-// Mapping: one column per lane, 64 columns per wavefront, blockDim columns
-// per workgroup.  The ascent is compute-bound (one column's ~2000
-// fixed-point iterations against ~400 bytes of profile), so it reads the
-// profiles in whichever layout the caller holds them.  The helicity chain is
-// light arithmetic over many levels: it is given level-major profiles (the
-// C-ABI stages column-contiguous input through transpose_levels first), so
-// that the 64 lanes of a wave read 64 consecutive words.
-//
-// The CAPE, helicity and surrogate kernels are separate launches: each keeps
-// its own register budget (the FP64 helicity chain and the MFMA evaluation
-// would otherwise cap the ascent's occupancy).
+// Every kernel here reads level-major profiles, level k of column j at
+// k * ls + j (the C-ABI stages column-contiguous input through the LDS
+// transpose first), so that the 64 lanes of a wave read 64 consecutive words
+// per level; one column per lane.  The plain method is one launch
+// (cape_grid_kernel); the warm start is a setup pass, the surrogate's
+// evaluation and a persistent ascent (see the warm section below).  The
+// CAPE, helicity and surrogate work are separate launches: each keeps its own
+// register budget (the FP64 helicity chain and the MFMA evaluation would
+// otherwise cap the ascent's occupancy).
 #pragma once
 #include "cape_column.h"
 #include "cape_launch.h"

@@ -119,9 +119,9 @@ int check_cape_params(double pinc, int source, int adiabat)
     return XCAPE_OK;
 }
 
-// Threads per block of the ascent launch.  XCAPE_CAPE_BLOCK (64, 128 or 256)
-// overrides the default, for tuning.
-int cape_block(bool warm)
+// Threads per block of the ascent launches.  XCAPE_CAPE_BLOCK (64, 128 or
+// 256) overrides the default, for tuning.
+int cape_block()
 {
     static const int env = [] {
         const char *v = getenv("XCAPE_CAPE_BLOCK");
@@ -129,7 +129,6 @@ int cape_block(bool warm)
         return (b == 64 || b == 128 || b == 256) ? b : 0;
     }();
     if (env) return env;
-    (void)warm;
     return 256;
 }
 
@@ -173,7 +172,7 @@ hipError_t launch_grid(const CapeGridArgs &a, bool cape, bool srh, int math, hip
         }
     }
     if (cape) {
-        const int block = cape_block(a.warm);
+        const int block = cape_block();
         if (math == 2)
             e = launch_cape_grid_accurate(a, true, GRID_CAPE, block, s);
         else if (math == 1)

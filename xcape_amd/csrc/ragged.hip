@@ -109,7 +109,6 @@ struct RaggedArgs {
     unsigned long long *stats;
     int chunk;                     // columns a wavefront takes per grab
     int tile;                      // levels staged in LDS (0 = none)
-    float fallback_tol;
 };
 
 // Sorted column j of the staged batch: level k >= 1 at [(k-1) * ls + j] of
@@ -461,7 +460,7 @@ int xcape_ragged_plan_create(const int64_t *offsets_host, const int32_t *source_
     a.lp = sa.lp; a.lt = sa.lt; a.ltd = sa.ltd; a.sps = sa.sps; a.sts = sa.sts; a.stds = sa.stds;
     a.ncol = ncol; a.pinc = pinc; a.ml_depth = ml_depth;
     a.cape = cape; a.cin = cin; a.zout = zmulev; a.mulvl = mulev;
-    a.chunk = chunk; a.tile = tile; a.fallback_tol = 1.0e-3f;
+    a.chunk = chunk; a.tile = tile;
     if (pl->warm) {
         // each segment as a grid of its own for the warm kernels: its columns
         // start seg_begin into each level row of the staged copies
