@@ -415,6 +415,39 @@ def main():
         log('[bench] ragged batches')
         rag = ragged_section(bool(args.fast_math), log)
 
+    # the PCIe-inclusive rate: the same combined call on host (numpy) arrays,
+    # as the reference's callers hold them, over the whole grid; the library
+    # streams them through pinned staging, uploads and downloads inside the
+    # timed region.  Its roofline is the host-to-device copy rate from pinned
+    # memory, measured here on a 512 MiB copy.
+    log('[bench] host-array path')
+    host = [x.cpu().numpy() for x in a]
+    torch.cuda.synchronize()
+    kw_h = dict(pinc=PINC, source=SOURCE, adiabat=ADIABAT, fast_math=args.fast_math,
+                depth=DEPTH, warm_start=not args.plain)
+    _hip.cape_srh(*host, **kw_h)
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        _hip.cape_srh(*host, **kw_h)
+    host_ms = (time.perf_counter() - t0) / reps * 1e3
+    in_bytes = int(sum(x.nbytes for x in host))
+    del host
+    pin = torch.empty(1 << 27, dtype=torch.float32, pin_memory=True)
+    dst = torch.empty(1 << 27, dtype=torch.float32, device='cuda')
+    dst.copy_(pin, non_blocking=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        dst.copy_(pin, non_blocking=True)
+    torch.cuda.synchronize()
+    h2d_gbs = 5 * pin.numel() * 4 / (time.perf_counter() - t0) / 1e9
+    del pin, dst
+    host_gbs = in_bytes / (host_ms / 1e3) / 1e9
+    host_path = {'columns': ncol, 'ms_per_call': host_ms, 'columns_per_s': ncol / (host_ms / 1e3),
+                 'input_bytes': in_bytes, 'input_gbs': host_gbs,
+                 'pinned_h2d_gbs': h2d_gbs, 'frac_of_pinned_h2d': host_gbs / h2d_gbs}
+
     cpu = None
     if not args.no_cpu:
         threads = len(os.sched_getaffinity(0))
@@ -465,6 +498,7 @@ def main():
         'surrogate': sur,
         'accuracy': acc,
         'ragged_200k': rag,
+        'host_arrays_pcie_inclusive': host_path,
         'cpu_baseline': cpu,
     }
     print(json.dumps(line))

@@ -84,6 +84,19 @@ const char *xcape_last_error(void);
 int xcape_device_count(void);
 
 /*
+ * Host-array calls (no XCAPE_DEVICE_PTRS, column-contiguous arrays) stream
+ * their columns in chunks of about 128 MB of input (XCAPE_HOST_CHUNK_MB)
+ * through pinned host memory and device buffers that stay allocated for the
+ * next call: at most two such staging sets per process (three slots of
+ * pinned input, pinned output and device buffers each), so at most two
+ * host-array calls stream at once and a third waits for one of them.  This
+ * frees the sets no call is using; it returns how many it freed.  No
+ * counterpart in the reference (whose f2py routines take host arrays only
+ * and hold no memory between calls).
+ */
+int xcape_release_host_staging(void);
+
+/*
  * Column CAPE/CIN, the most-unstable level and the height of the last ascent
  * level.  Replaces the f2py routines loopcape_ml
  * (reference src/xcape/fortran/CAPE_CODE_model_lev.pyf:6-24 /

@@ -56,6 +56,7 @@ SIGNATURES = {
     'xcape_version': (ctypes.c_char_p, []),
     'xcape_last_error': (ctypes.c_char_p, []),
     'xcape_device_count': (_I, []),
+    'xcape_release_host_staging': (_I, []),
     'xcape_cape_f32': (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _I64, _I64, _F, _I, _F, _I,
                             _P, _P, _P, _P, _U32, _P, _P]),
     'xcape_cape_levels_f32': (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _I64, _I64, _F, _I, _F, _I,
@@ -136,6 +137,12 @@ def check(rc):
 
 def device_count():
     return lib().xcape_device_count()
+
+
+def release_host_staging():
+    """Frees the pinned and device staging that host-array calls keep between
+    calls (the sets no call is using); returns how many sets it freed."""
+    return lib().xcape_release_host_staging()
 
 
 def _is_torch(a):

@@ -1,8 +1,9 @@
 // C-ABI entry points of libxcape_hip.so (declared in include/xcape_hip.h).
 //
-// Host-pointer calls stage inputs through stream-ordered device allocations
-// on the calling thread's stream; device-pointer calls launch directly.  No
-// mutable global state besides the per-thread stream and error message.
+// Host-pointer calls stream their columns through pinned staging slots kept
+// between calls (level-major host arrays: stream-ordered device buffers on
+// the calling thread's stream); device-pointer calls launch directly.  Global
+// state: the per-thread streams and error message, and the idle staging sets.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdarg.h>
@@ -10,7 +11,11 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <condition_variable>
+#include <mutex>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "../../include/xcape_hip.h"
 #include "capi_common.h"
@@ -187,7 +192,21 @@ hipError_t launch_grid(const CapeGridArgs &a, bool cape, bool srh, int math, hip
     return e;
 }
 
-hipError_t run_grid(const CapeGridArgs &a0, bool cape, bool srh, int math, hipStream_t s)
+// Scratch floats per column of one run_grid chunk (staged profiles plus the
+// warm-started ascent's records); 0 when the call needs none.
+int64_t grid_scratch_floats(const CapeGridArgs &a0, bool cape, bool srh, int math)
+{
+    const bool stage = a0.ls == 1 && a0.nk_in > 1;
+    const bool warm = cape && a0.warm != 0 && math != 2;
+    const int narr = stage ? 2 + (a0.p != nullptr) + (srh ? 2 : 0) : 0;
+    return (int64_t)narr * a0.nk_in + (warm ? warm_scratch_floats(a0.nk_in) : 0);
+}
+
+// ext/ext_work: caller-owned scratch (grid_scratch_floats per column of the
+// whole call, which holds any chunk's, and 8 counters) used instead of
+// allocating; NULL to allocate.
+hipError_t run_grid(const CapeGridArgs &a0, bool cape, bool srh, int math, hipStream_t s,
+                    float *ext = nullptr, unsigned int *ext_work = nullptr)
 {
     const bool stage = a0.ls == 1 && a0.nk_in > 1;
     const bool warm = cape && a0.warm != 0 && math != 2;
@@ -219,11 +238,18 @@ hipError_t run_grid(const CapeGridArgs &a0, bool cape, bool srh, int math, hipSt
         if (chunk > cap) chunk = cap;
     }
     if (chunk > a0.ncol) chunk = a0.ncol;
-    DevBuf<float> scratch;
-    DevBuf<unsigned int> work;
-    hipError_t e = scratch.alloc((size_t)(narr * nk + wfl) * (size_t)chunk, s);
-    if (e != hipSuccess) return e;
-    if (warm && (e = work.alloc(8, s)) != hipSuccess) return e;   // WORK_PARTS counters
+    DevBuf<float> scratch_;
+    DevBuf<unsigned int> work_;
+    float *scratch = ext;
+    unsigned int *work = ext_work;
+    hipError_t e = hipSuccess;
+    if (!ext) {
+        if ((e = scratch_.alloc((size_t)(narr * nk + wfl) * (size_t)chunk, s)) != hipSuccess)
+            return e;
+        if (warm && (e = work_.alloc(8, s)) != hipSuccess) return e;   // WORK_PARTS counters
+        scratch = scratch_.ptr;
+        work = work_.ptr;
+    }
     const int64_t esz = math == 2 ? 8 : 4;
     for (int64_t c0 = 0; c0 < a0.ncol; c0 += chunk) {
         const int64_t n = chunk < a0.ncol - c0 ? chunk : a0.ncol - c0;
@@ -235,7 +261,7 @@ hipError_t run_grid(const CapeGridArgs &a0, bool cape, bool srh, int math, hipSt
             float *dst[5];
             for (int i = 0; i < narr; ++i) {
                 src[i] = src0[i] + o;
-                dst[i] = scratch.ptr + (size_t)i * (size_t)nk * (size_t)n;
+                dst[i] = scratch + (size_t)i * (size_t)nk * (size_t)n;
             }
             if ((e = launch_transpose_levels(src, dst, narr, n, nk, s)) != hipSuccess) return e;
             a.ls = n;
@@ -253,12 +279,12 @@ hipError_t run_grid(const CapeGridArgs &a0, bool cape, bool srh, int math, hipSt
         a.ps += c0; a.ts += c0; a.tds += c0;
         if (a.start) a.start += c0;
         if (warm) {
-            float *w = scratch.ptr + (size_t)narr * (size_t)nk * (size_t)n;
+            float *w = scratch + (size_t)narr * (size_t)nk * (size_t)n;
             const size_t lv = (size_t)(nk + 1) * (size_t)n;
             a.env = w;
             a.par = w + 4 * lv;
             a.sur_in = a.par + 16 * (size_t)n;
-            a.work = work.ptr;
+            a.work = work;
         }
         if (cape) {
             a.cape = static_cast<char *>(a.cape) + c0 * esz;
@@ -303,6 +329,394 @@ struct GridCall {
     xcape_stats *stats = nullptr;
     void *stream = nullptr;
 };
+
+// ---------------------------------------------------------------------------
+// Host arrays in the reference's layout (each column contiguous, as Fortran's
+// p3d(nk, n2)) stream through pinned staging.  Pageable host memory reaches
+// the device at about 25 GB/s, pinned memory at about 57 GB/s (PCIe 5 x16 on
+// MI355X), and the uploads, not the kernels, set the pace of a host-array
+// call.  The call's columns go in chunks:
+//   * host threads copy a chunk's inputs into one of STAGE_IN pinned input
+//     buffers, free again as soon as its last upload has ended;
+//   * one upload stream carries every chunk's inputs, one after the other, so
+//     the copy engine runs at its single-stream rate without gaps;
+//   * STAGE_SLOTS compute slots (device buffers, a stream and a pinned output
+//     area each) run the chunk's kernels once its upload has ended, download
+//     its outputs, and are drained into the caller's arrays when the calling
+//     thread comes back to the slot.
+// The first chunks are a quarter and a half of the others, so the first
+// upload starts early.  A chunk of an array is one contiguous range of the
+// caller's array and one contiguous run of the staging areas: a chunk moves
+// in one upload and one download.
+
+constexpr int STAGE_IN = 3;
+constexpr int STAGE_SLOTS = 4;
+constexpr int STAGE_SETS = 2;   // at most this many calls stream at once; others wait
+constexpr size_t STAGE_ALIGN = 256;
+
+size_t align_up(size_t x) { return (x + STAGE_ALIGN - 1) / STAGE_ALIGN * STAGE_ALIGN; }
+
+// The per-column arrays of one direction (inputs or outputs) of a call, and
+// their places in a slot's area for a chunk of n columns: array after array,
+// each STAGE_ALIGN-aligned.
+struct Slab {
+    int count = 0;
+    const char *src[12];   // inputs: the caller's arrays
+    char *dst[12];         // outputs: the caller's arrays
+    size_t cb[12];         // bytes per column
+    int add(const void *in, void *out, size_t col_bytes)
+    {
+        src[count] = static_cast<const char *>(in);
+        dst[count] = static_cast<char *>(out);
+        cb[count] = col_bytes;
+        return count++;
+    }
+    size_t place(int64_t n, size_t *off) const
+    {
+        size_t at = 0;
+        for (int i = 0; i < count; ++i) {
+            off[i] = at;
+            at = align_up(at + (size_t)n * cb[i]);
+        }
+        return at;
+    }
+};
+
+struct CopyJob {
+    char *dst;
+    const char *src;
+    size_t n;
+};
+
+// Copies the jobs' bytes with up to `threads` threads, each taking an equal
+// share of the jobs' concatenation.
+void parallel_copy(const CopyJob *jobs, int njobs, int threads)
+{
+    size_t total = 0;
+    for (int j = 0; j < njobs; ++j) total += jobs[j].n;
+    if (total < ((size_t)4 << 20)) threads = 1;
+    const size_t per = (total + threads - 1) / (size_t)threads;
+    auto share = [jobs, njobs, total, per](int t) {
+        const size_t lo = (size_t)t * per, hi = lo + per < total ? lo + per : total;
+        size_t base = 0;
+        for (int j = 0; j < njobs && base < hi; base += jobs[j].n, ++j) {
+            const size_t b0 = lo > base ? lo : base;
+            const size_t b1 = hi < base + jobs[j].n ? hi : base + jobs[j].n;
+            if (b0 < b1) memcpy(jobs[j].dst + (b0 - base), jobs[j].src + (b0 - base), b1 - b0);
+        }
+    };
+    std::vector<std::thread> pool;
+    try {
+        for (int t = 1; t < threads; ++t) pool.emplace_back(share, t);
+    } catch (...) {
+        // fewer threads than asked: the calling thread takes the other shares
+    }
+    for (int t = 1 + (int)pool.size(); t < threads; ++t) share(t);
+    share(0);
+    for (auto &th : pool) th.join();
+}
+
+int host_threads()
+{
+    static const int n = [] {
+        const char *v = getenv("XCAPE_HOST_THREADS");
+        const int k = v ? atoi(v) : 8;
+        return k < 1 ? 1 : (k > 64 ? 64 : k);
+    }();
+    return n;
+}
+
+// Pinned input buffers, compute slots and streams of one streaming call; kept
+// for the next call (xcape_release_host_staging frees the idle sets).
+struct StagingSet {
+    int device = -1;
+    size_t in_cap = 0, out_cap = 0, dev_cap = 0;
+    char *pin_in[STAGE_IN] = {}, *pin_out[STAGE_SLOTS] = {};
+    char *dev = nullptr;
+    hipStream_t up = nullptr;                 // every upload
+    hipEvent_t uploaded[STAGE_IN] = {};       // a pinned input buffer's last upload
+    hipStream_t stream[STAGE_SLOTS] = {};     // a slot's kernels and download
+    hipEvent_t done[STAGE_SLOTS] = {};        // a slot's last download
+
+    void free_pinned()
+    {
+        for (auto &x : pin_in) {
+            if (x) (void)hipHostFree(x);
+            x = nullptr;
+        }
+        for (auto &x : pin_out) {
+            if (x) (void)hipHostFree(x);
+            x = nullptr;
+        }
+        in_cap = out_cap = 0;
+    }
+    void destroy()
+    {
+        free_pinned();
+        if (dev) (void)hipFree(dev);
+        dev = nullptr;
+        dev_cap = 0;
+        if (up) (void)hipStreamDestroy(up);
+        up = nullptr;
+        for (auto &x : uploaded) {
+            if (x) (void)hipEventDestroy(x);
+            x = nullptr;
+        }
+        for (int i = 0; i < STAGE_SLOTS; ++i) {
+            if (stream[i]) (void)hipStreamDestroy(stream[i]);
+            if (done[i]) (void)hipEventDestroy(done[i]);
+            stream[i] = nullptr;
+            done[i] = nullptr;
+        }
+        device = -1;
+    }
+    // sizes the set for the current device (the caller made it current)
+    hipError_t reserve(int dv, size_t in, size_t out, size_t devb)
+    {
+        hipError_t e;
+        if (device != dv) destroy();
+        device = dv;
+        const unsigned ev = hipEventDisableTiming;
+        if (!up && (e = hipStreamCreateWithFlags(&up, hipStreamNonBlocking))) return e;
+        for (auto &x : uploaded)
+            if (!x && (e = hipEventCreateWithFlags(&x, ev))) return e;
+        for (int i = 0; i < STAGE_SLOTS; ++i) {
+            if (!stream[i] && (e = hipStreamCreateWithFlags(&stream[i], hipStreamNonBlocking)))
+                return e;
+            if (!done[i] && (e = hipEventCreateWithFlags(&done[i], ev))) return e;
+        }
+        if (in > in_cap || out > out_cap) {
+            free_pinned();
+            for (auto &x : pin_in)
+                if ((e = hipHostMalloc((void **)&x, in, hipHostMallocDefault))) return e;
+            for (auto &x : pin_out)
+                if ((e = hipHostMalloc((void **)&x, out, hipHostMallocDefault))) return e;
+            in_cap = in;
+            out_cap = out;
+        }
+        if (devb > dev_cap) {
+            if (dev) (void)hipFree(dev);
+            dev = nullptr;
+            dev_cap = 0;
+            if ((e = hipMalloc((void **)&dev, devb))) return e;
+            dev_cap = devb;
+        }
+        return hipSuccess;
+    }
+};
+
+std::mutex g_stage_mu;
+std::condition_variable g_stage_cv;
+std::vector<StagingSet *> g_stage_idle;
+int g_stage_sets = 0;
+
+StagingSet *acquire_staging(int dv)
+{
+    std::unique_lock<std::mutex> lk(g_stage_mu);
+    for (;;) {
+        for (size_t i = 0; i < g_stage_idle.size(); ++i)
+            if (g_stage_idle[i]->device == dv) {
+                StagingSet *st = g_stage_idle[i];
+                g_stage_idle.erase(g_stage_idle.begin() + (long)i);
+                return st;
+            }
+        if (g_stage_sets < STAGE_SETS) {
+            ++g_stage_sets;
+            return new StagingSet;
+        }
+        if (!g_stage_idle.empty()) {   // another device's set: reserve() rebuilds it
+            StagingSet *st = g_stage_idle.back();
+            g_stage_idle.pop_back();
+            return st;
+        }
+        g_stage_cv.wait(lk);
+    }
+}
+
+void release_staging(StagingSet *st)
+{
+    {
+        std::lock_guard<std::mutex> lk(g_stage_mu);
+        g_stage_idle.push_back(st);
+    }
+    g_stage_cv.notify_one();
+}
+
+// Input megabytes per chunk; XCAPE_HOST_CHUNK_MB overrides the 128 MB default
+// (read on every call).
+int host_chunk_mb()
+{
+    const char *v = getenv("XCAPE_HOST_CHUNK_MB");
+    const int k = v ? atoi(v) : 128;
+    return k < 1 ? 1 : (k > 4096 ? 4096 : k);
+}
+
+// The streamed host-array call: tmpl carries the call's parameters, c its
+// host arrays (column-contiguous).
+int stream_host_grid(const GridCall &c, const CapeGridArgs &tmpl, bool want_cape, bool want_srh,
+                     int math)
+{
+    const int64_t nk = c.nk, ncol = c.ncol;
+    const size_t osz = c.fp64 ? sizeof(double) : sizeof(float);
+    const size_t prof = (size_t)nk * sizeof(float);
+    Slab in, out;
+    const int i_t = in.add(c.t, nullptr, prof), i_td = in.add(c.td, nullptr, prof);
+    const int i_p = c.p_shared ? -1 : in.add(c.p, nullptr, prof);
+    const int i_ps = in.add(c.ps, nullptr, 4), i_ts = in.add(c.ts, nullptr, 4);
+    const int i_tds = in.add(c.tds, nullptr, 4);
+    const int i_st = (c.p_shared && c.start) ? in.add(c.start, nullptr, 4) : -1;
+    int i_u = -1, i_v = -1, i_us = -1, i_vs = -1;
+    if (want_srh) {
+        i_u = in.add(c.u, nullptr, prof);
+        i_v = in.add(c.v, nullptr, prof);
+        i_us = in.add(c.us, nullptr, 4);
+        i_vs = in.add(c.vs, nullptr, 4);
+    }
+    int o_cape = -1, o_cin = -1, o_z = -1, o_mul = -1, o_lfc = -1, o_el = -1;
+    int o_srm = -1, o_slm = -1, o_rm = -1, o_lm = -1, o_mw = -1;
+    if (want_cape) {
+        o_cape = out.add(nullptr, c.cape, osz);
+        o_cin = out.add(nullptr, c.cin, osz);
+        o_z = out.add(nullptr, c.zout, osz);
+        o_mul = out.add(nullptr, c.mulev, 4);
+        if (c.p_lfc) {
+            o_lfc = out.add(nullptr, c.p_lfc, 4);
+            o_el = out.add(nullptr, c.p_el, 4);
+        }
+    }
+    if (want_srh) {
+        o_srm = out.add(nullptr, c.srh_rm, 8);
+        o_slm = out.add(nullptr, c.srh_lm, 8);
+        if (c.rm) {
+            o_rm = out.add(nullptr, c.rm, 2 * osz);
+            o_lm = out.add(nullptr, c.lm, 2 * osz);
+            o_mw = out.add(nullptr, c.mean6, 2 * osz);
+        }
+    }
+    size_t in_col = 0;
+    for (int i = 0; i < in.count; ++i) in_col += in.cb[i];
+    // columns per chunk: about XCAPE_HOST_CHUNK_MB of input, a multiple of 64
+    int64_t m = ((int64_t)host_chunk_mb() << 20) / (int64_t)in_col / 64 * 64;
+    if (m < 64) m = 64;
+    if (m > ncol) m = ncol;
+    size_t off_in[12], off_out[12];
+    const size_t in_bytes = in.place(m, off_in), out_bytes = out.place(m, off_out);
+    CapeGridArgs shape = tmpl;   // the chunks' launches read column-contiguous profiles
+    shape.ls = 1;
+    shape.p = c.p_shared ? nullptr : c.p;
+    const size_t scr_bytes = align_up((size_t)grid_scratch_floats(shape, want_cape, want_srh, math) *
+                                      (size_t)m * sizeof(float));
+    const size_t slot_bytes = in_bytes + out_bytes + scr_bytes + STAGE_ALIGN;   // + work counters
+    const size_t shared = align_up((size_t)nk * sizeof(float)) +
+                          align_up(ST_COUNT * sizeof(unsigned long long));
+    int dv = 0;
+    hipError_t e = hipGetDevice(&dv);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    if (c.stream && (e = hipStreamSynchronize((hipStream_t)c.stream)) != hipSuccess)
+        return hip_fail(e, "hipStreamSynchronize(stream)");
+    StagingSet *st = acquire_staging(dv);
+    struct Finish {   // on every return: the set's streams drained, the set back in the pool
+        StagingSet *st;
+        ~Finish()
+        {
+            if (st->up) (void)hipStreamSynchronize(st->up);
+            for (int i = 0; i < STAGE_SLOTS; ++i)
+                if (st->stream[i]) (void)hipStreamSynchronize(st->stream[i]);
+            release_staging(st);
+        }
+    } finish{st};
+    if ((e = st->reserve(dv, in_bytes, out_bytes, shared + STAGE_SLOTS * slot_bytes)))
+        return hip_fail(e, "pinned staging allocation");
+    char *const p1d = st->dev;
+    unsigned long long *const dstats =
+        reinterpret_cast<unsigned long long *>(st->dev + align_up((size_t)nk * sizeof(float)));
+#define XC_TRY(x) if ((e = (x)) != hipSuccess) return hip_fail(e, #x)
+    if (c.p_shared)
+        XC_TRY(hipMemcpyAsync(p1d, c.p, (size_t)nk * sizeof(float), hipMemcpyHostToDevice, st->up));
+    if (c.stats && want_cape) XC_TRY(zero_counters(dstats, ST_COUNT, st->up));
+    XC_TRY(hipStreamSynchronize(st->up));
+    struct Pending {
+        int64_t c0 = 0, n = 0;
+        bool busy = false;
+    } pend[STAGE_SLOTS];
+    bool in_used[STAGE_IN] = {};
+    CopyJob jobs[12];
+    const int threads = host_threads();
+    int64_t chunk_no = 0;
+    // the outputs of the chunk that slot sl last ran, into the caller's arrays
+    auto drain = [&](int sl) -> hipError_t {
+        const hipError_t r = hipEventSynchronize(st->done[sl]);
+        if (r != hipSuccess) return r;
+        size_t off[12];
+        out.place(pend[sl].n, off);
+        for (int i = 0; i < out.count; ++i)
+            jobs[i] = {out.dst[i] + (size_t)pend[sl].c0 * out.cb[i], st->pin_out[sl] + off[i],
+                       (size_t)pend[sl].n * out.cb[i]};
+        parallel_copy(jobs, out.count, threads);
+        pend[sl].busy = false;
+        return hipSuccess;
+    };
+    for (int64_t c0 = 0, n = 0; c0 < ncol; c0 += n, ++chunk_no) {
+        const int r = (int)(chunk_no % STAGE_IN), sl = (int)(chunk_no % STAGE_SLOTS);
+        // a quarter, a half, then whole chunks
+        n = chunk_no >= 2 ? m : ((m >> (2 - chunk_no)) + 63) / 64 * 64;
+        if (n < 64) n = 64;
+        if (n > m) n = m;
+        if (n > ncol - c0) n = ncol - c0;
+        if (in_used[r]) XC_TRY(hipEventSynchronize(st->uploaded[r]));
+        size_t oi[12], oo[12];
+        const size_t ib = in.place(n, oi), ob = out.place(n, oo);
+        for (int i = 0; i < in.count; ++i)
+            jobs[i] = {st->pin_in[r] + oi[i], in.src[i] + (size_t)c0 * in.cb[i],
+                       (size_t)n * in.cb[i]};
+        parallel_copy(jobs, in.count, threads);
+        // the slot's device buffers are free once its previous chunk is done
+        if (pend[sl].busy) XC_TRY(drain(sl));
+        char *din = st->dev + shared + (size_t)sl * slot_bytes;
+        char *dout = din + in_bytes;
+        float *dscr = reinterpret_cast<float *>(dout + out_bytes);
+        unsigned int *dwork = reinterpret_cast<unsigned int *>(dout + out_bytes + scr_bytes);
+        hipStream_t ss = st->stream[sl];
+        XC_TRY(hipMemcpyAsync(din, st->pin_in[r], ib, hipMemcpyHostToDevice, st->up));
+        XC_TRY(hipEventRecord(st->uploaded[r], st->up));
+        in_used[r] = true;
+        XC_TRY(hipStreamWaitEvent(ss, st->uploaded[r], 0));
+        auto fin = [&](int i) { return i < 0 ? nullptr : reinterpret_cast<const float *>(din + oi[i]); };
+        auto fout = [&](int i) { return i < 0 ? nullptr : static_cast<void *>(dout + oo[i]); };
+        CapeGridArgs a = tmpl;
+        a.ncol = n;
+        a.ls = 1;
+        a.t = fin(i_t); a.td = fin(i_td);
+        a.p = c.p_shared ? nullptr : fin(i_p);
+        a.p1d = c.p_shared ? reinterpret_cast<const float *>(p1d) : nullptr;
+        a.start = reinterpret_cast<const int32_t *>(fin(i_st));
+        a.ps = fin(i_ps); a.ts = fin(i_ts); a.tds = fin(i_tds);
+        a.u = fin(i_u); a.v = fin(i_v); a.us = fin(i_us); a.vs = fin(i_vs);
+        a.cape = fout(o_cape); a.cin = fout(o_cin); a.zout = fout(o_z);
+        a.mulvl = static_cast<int32_t *>(fout(o_mul));
+        a.p_lfc = static_cast<float *>(fout(o_lfc)); a.p_el = static_cast<float *>(fout(o_el));
+        a.srh_rm = static_cast<double *>(fout(o_srm)); a.srh_lm = static_cast<double *>(fout(o_slm));
+        a.rm = fout(o_rm); a.lm = fout(o_lm); a.mean6 = fout(o_mw);
+        a.stats = (c.stats && want_cape) ? dstats : nullptr;
+        if ((e = run_grid(a, want_cape, want_srh, math, ss, dscr, dwork)) != hipSuccess)
+            return hip_fail(e, "kernel launch");
+        XC_TRY(hipMemcpyAsync(st->pin_out[sl], dout, ob, hipMemcpyDeviceToHost, ss));
+        XC_TRY(hipEventRecord(st->done[sl], ss));
+        pend[sl] = {c0, n, true};
+    }
+    // the last chunks' outputs, oldest first
+    for (int64_t k = 0; k < STAGE_SLOTS; ++k) {
+        const int sl = (int)((chunk_no + k) % STAGE_SLOTS);
+        if (pend[sl].busy) XC_TRY(drain(sl));
+    }
+    if (c.stats && want_cape) {
+        unsigned long long h[ST_COUNT];
+        XC_TRY(hipMemcpy(h, dstats, sizeof h, hipMemcpyDeviceToHost));
+        copy_stats(h, c.stats);
+    }
+#undef XC_TRY
+    return XCAPE_OK;
+}
 
 int grid_entry(const GridCall &c)
 {
@@ -367,6 +781,11 @@ int grid_entry(const GridCall &c)
     a.fallback_tol = 1.0e-3f;
     a.surrogate = (a.warm && !(c.flags & XCAPE_NO_SURROGATE)) ? surrogate_device() : nullptr;
     hipError_t e = hipSuccess;
+    const int math = c.fp64 ? 2 : ((c.flags & XCAPE_FAST_MATH) ? 1 : 0);
+    // host arrays in column order stream through pinned staging; level-major
+    // host arrays go whole through stream-ordered buffers
+    if (!dev && !(c.flags & XCAPE_LEVEL_MAJOR))
+        return stream_host_grid(c, a, want_cape, want_srh, math);
     if (dev) {
         a.p = c.p_shared ? nullptr : c.p;
         a.p1d = c.p_shared ? c.p : nullptr;
@@ -420,7 +839,6 @@ int grid_entry(const GridCall &c)
         XC_TRY(zero_counters(dstats_.ptr, ST_COUNT, s));
         a.stats = dstats_.ptr;
     }
-    const int math = c.fp64 ? 2 : ((c.flags & XCAPE_FAST_MATH) ? 1 : 0);
     if ((e = run_grid(a, want_cape, want_srh, math, s)) != hipSuccess)
         return hip_fail(e, "kernel launch");
     if (!dev) {
@@ -464,6 +882,26 @@ extern "C" {
 const char *xcape_version(void) { return "xcape-hip 0.1.0 (gfx950)"; }
 
 const char *xcape_last_error(void) { return xcape::g_err.c_str(); }
+
+int xcape_release_host_staging(void)
+{
+    std::vector<StagingSet *> idle;
+    {
+        std::lock_guard<std::mutex> lk(g_stage_mu);
+        idle.swap(g_stage_idle);
+        g_stage_sets -= (int)idle.size();
+    }
+    g_stage_cv.notify_all();
+    int dv = -1;
+    (void)hipGetDevice(&dv);
+    for (StagingSet *st : idle) {
+        if (st->device >= 0) (void)hipSetDevice(st->device);
+        st->destroy();
+        delete st;
+    }
+    if (dv >= 0) (void)hipSetDevice(dv);
+    return (int)idle.size();
+}
 
 int xcape_device_count(void)
 {
