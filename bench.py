@@ -120,6 +120,20 @@ def cpu_baseline(host, threads, seconds):
                       f'on {threads} threads'}
 
 
+def full_grid_agreement(outs, ref):
+    """CAPE, CIN, MUlev of every column against a reference run on the GPU,
+    with the tolerance max(0.5 J/kg, 0.1 %)."""
+    res = {'columns': int(ref[0].numel())}
+    for i, name in enumerate(('cape', 'cin')):
+        g, r = outs[i].double(), ref[i].double()
+        d = (g - r).abs()
+        tol = (1e-3 * r.abs()).clamp(min=0.5)
+        res[name] = {'max_abs_diff': float(d.max()),
+                     'frac_within_tol': float((d <= tol).double().mean())}
+    res['mulev_frac_equal'] = float((outs[2] == ref[2]).double().mean())
+    return res
+
+
 def accuracy(outs, inp, sample, fp64_outs):
     """Outputs of the first `sample` columns against the FP32 oracle (CAPE,
     CIN, MUlev, zMUlev, SRH, Bunkers) and against the FP64 re-run of the GPU
@@ -253,6 +267,46 @@ def ragged_section(fast_math, log, ncol=200_000):
     return res
 
 
+def host_array_section(a, ncol, args, log):
+    """The PCIe-inclusive rate: the combined call on host (numpy) copies of
+    the device inputs `a`, as the reference's callers hold them, over the
+    whole grid.  The library streams them through pinned staging; uploads and
+    downloads are inside the timed region.  Its roofline is the host-to-device
+    copy rate from pinned memory, measured here on a 512 MiB copy."""
+    import torch
+
+    from xcape_amd import _hip
+
+    log('[bench] host-array path')
+    host = [x.cpu().numpy() for x in a]
+    torch.cuda.synchronize()
+    kw_h = dict(pinc=PINC, source=SOURCE, adiabat=ADIABAT, fast_math=args.fast_math,
+                depth=DEPTH, warm_start=not args.plain)
+    _hip.cape_srh(*host, **kw_h)
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        _hip.cape_srh(*host, **kw_h)
+    host_ms = (time.perf_counter() - t0) / reps * 1e3
+    in_bytes = int(sum(x.nbytes for x in host))
+    del host
+    pin = torch.empty(1 << 27, dtype=torch.float32, pin_memory=True)
+    dst = torch.empty(1 << 27, dtype=torch.float32, device='cuda')
+    dst.copy_(pin, non_blocking=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        dst.copy_(pin, non_blocking=True)
+    torch.cuda.synchronize()
+    h2d_gbs = 5 * pin.numel() * 4 / (time.perf_counter() - t0) / 1e9
+    del pin, dst
+    host_gbs = in_bytes / (host_ms / 1e3) / 1e9
+    host_path = {'columns': ncol, 'ms_per_call': host_ms, 'columns_per_s': ncol / (host_ms / 1e3),
+                 'input_bytes': in_bytes, 'input_gbs': host_gbs,
+                 'pinned_h2d_gbs': h2d_gbs, 'frac_of_pinned_h2d': host_gbs / h2d_gbs}
+    return host_path
+
+
 def load_surrogate(path, log):
     """Activate the committed surrogate, or train one on the GPU when the
     file is absent.  Returns a description for the JSON line."""
@@ -307,6 +361,8 @@ def main():
     ap.add_argument('--pmc-summary', nargs=2, metavar=('FETCH_DIR', 'WRITE_DIR'),
                     help='summarise two rocprofv3 counter runs of --profile-pass')
     ap.add_argument('--pmc-out', default=PMC_FILE)
+    ap.add_argument('--host-only', action='store_true',
+                    help='only the host-array (PCIe-inclusive) section, for traces')
     args = ap.parse_args()
     if args.pmc_summary:
         pmc_summary(*args.pmc_summary, args.pmc_out, args.ncol, bool(args.fast_math))
@@ -344,6 +400,10 @@ def main():
     kw = dict(pinc=PINC, source=SOURCE, adiabat=ADIABAT, fast_math=args.fast_math)
     stream = torch.cuda.current_stream()
     sur = load_surrogate(args.surrogate, log)
+    if args.host_only:
+        print(json.dumps({'host_arrays_pcie_inclusive': host_array_section(a, ncol, args, log),
+                          'surrogate': sur}))
+        return
 
     def step(warm, stats=False):
         return _hip.cape_srh(*a, depth=DEPTH, warm_start=warm, stats=stats, **kw)
@@ -398,6 +458,11 @@ def main():
     fp64 = _hip.cape(*sl, pinc=PINC, source=SOURCE, adiabat=ADIABAT, fp64=True)
     acc = accuracy(outs, inp, sample, fp64)
     acc_plain = accuracy(outs_p, inp, sample, fp64)
+    # every column of the reported method against the reference method in the
+    # accurate math mode, on the GPU (the oracle is too slow for the full grid)
+    ref_full = _hip.cape(*cape_lm, level_major=True, pinc=PINC, source=SOURCE, adiabat=ADIABAT)
+    acc['full_grid_vs_plain_accurate_gpu'] = full_grid_agreement(outs, ref_full)
+    del ref_full
 
     # the dominant kernel: the ascent (CAPE-only launch of the reported method)
     dom = 'warm' if warm else 'plain'
@@ -415,38 +480,7 @@ def main():
         log('[bench] ragged batches')
         rag = ragged_section(bool(args.fast_math), log)
 
-    # the PCIe-inclusive rate: the same combined call on host (numpy) arrays,
-    # as the reference's callers hold them, over the whole grid; the library
-    # streams them through pinned staging, uploads and downloads inside the
-    # timed region.  Its roofline is the host-to-device copy rate from pinned
-    # memory, measured here on a 512 MiB copy.
-    log('[bench] host-array path')
-    host = [x.cpu().numpy() for x in a]
-    torch.cuda.synchronize()
-    kw_h = dict(pinc=PINC, source=SOURCE, adiabat=ADIABAT, fast_math=args.fast_math,
-                depth=DEPTH, warm_start=not args.plain)
-    _hip.cape_srh(*host, **kw_h)
-    reps = 3
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        _hip.cape_srh(*host, **kw_h)
-    host_ms = (time.perf_counter() - t0) / reps * 1e3
-    in_bytes = int(sum(x.nbytes for x in host))
-    del host
-    pin = torch.empty(1 << 27, dtype=torch.float32, pin_memory=True)
-    dst = torch.empty(1 << 27, dtype=torch.float32, device='cuda')
-    dst.copy_(pin, non_blocking=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(5):
-        dst.copy_(pin, non_blocking=True)
-    torch.cuda.synchronize()
-    h2d_gbs = 5 * pin.numel() * 4 / (time.perf_counter() - t0) / 1e9
-    del pin, dst
-    host_gbs = in_bytes / (host_ms / 1e3) / 1e9
-    host_path = {'columns': ncol, 'ms_per_call': host_ms, 'columns_per_s': ncol / (host_ms / 1e3),
-                 'input_bytes': in_bytes, 'input_gbs': host_gbs,
-                 'pinned_h2d_gbs': h2d_gbs, 'frac_of_pinned_h2d': host_gbs / h2d_gbs}
+    host_path = host_array_section(a, ncol, args, log)
 
     cpu = None
     if not args.no_cpu:

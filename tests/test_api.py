@@ -149,3 +149,18 @@ def test_levels_keyword_appends_two_outputs(p_t_td_3d, p_t_td_surface):
     res = core.calc_cape(p, t, td, ps, ts, tds, source='surface', vertical_lev='sigma',
                          method='dummy', levels=True)
     assert len(res) == 4
+
+
+def test_memory_mapped_profiles_reach_the_library_uncopied(tmp_path):
+    """Out-of-core grids: a memory-mapped float32 (X.., Z) array reaches the
+    C-ABI as a view of the mapping (reshaped, never copied into memory); the
+    library's host threads read it chunk by chunk as they stage it."""
+    m = np.lib.format.open_memmap(tmp_path / 't.npy', mode='w+', dtype=np.float32,
+                                  shape=(30, 40, 50))
+    m[:] = 1.0
+    m.flush()
+    r = np.load(tmp_path / 't.npy', mmap_mode='r')
+    (flat,) = core._reshape_inputs(r)
+    passed = _hip._Args([flat]).f32(flat)
+    assert passed.shape == (1200, 50) and passed.flags['C_CONTIGUOUS']
+    assert np.shares_memory(passed, r)

@@ -150,3 +150,27 @@ def test_release_host_staging(hip):
     again = hip.cape(*args, source=2, adiabat=3)
     for a, b in zip(first, again):
         np.testing.assert_array_equal(a, b)
+
+
+def test_out_of_core_memmap(hip, tmp_path):
+    """Grids larger than host memory: the profiles as memory-mapped files go
+    through calc_cape as they are (no copy into memory first); the host
+    threads read each chunk from the mapping as they stage it."""
+    from xcape_amd.core import calc_cape
+    c = synthetic.sigma_columns(6000, NK, 12)
+    mm = {}
+    for k in ('p', 't', 'td'):
+        m = np.lib.format.open_memmap(tmp_path / f'{k}.npy', mode='w+', dtype=np.float32,
+                                      shape=(60, 100, NK))
+        m[:] = c[k].reshape(60, 100, NK)
+        m.flush()
+        mm[k] = np.load(tmp_path / f'{k}.npy', mmap_mode='r')
+    sfc = [c[k].reshape(60, 100) for k in ('ps', 'ts', 'tds')]
+    kw = dict(source='most-unstable', adiabat='pseudo-ice', pinc=500., warm_start=True,
+              vertical_lev='sigma')
+    got = calc_cape(mm['p'], mm['t'], mm['td'], *sfc, **kw)
+    ref = calc_cape(*(np.array(mm[k]) for k in ('p', 't', 'td')), *sfc, **kw)
+    assert len(got) == 4
+    for a, b in zip(got, ref):
+        assert a.shape == (60, 100)
+        np.testing.assert_array_equal(a, b)
