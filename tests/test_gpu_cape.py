@@ -377,3 +377,35 @@ def test_lfc_el_pressure_grid_and_small(hip, oracle_lib):
     np.testing.assert_array_equal(np.isfinite(w[4]), w[0] > 0)
     pos = (w[0] > 0) & (r[0] > 0)
     assert np.all(np.abs(w[4][pos] - r[4][pos]) <= 1.0)
+
+
+def test_cin_disagreements_only_where_cape_vanishes(hip):
+    """On the 2048x1024x137 grid, every variant's CAPE agrees with the plain
+    accurate kernel within max(0.5 J/kg, 0.1 %) everywhere.  CIN may not
+    agree only where the positive area is a sliver: the reference commits
+    the accumulated CIN on the first entry into positive buoyancy
+    (CAPE_CODE_model_lev.f90:509-523), so a layer whose buoyancy is within
+    rounding of zero commits it under one rounding and not under another.
+    Such columns have CAPE ~ 0 in every variant, and they are rare."""
+    import torch
+    ncol = 2048 * 1024
+    inp = synthetic.sigma_columns_torch(ncol, 137, 4004, device='cuda')
+    lm = [inp[k].t().contiguous() for k in ('p', 't', 'td')] + \
+         [inp[k] for k in ('ps', 'ts', 'tds')]
+    del inp
+    kw = dict(pinc=500., source=2, adiabat=3, level_major=True)
+    ref = hip.cape(*lm, **kw)
+    for extra in (dict(warm_start=True), dict(fast_math=True),
+                  dict(fast_math=True, warm_start=True)):
+        got = hip.cape(*lm, **kw, **extra)
+        tol = [(1e-3 * r.double().abs()).clamp(min=0.5) for r in ref[:2]]
+        dcape = (got[0].double() - ref[0].double()).abs()
+        dcin = (got[1].double() - ref[1].double()).abs()
+        assert bool((dcape <= tol[0]).all()), extra
+        off = dcin > tol[1]
+        n_off = int(off.sum())
+        print(extra, 'CIN outside tolerance:', n_off, flush=True)
+        assert n_off <= ncol // 10000, extra
+        if n_off:
+            assert float(torch.maximum(got[0][off], ref[0][off]).max()) < 0.05, extra
+    torch.cuda.synchronize()

@@ -487,8 +487,12 @@ XC_HD void cape_column(const C &col, int nk, T pinc, int source, T ml_depth,
 // Warm-started driver.
 //
 // The fixed point of each sub-step is the reference's (same evaluation, same
-// relaxation 0.3, same 2e-4 K test, same 100-iteration limit); only the first
-// iterate differs.  It is predicted from the sub-steps already converged in
+// 2e-4 K test, same 100-iteration limit, relaxation 0.3 from the third
+// iterate on); the first two iterates differ.  When the predicted first
+// iterate fails the test, the second is the secant step
+// x + (f(x) - x) / (1 - f') with the map's slope f' measured on the
+// sub-steps before (most of those sub-steps then pass at the second).  The
+// first iterate is predicted from the sub-steps already converged in
 // this column: the rate r = (x* - th1) / ln(p2/p1) of the last two sub-steps
 // extrapolated linearly in ln p to the midpoint of the new sub-step (second
 // order; first order with one sub-step of history).  x* is the secant
@@ -531,20 +535,22 @@ template <class C> struct HasEnv<C, decltype((void)C::has_env)> {
 
 // Shape of d(theta)/d(ln p) along a saturated pseudo-adiabat at (t, p, th),
 // up to a constant factor: th qs (a1 - t) / (t^2 + a2 qs), a1 = L kappa / Rv,
-// a2 = L^2 / (cp Rv), with qs and L of liquid water, or of ice (ICE_PHASE).
+// a2 = L^2 / (cp Rv), with qs and L of liquid water, or of ice (`ice`).
 // From dtheta / theta = -L dqs / (cp t) with Clausius-Clapeyron for dqs and
 // t = theta pi.  The predictor uses only ratios of it, to scale the last
 // measured rate from one sub-step to the next; its arithmetic is the
-// predictor's (hardware exp and reciprocal on the device).
-template <typename T, bool ICE_PHASE>
-XC_HD __forceinline__ T adiabat_shape(T t, T p, T th)
+// predictor's (hardware exp and reciprocal on the device).  One code path
+// for both phases, the constants selected per lane.
+template <typename T>
+XC_HD __forceinline__ T adiabat_shape(T t, T p, T th, bool ice)
 {
     using K = Phys<T>;
-    constexpr T a = ICE_PHASE ? T(21.8745584) : T(17.67);
-    constexpr T b = ICE_PHASE ? T(7.66) : T(29.65);
-    constexpr T lat = ICE_PHASE ? K::xls : K::xlv;
-    constexpr T a1 = lat * K::rddcp * K::cp / K::rv;     // L kappa / Rv
-    constexpr T a2 = lat * lat / (K::cp * K::rv);        // L^2 / (cp Rv)
+    constexpr T a1l = K::xlv * K::rddcp * K::cp / K::rv, a2l = K::xlv * K::xlv / (K::cp * K::rv);
+    constexpr T a1i = K::xls * K::rddcp * K::cp / K::rv, a2i = K::xls * K::xls / (K::cp * K::rv);
+    const T a = ice ? T(21.8745584) : T(17.67);
+    const T b = ice ? T(7.66) : T(29.65);
+    const T a1 = ice ? a1i : a1l;     // L kappa / Rv
+    const T a2 = ice ? a2i : a2l;     // L^2 / (cp Rv)
     const T es = T(611.2) * pred_exp(pred_div(a * (t - T(273.15)), t - b));
     const T qs = K::eps * pred_div(es, p - es);
     return pred_div(th * qs * (a1 - t), t * t + a2 * qs);
@@ -724,7 +730,12 @@ struct WarmAscent {
                 it = 0;
                 st.fallbacks++;
             } else if (ad > K::converge) {
-                thlast = x + T(0.3) * (th2 - x);
+                // the reference's relaxed step; a warm-started sub-step whose
+                // first iterate missed takes the secant step instead, with the
+                // map's slope f' from the sub-steps before
+                T nxt = x + T(0.3) * (th2 - x);
+                if (warm && it == 1) nxt = x + pdiv(th2 - x, T(1.0) - fpr);
+                thlast = nxt;
                 x_prev = x;
                 f_prev = th2;
             } else {
@@ -749,19 +760,13 @@ struct WarmAscent {
             // the rate's shape at the next sub-step's start, for the
             // pseudo-adiabats (the shape assumes the condensate falls out),
             // where the condensate is all liquid (above 0 degC; everywhere
-            // for the liquid adiabat) and, in the fast math mode, where it is
-            // all ice (below -40 degC; in the accurate mode the second set of
-            // constants would cost the ascent kernel its eighth wave per
-            // SIMD); none in the ice adiabat's mixed range
+            // for the liquid adiabat) or all ice (below -40 degC); none in
+            // the ice adiabat's mixed range
             const bool sat_now = qv2 < qt;
             T ge = T(NAN);
             if (PSEUDO && predict) {
-                constexpr bool ICE_SHAPE = ICE && MATH == MATH_FAST;
-                const bool ice = ICE_SHAPE && t2 < T(233.15);
-                if (ice)
-                    ge = adiabat_shape<T, true>(t2, p2, th2);
-                else if (!ICE || t2 > T(273.15))
-                    ge = adiabat_shape<T, false>(t2, p2, th2);
+                const bool ice = ICE && t2 < T(233.15);
+                if (ice || !ICE || t2 > T(273.15)) ge = adiabat_shape<T>(t2, p2, th2, ice);
             }
             if (logp != T(0.0)) {
                 const T xs = x + pdiv(th2 - x, T(1.0) - fpr);
