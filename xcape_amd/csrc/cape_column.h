@@ -545,8 +545,8 @@ template <typename T>
 XC_HD __forceinline__ T adiabat_shape(T t, T p, T th, bool ice)
 {
     using K = Phys<T>;
-    constexpr T a1l = K::xlv * K::rddcp * K::cp / K::rv, a2l = K::xlv * K::xlv / (K::cp * K::rv);
-    constexpr T a1i = K::xls * K::rddcp * K::cp / K::rv, a2i = K::xls * K::xls / (K::cp * K::rv);
+    constexpr T a1l = K::xlv * K::rddcp / K::rv, a2l = K::xlv * K::xlv / (K::cp * K::rv);
+    constexpr T a1i = K::xls * K::rddcp / K::rv, a2i = K::xls * K::xls / (K::cp * K::rv);
     const T a = ice ? T(21.8745584) : T(17.67);
     const T b = ice ? T(7.66) : T(29.65);
     const T a1 = ice ? a1i : a1l;     // L kappa / Rv
