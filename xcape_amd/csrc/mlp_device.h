@@ -59,14 +59,16 @@ __host__ __device__ __forceinline__ float bf16_round(float f)
     return r;
 }
 
-// tanh-form GELU and its derivative, FP32: 0.5 z (1 + tanh(c (z + a z^3)))
+// tanh-form GELU and its derivative, FP32: 0.5 z (1 + tanh(c (z + a z^3))).
+// tanh(u) = 1 - 2 / (1 + exp(2u)), with the hardware exp and reciprocal
+// (about 1 ulp, far below the BF16 rounding of the activations); saturates
+// cleanly for large |u| (the reciprocal of inf is 0).
 __device__ __forceinline__ float gelu_f(float z)
 {
     const float c = 0.7978845608028654f, a = 0.044715f;
     const float u = c * (z + a * z * z * z);
-    // tanh(u) = 1 - 2 / (1 + exp(2u)), saturating cleanly for large |u|
     const float e = __expf(2.0f * u);
-    const float t = 1.0f - 2.0f / (1.0f + e);
+    const float t = 1.0f - 2.0f * __builtin_amdgcn_rcpf(1.0f + e);
     return 0.5f * z * (1.0f + t);
 }
 __device__ __forceinline__ float gelu_grad_f(float z)
@@ -74,7 +76,7 @@ __device__ __forceinline__ float gelu_grad_f(float z)
     const float c = 0.7978845608028654f, a = 0.044715f;
     const float u = c * (z + a * z * z * z);
     const float e = __expf(2.0f * u);
-    const float t = 1.0f - 2.0f / (1.0f + e);
+    const float t = 1.0f - 2.0f * __builtin_amdgcn_rcpf(1.0f + e);
     return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * c * (1.0f + 3.0f * a * z * z);
 }
 
@@ -114,18 +116,26 @@ __device__ __forceinline__ float surrogate_eval_wave(const SurrogateDev *S, floa
     float out0 = 0.0f, out1 = 0.0f;
 #pragma unroll 1
     for (int tn = 0; tn < 2; ++tn) {
-        // ---- layer 1 on the VALU: the lanes of half tn write their rows ----
-        if (half == tn) {
-            uint32_t *row = lds + col * MLP_LDS_ROW_WORDS;
+        // ---- layer 1 on the VALU: the hidden-1 rows of samples 32 tn ..
+        // 32 tn + 31; lane (half, col) computes units 32 half .. 32 half + 31
+        // of sample 32 tn + col, so that every lane works.  The weights of a
+        // unit are wave-uniform loads, selected per half ----
+        const int src = 32 * tn + col;
+        const float v0 = __shfl(u0, src, 64), v1 = __shfl(u1, src, 64), v2 = __shfl(u2, src, 64);
+        uint32_t *row = lds + col * MLP_LDS_ROW_WORDS + 16 * half;
 #pragma unroll 4
-            for (int j = 0; j < MLP_H; j += 2) {
-                const float za = S->b1[j] + S->w1[3 * j] * u0 + S->w1[3 * j + 1] * u1 +
-                                 S->w1[3 * j + 2] * u2;
-                const float zb = S->b1[j + 1] + S->w1[3 * j + 3] * u0 + S->w1[3 * j + 4] * u1 +
-                                 S->w1[3 * j + 5] * u2;
-                row[j >> 1] = (uint32_t)bf16_bits(gelu_f(za)) |
-                              ((uint32_t)bf16_bits(gelu_f(zb)) << 16);
-            }
+        for (int j = 0; j < MLP_H / 2; j += 2) {
+            const int k = j + MLP_H / 2;
+            const float za = half ? S->b1[k] + S->w1[3 * k] * v0 + S->w1[3 * k + 1] * v1 +
+                                        S->w1[3 * k + 2] * v2
+                                  : S->b1[j] + S->w1[3 * j] * v0 + S->w1[3 * j + 1] * v1 +
+                                        S->w1[3 * j + 2] * v2;
+            const float zb = half ? S->b1[k + 1] + S->w1[3 * k + 3] * v0 +
+                                        S->w1[3 * k + 4] * v1 + S->w1[3 * k + 5] * v2
+                                  : S->b1[j + 1] + S->w1[3 * j + 3] * v0 +
+                                        S->w1[3 * j + 4] * v1 + S->w1[3 * j + 5] * v2;
+            row[j >> 1] = (uint32_t)bf16_bits(gelu_f(za)) |
+                          ((uint32_t)bf16_bits(gelu_f(zb)) << 16);
         }
         __syncthreads();
         // ---- layer 2 (MFMA) + layer 3 for the two tiles of these samples ----
