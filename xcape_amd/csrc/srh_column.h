@@ -12,8 +12,9 @@
 //    (Bunkers_model_lev.f90:218-231), which is the last one bottom-up.
 //    Heights fall with k only for a column whose top is below its surface;
 //    that orientation (IP = 1 in the reference) is handled by a second walk.
-//  * pass 2 rebuilds H(k) from the surface up to the first level above
-//    `depth` and sums the helicity terms in the reference's order.
+//  * the same walk sums the helicity terms up to the first level above
+//    `depth` with the storm motion factored out (it is known only once the
+//    walk has ended); it enters the sum at the end.
 //
 // Literal constants of the reference's DOUBLE PRECISION routines are default
 // REAL literals, so their values are those of the nearest float (SPD below).
@@ -68,7 +69,8 @@ __device__ void srh_column(const C &col, int nk, double depth, double hs, SrhOut
 #pragma unroll
     for (int i = 0; i < 12; ++i) { iu[i] = TB(-999999.0); iv[i] = TB(-999999.0); }
 
-    // ---- pass 1: heights and Bunkers brackets (increasing orientation) ----
+    // ---- pass 1: heights and Bunkers brackets (increasing orientation),
+    // and the helicity sum's storm-motion-free part ----
     const double Ps = (double)col.P(0);
     double tv_prev = std_tv((double)col.Tc(0), (double)col.Tdc(0), Ps);
     double P_prev = Ps;
@@ -76,6 +78,15 @@ __device__ void srh_column(const C &col, int nk, double depth, double hs, SrhOut
     TB z_prev = (TB)hs;
     TB u_prev = col.U(0), v_prev = col.V(0);
     const TB zsurf = (TB)hs;
+    // SREH 0..depth (SREH_model_lev.f90:82-114) sums, over the levels up to
+    // the first one above `depth` (its wind interpolated to `depth`),
+    // (u - cu)(v - v') - (v - cv)(u - u') with c the storm motion, known only
+    // after this walk.  The sum splits into s0 = sum u(v - v') - v(u - u')
+    // and the telescoping sums of (v - v') and (u - u'), v_top - v_0 and
+    // u_top - u_0, so c enters at the end and the heights are computed once.
+    // FP64; the split moves the sum by rounding only.
+    double s0 = 0.0, z1prev = hs, ut_prev = (double)col.U(0), vt_prev = (double)col.V(0);
+    bool reached = false;
     for (int k = 1; k < nk; ++k) {
         const double Pk = (double)col.P(k);
         const double tv = std_tv((double)col.Tc(k), (double)col.Tdc(k), Pk);
@@ -90,6 +101,22 @@ __device__ void srh_column(const C &col, int nk, double depth, double hs, SrhOut
                 iu[i] = w1 * u_prev + w2 * uk;
                 iv[i] = w1 * v_prev + w2 * vk;
             }
+        }
+        if (!reached) {
+            double ut = (double)uk, vt = (double)vk;
+            const bool top = H > depth;
+            if (top) {
+                // INTERP1 (SREH_model_lev.f90:123-129): X1 = aglh(ktop) is
+                // lowered by 0.01 (a REAL literal) when it equals aglh(ktop-1)
+                double x1 = H;
+                if (z1prev == x1) x1 = x1 - XC_SPD(0.01);
+                ut = ut + ((ut_prev - ut) * ((depth - x1) / (z1prev - x1)));
+                vt = vt + ((vt_prev - vt) * ((depth - x1) / (z1prev - x1)));
+            }
+            s0 = s0 + ((ut * (vt - vt_prev)) - (vt * (ut - ut_prev)));
+            reached = top;
+            // past the top, (ut_prev, vt_prev) keep its interpolated wind
+            z1prev = H; ut_prev = ut; vt_prev = vt;
         }
         z_prev = z; u_prev = uk; v_prev = vk;
         tv_prev = tv; P_prev = Pk;
@@ -148,39 +175,14 @@ __device__ void srh_column(const C &col, int nk, double depth, double hs, SrhOut
     out.mean_u = mu;
     out.mean_v = mv;
 
-    // ---- pass 2: SREH 0..depth (SREH_model_lev.f90:82-114), FP64 ----
-    const double cu_rm = (double)out.rm_u, cv_rm = (double)out.rm_v;
-    const double cu_lm = (double)out.lm_u, cv_lm = (double)out.lm_v;
-    double srm = 0.0, slm = 0.0;
-    tv_prev = std_tv((double)col.Tc(0), (double)col.Tdc(0), Ps);
-    P_prev = Ps;
-    H = hs;
-    double z1prev = hs;
-    double ut_prev = (double)col.U(0), vt_prev = (double)col.V(0);
-    for (int k = 1; k < nk; ++k) {
-        const double Pk = (double)col.P(k);
-        const double tv = std_tv((double)col.Tc(k), (double)col.Tdc(k), Pk);
-        H = H + std_dh(tv, tv_prev, Pk, P_prev);
-        double ut = (double)col.U(k), vt = (double)col.V(k);
-        const bool top = H > depth;
-        if (top) {
-            // INTERP1 (SREH_model_lev.f90:123-129): X1 = aglh(ktop) is lowered
-            // by 0.01 (a REAL literal) when it equals aglh(ktop-1)
-            double x1 = H;
-            if (z1prev == x1) x1 = x1 - XC_SPD(0.01);
-            ut = ut + ((ut_prev - ut) * ((depth - x1) / (z1prev - x1)));
-            vt = vt + ((vt_prev - vt) * ((depth - x1) / (z1prev - x1)));
-        }
-        const double xr = ((ut - cu_rm) * (vt - vt_prev)) - ((vt - cv_rm) * (ut - ut_prev));
-        const double xl = ((ut - cu_lm) * (vt - vt_prev)) - ((vt - cv_lm) * (ut - ut_prev));
-        srm = srm + xr;
-        slm = slm + xl;
-        if (top) break;
-        z1prev = H; ut_prev = ut; vt_prev = vt;
-        tv_prev = tv; P_prev = Pk;
-    }
+    // ---- SREH: the storm motions enter the sums of pass 1 ----
     // no level above depth: the reference's sum is empty (ktop = 0)
-    if (!(H > depth)) { srm = 0.0; slm = 0.0; }
+    double srm = 0.0, slm = 0.0;
+    if (reached) {
+        const double du = ut_prev - (double)col.U(0), dv = vt_prev - (double)col.V(0);
+        srm = (s0 - (double)out.rm_u * dv) + (double)out.rm_v * du;
+        slm = (s0 - (double)out.lm_u * dv) + (double)out.lm_v * du;
+    }
     out.srh_rm = -srm;
     out.srh_lm = -slm;
 }
