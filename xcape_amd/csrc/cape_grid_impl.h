@@ -249,7 +249,8 @@ struct EnvCol {
 };
 
 template <bool ICE, bool PSEUDO, bool SUR, bool STATS>
-__global__ void __launch_bounds__(256) warm_setup_kernel(CapeGridArgs a)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
+warm_setup_kernel(CapeGridArgs a)
 {
     using T = float;
     const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
