@@ -105,6 +105,11 @@ int xcape_release_host_staging(void);
  * as called by cape_fortran.cape (src/xcape/cape_fortran.py:49-67).
  *
  *   p_shared == 0: p is (ncol, nk) model-level pressure (loopcape_ml);
+ *                  with start_lev != NULL, pressure per column on a
+ *                  pressure grid, each column from its 1-based start level
+ *                  (loopcape_pl, CAPE_CODE_pressure_lev.f90:4-84, which the
+ *                  reference's Python never calls; the helicity entry points
+ *                  reject this combination);
  *   p_shared == 1: p is (nk,) pressure levels shared by all columns
  *                  (loopcape_pl1d); start_lev (ncol,) gives each column's
  *                  1-based first level (the reference's pres_lev_pos); when

@@ -86,7 +86,9 @@ void xo_cape_levels##SUFX(const R *p, const R *p1d, const R *t, const R *td,    
                 s0 = start_lev[i] - 1;                                               \
                 pc = p1d + s0;                                                       \
             } else {                                                                 \
-                pc = p + i * nk_in;                                                  \
+                /* p per column from its start level: pressure_lev.f90:69-75   */ \
+                if (start_lev) s0 = start_lev[i] - 1;                                \
+                pc = p + i * nk_in + s0;                                             \
             }                                                                        \
             if (st) st->columns++;                                                   \
             getcape##SUFX((int)(nk_in - s0), pc, tc + s0, tdc + s0, ps[i], ts[i],    \

@@ -40,7 +40,9 @@ typedef struct xo_stats {
 /* CAPE_CODE_model_lev.f90:4-91 (loopcape_ml).  start_lev == NULL for sigma
  * grids.  For the pressure grid (loopcape_pl1d, CAPE_CODE_pressure_lev.f90:
  * 88-169) pass p1d != NULL (shared nk_in levels) and the 1-based start level
- * per column; p is then ignored. */
+ * per column; p is then ignored.  p per column with start_lev != NULL is
+ * loopcape_pl (CAPE_CODE_pressure_lev.f90:4-84): each column from its start
+ * level of its own p. */
 void xo_cape_f32(const float *p, const float *p1d, const float *t, const float *td,
                  const float *ps, const float *ts, const float *tds,
                  const int32_t *start_lev, int64_t nk_in, int64_t ncol,

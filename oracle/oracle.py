@@ -65,8 +65,9 @@ def cape(p, t, td, ps, ts, tds, pinc=500., source=1, ml_depth=500., adiabat=1,
          p1d=None, start_lev=None, fp64=False, stats=False, levels=False):
     """Column CAPE/CIN/MUlvl/z_out.  ``p`` is (ncol, nk) (ignored when ``p1d``
     gives shared pressure levels, in which case ``start_lev`` is required).
-    With ``levels`` the LFC and EL pressures [hPa] follow z_out (NaN without
-    positive area)."""
+    ``p`` with ``start_lev``: each column from its start level of its own p
+    (loopcape_pl).  With ``levels`` the LFC and EL pressures [hPa] follow
+    z_out (NaN without positive area)."""
     dt = np.float64 if fp64 else np.float32
     t = _c(t, dt)
     ncol, nk = t.shape
@@ -78,6 +79,8 @@ def cape(p, t, td, ps, ts, tds, pinc=500., source=1, ml_depth=500., adiabat=1,
         p = None
     else:
         p = _c(p, dt)
+        if start_lev is not None:
+            start_lev = _c(start_lev, np.int32)
     cape_o = np.zeros(ncol, dt)
     cin_o = np.zeros(ncol, dt)
     mul = np.zeros(ncol, np.int32)

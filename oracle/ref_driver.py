@@ -78,6 +78,27 @@ def loopcape_ml(p, t, td, ps, ts, tds, pinc, source, ml_depth, adiabat):
     return cape, cin, mul, zo
 
 
+def loopcape_pl(p, t, td, ps, ts, tds, pinc, source, ml_depth, adiabat, start_3d):
+    """Reference ``loopcape_pl`` (CAPE_CODE_pressure_lev.f90:4-84): pressure
+    per column, each column from its own 1-based start level.
+
+    p, t, td: (nk, n2) hPa / degC; ps, ts, tds: (n2,); start_3d: (n2,).
+    """
+    p, t, td = _f32(p), _f32(t), _f32(td)
+    ps, ts, tds = _f32(ps), _f32(ts), _f32(tds)
+    st = _i32(start_3d)
+    nk, n2 = t.shape
+    cape = np.zeros(n2, np.float32)
+    cin = np.zeros(n2, np.float32)
+    mul = np.zeros(n2, np.int32)
+    zo = np.zeros(n2, np.float32)
+    _lib('CAPE_CODE_pressure_lev').loopcape_pl_(
+        _p(p), _p(t), _p(td), _p(ps), _p(ts), _p(tds),
+        _rf(pinc), _ri(source), _rf(ml_depth), _ri(adiabat), _p(st),
+        _ri(nk), _ri(n2), _p(cape), _p(cin), _p(mul), _p(zo))
+    return cape, cin, mul, zo
+
+
 def loopcape_pl1d(t, td, p1d, ps, ts, tds, pinc, source, ml_depth, adiabat, start_3d):
     """Reference ``loopcape_pl1d`` (CAPE_CODE_pressure_lev.f90:88-169).
 

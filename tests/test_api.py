@@ -133,6 +133,18 @@ def test_invalid_arguments_rejected_without_device():
     assert call(pinc=0.0) == -1 and b'pinc' in L.xcape_last_error()
     assert call(pinc=float('nan')) == -1
     assert call(nk=0) == -1
+    # per-column p with start levels (loopcape_pl): start levels are checked
+    # against 1..nk, and the helicity chain does not take them
+    st = np.array([1, 4], np.int32)
+    assert L.xcape_cape_f32(P(x), 0, P(st), P(x), P(x), P(s), P(s), P(s), 3, 2, 500., 1,
+                            500., 1, P(o), P(o), P(m), P(o), 0, None, None) == -1
+    assert b'start_lev' in L.xcape_last_error()
+    d = np.zeros(2, np.float64)
+    w = np.zeros(4, np.float32)
+    assert L.xcape_srh_f32(P(x), 0, P(np.ones(2, np.int32)), P(x), P(x), P(x), P(x), P(s),
+                           P(s), P(s), P(s), P(s), 3, 2, 3000., P(d), P(d), P(w), P(w), P(w),
+                           0, None) == -1
+    assert b'CAPE calls only' in L.xcape_last_error()
     assert L.xcape_version().startswith(b'xcape-hip')
 
 

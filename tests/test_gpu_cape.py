@@ -409,3 +409,32 @@ def test_cin_disagreements_only_where_cape_vanishes(hip):
         if n_off:
             assert float(torch.maximum(got[0][off], ref[0][off]).max()) < 0.05, extra
     torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize('warm', [False, True])
+def test_pressure_per_column_with_start_levels(hip, oracle_lib, warm):
+    """loopcape_pl (CAPE_CODE_pressure_lev.f90:4-84): pressure per column on a
+    pressure grid, each column from its own start level.  Against the oracle
+    (pinned bit-exact to the compiled loopcape_pl on the CPU); with the shared
+    levels repeated per column, equal to the shared-levels call."""
+    c = synthetic.pressure_columns(3000, 1013)
+    st = oracle_lib.start_levels(c['p'], c['ps'])
+    rng = np.random.default_rng(11)
+    p2 = (c['p'][None, :] * (1.0 + 0.001 * rng.uniform(-1.0, 1.0, (3000, 1)))).astype(np.float32)
+    kw = dict(pinc=500., source=2, adiabat=3, warm_start=warm)
+    g = hip.cape(p2, c['t'], c['td'], c['ps'], c['ts'], c['tds'], start_lev=st, **kw)
+    o = oracle_lib.cape(p2, c['t'], c['td'], c['ps'], c['ts'], c['tds'], 500., 2, 500., 3,
+                        start_lev=st)
+    assert np.all(np.isfinite(g[0])) and np.all(np.isfinite(g[1]))
+    assert not _report(f'pressure per column warm={warm} cape', g[0], o[0]).any()
+    assert not _report(f'pressure per column warm={warm} cin', g[1], o[1]).any()
+    assert np.mean(g[2] == o[2]) >= 0.999
+    tiled = np.tile(np.asarray(c['p'], np.float32), (3000, 1))
+    a = hip.cape(tiled, c['t'], c['td'], c['ps'], c['ts'], c['tds'], start_lev=st, **kw)
+    b = hip.cape(c['p'], c['t'], c['td'], c['ps'], c['ts'], c['tds'], p_shared=True,
+                 start_lev=st, **kw)
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+    with pytest.raises(RuntimeError):
+        hip.srh(p2, c['t'], c['td'], c['t'], c['td'], c['ps'], c['ts'], c['tds'], c['ts'],
+                c['tds'], start_lev=st)

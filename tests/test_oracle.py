@@ -106,6 +106,33 @@ def test_cape_bitexact_vs_reference_pressure(oracle_lib, source, adiabat):
         assert _bitexact(x, y)
 
 
+@pytest.mark.parametrize('adiabat', [1, 3])
+@pytest.mark.parametrize('source', [1, 2, 3])
+def test_cape_bitexact_vs_reference_pressure_2d(oracle_lib, source, adiabat):
+    """loopcape_pl (CAPE_CODE_pressure_lev.f90:4-84), unreachable from the
+    reference's Python: pressure per column, each column from its own start
+    level.  The columns' pressures differ (the shared levels scaled by up to
+    0.1 % per column)."""
+    R = _ref()
+    c = synthetic.pressure_columns(384, 1002)
+    st = oracle_lib.start_levels(c['p'], c['ps'])
+    rng = np.random.default_rng(7)
+    p2 = (c['p'][None, :] * (1.0 + 0.001 * rng.uniform(-1.0, 1.0, (384, 1)))).astype(np.float32)
+    r = R.loopcape_pl(p2.T, c['t'].T, c['td'].T, c['ps'], c['ts'], c['tds'], 500., source,
+                      500., adiabat, st)
+    o = oracle_lib.cape(p2, c['t'], c['td'], c['ps'], c['ts'], c['tds'], 500., source, 500.,
+                        adiabat, start_lev=st)
+    for x, y in zip(o, r):
+        assert _bitexact(x, y)
+    # the shared levels repeated per column give loopcape_pl1d's results
+    o1 = oracle_lib.cape(None, c['t'], c['td'], c['ps'], c['ts'], c['tds'], 500., source,
+                         500., adiabat, p1d=c['p'], start_lev=st)
+    ot = oracle_lib.cape(np.tile(np.asarray(c['p'], np.float32), (384, 1)), c['t'], c['td'],
+                         c['ps'], c['ts'], c['tds'], 500., source, 500., adiabat, start_lev=st)
+    for x, y in zip(o1, ot):
+        assert _bitexact(x, y)
+
+
 def test_cape_edge_branches_vs_reference(oracle_lib):
     """Branches no fixture reaches: ML with the second level above ml_depth and
     with the whole column inside it, MU with a surface above 500 hPa, the

@@ -219,7 +219,9 @@ def cape(p, t, td, ps, ts, tds, *, pinc=500., source=1, ml_depth=500., adiabat=1
          p_shared=False, start_lev=None, fast_math=False, warm_start=False, fp64=False,
          stats=False, level_major=False, levels=False):
     """Run xcape_cape_f32 (or _f64, or with ``levels`` xcape_cape_levels_f32).
-    t, td: (ncol, nk); p: (ncol, nk) or, with p_shared, (nk,); with
+    t, td: (ncol, nk); p: (ncol, nk) or, with p_shared, (nk,); a
+    ``start_lev`` with (ncol, nk) p starts each column at its own level of
+    its own p (the reference's loopcape_pl); with
     ``level_major`` the profile arrays are (nk, ncol).  Returns cape, cin,
     mulev, zmulev (then p_lfc, p_el [hPa] with ``levels``; then a stats dict
     with ``stats``)."""

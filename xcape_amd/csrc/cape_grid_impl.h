@@ -107,10 +107,19 @@ __device__ __forceinline__ bool column_at(const CapeGridArgs &a, int64_t col, Si
         off += (IDX)s0 * c.ls;
         nk = (int)a.nk_in - s0 + 1;
     } else {
+        if (a.start) {
+            // p per column from its own start level (loopcape_pl,
+            // CAPE_CODE_pressure_lev.f90:69-75), clamped as above
+            int s1 = a.start[col];
+            s1 = s1 < 1 ? 1 : (s1 > (int)a.nk_in ? (int)a.nk_in : s1);
+            off += (IDX)(s1 - 1) * c.ls;
+            nk = (int)a.nk_in - (s1 - 1) + 1;
+        } else {
+            nk = (a.nk_col ? (int)a.nk_col[col] : (int)a.nk_in) + 1;
+        }
         c.p = a.p;
         c.poff = off;
         c.pls = c.ls;
-        nk = (a.nk_col ? (int)a.nk_col[col] : (int)a.nk_in) + 1;
     }
     c.off = off;
     c.t = a.t;

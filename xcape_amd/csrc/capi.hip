@@ -564,7 +564,7 @@ int stream_host_grid(const GridCall &c, const CapeGridArgs &tmpl, bool want_cape
     const int i_p = c.p_shared ? -1 : in.add(c.p, nullptr, prof);
     const int i_ps = in.add(c.ps, nullptr, 4), i_ts = in.add(c.ts, nullptr, 4);
     const int i_tds = in.add(c.tds, nullptr, 4);
-    const int i_st = (c.p_shared && c.start) ? in.add(c.start, nullptr, 4) : -1;
+    const int i_st = c.start ? in.add(c.start, nullptr, 4) : -1;
     int i_u = -1, i_v = -1, i_us = -1, i_vs = -1;
     if (want_srh) {
         i_u = in.add(c.u, nullptr, prof);
@@ -743,7 +743,9 @@ int grid_entry(const GridCall &c)
     if (c.flags & ~known) return fail(XCAPE_EINVAL, "unknown flags 0x%x", c.flags & ~known);
     if (c.fp64 && (c.flags & (XCAPE_FAST_MATH | XCAPE_WARM_START)))
         return fail(XCAPE_EINVAL, "the FP64 entry point supports neither FAST_MATH nor WARM_START");
-    if (!(c.flags & XCAPE_DEVICE_PTRS) && c.p_shared && c.start) {
+    if (want_srh && !c.p_shared && c.start)
+        return fail(XCAPE_EINVAL, "start_lev with per-column p is for CAPE calls only (loopcape_pl)");
+    if (!(c.flags & XCAPE_DEVICE_PTRS) && c.start) {
         for (int64_t i = 0; i < c.ncol; ++i)
             if (c.start[i] < 1 || c.start[i] > c.nk)
                 return fail(XCAPE_EINVAL, "start_lev[%lld]=%d outside 1..%lld", (long long)i,
@@ -789,7 +791,7 @@ int grid_entry(const GridCall &c)
     if (dev) {
         a.p = c.p_shared ? nullptr : c.p;
         a.p1d = c.p_shared ? c.p : nullptr;
-        a.start = c.p_shared ? c.start : nullptr;
+        a.start = c.start;
         a.t = c.t; a.td = c.td; a.ps = c.ps; a.ts = c.ts; a.tds = c.tds;
         a.cape = c.cape; a.cin = c.cin; a.zout = c.zout; a.mulvl = c.mulev;
         a.p_lfc = c.p_lfc; a.p_el = c.p_el;
@@ -803,12 +805,12 @@ int grid_entry(const GridCall &c)
         XC_TRY(dps_.alloc(ncol, s)); XC_TRY(dps_.upload(c.ps, ncol, s));
         XC_TRY(dts_.alloc(ncol, s)); XC_TRY(dts_.upload(c.ts, ncol, s));
         XC_TRY(dtds_.alloc(ncol, s)); XC_TRY(dtds_.upload(c.tds, ncol, s));
-        if (c.p_shared && c.start) {
+        if (c.start) {
             XC_TRY(dst_.alloc(ncol, s)); XC_TRY(dst_.upload(c.start, ncol, s));
         }
         a.p = c.p_shared ? nullptr : dp_.ptr;
         a.p1d = c.p_shared ? dp_.ptr : nullptr;
-        a.start = (c.p_shared && c.start) ? dst_.ptr : nullptr;
+        a.start = c.start ? dst_.ptr : nullptr;
         a.t = dt_.ptr; a.td = dtd_.ptr; a.ps = dps_.ptr; a.ts = dts_.ptr; a.tds = dtds_.ptr;
         if (want_cape) {
             XC_TRY(dcape_.alloc(ncol * osz, s)); XC_TRY(dcin_.alloc(ncol * osz, s));
