@@ -81,6 +81,27 @@ def test_sigma_plain_bitexact_warm_within_tolerance(host_lib, nlev, seed, source
     assert sw['first'] == sp['first']
 
 
+def test_warm_start_iteration_budget(host_lib):
+    """What the warm start buys on the combined workload's recipe (137 sigma
+    levels, most unstable, pseudo-ice), without a surrogate: the reference
+    method takes about 9.4 iterations per sub-step; the predicted first
+    iterate, the secant second iterate and the liquid and ice shapes bring it
+    to about 1.45, with under 2 % of the sub-steps falling back to the cold
+    start.  Bounds with some margin, so that a lost piece fails here."""
+    c = S.sigma_columns(400, 137, 4004)
+    arr = [c[k] for k in ('p', 't', 'td', 'ps', 'ts', 'tds')]
+    _, sp = host_cape(host_lib, *arr, 2, 3, False)
+    warm, sw = host_cape(host_lib, *arr, 2, 3, True)
+    ref = O.cape(*arr, 500., 2, 500., 3)
+    assert cape_ok(warm[0], ref[0]).all() and cape_ok(warm[1], ref[1]).all()
+    per_sub = sw['iterations'] / sw['substeps']
+    print('iterations per sub-step: plain', sp['iterations'] / sp['substeps'], 'warm', per_sub,
+          'fallbacks', sw['fallbacks'] / sw['substeps'], flush=True)
+    assert per_sub < 1.55
+    assert sw['iterations'] < 0.17 * sp['iterations']
+    assert sw['fallbacks'] < 0.02 * sw['substeps']
+
+
 @pytest.mark.parametrize('source', [1, 2, 3])
 def test_pressure_levels_plain_bitexact(host_lib, source):
     c = S.pressure_columns(400, 77)
