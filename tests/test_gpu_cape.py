@@ -438,3 +438,39 @@ def test_pressure_per_column_with_start_levels(hip, oracle_lib, warm):
     with pytest.raises(RuntimeError):
         hip.srh(p2, c['t'], c['td'], c['t'], c['td'], c['ps'], c['ts'], c['tds'], c['ts'],
                 c['tds'], start_lev=st)
+
+
+def test_pressure_grid_full_size(hip, oracle_lib):
+    """The pressure-level configuration at its full size: 1024x512 columns x
+    37 fixed levels, surface parcel, pseudo-liquid, the shared-levels path with
+    start levels from the device's own rule.  Size-independent checks on every
+    column (outputs finite and non-negative, the warm start
+    within tolerance of the plain method for CAPE, CIN only where CAPE
+    vanishes, host arrays streamed equal to device arrays), and a random
+    sample of 2000 columns against the oracle."""
+    import torch
+    n = 1024 * 512
+    c = synthetic.pressure_columns(n, 2024)
+    args = (c['p'], c['t'], c['td'], c['ps'], c['ts'], c['tds'])
+    kw = dict(pinc=500., source=1, adiabat=1, p_shared=True)
+    dev = [torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in args]
+    plain = [x.cpu().numpy() for x in hip.cape(*dev, **kw)]
+    warm = [x.cpu().numpy() for x in hip.cape(*dev, warm_start=True, **kw)]
+    host = hip.cape(*args, **kw)
+    torch.cuda.synchronize()
+    for x, y in zip(host, plain):
+        np.testing.assert_array_equal(x, y)
+    assert np.all(np.isfinite(plain[0])) and np.all(np.isfinite(plain[1]))
+    assert (plain[0] >= 0).all() and (plain[1] >= 0).all()
+    assert cape_ok(warm[0], plain[0]).all()
+    off = ~cape_ok(warm[1], plain[1])
+    print('pressure grid: CIN outside tolerance', int(off.sum()), flush=True)
+    assert off.sum() <= n // 10000
+    assert (np.maximum(warm[0][off], plain[0][off]) < 0.05).all()
+    idx = np.sort(np.random.default_rng(5).choice(n, 2000, replace=False))
+    st = oracle_lib.start_levels(c['p'], c['ps'][idx])
+    o = oracle_lib.cape(None, c['t'][idx], c['td'][idx], c['ps'][idx], c['ts'][idx],
+                        c['tds'][idx], 500., 1, 500., 1, p1d=c['p'], start_lev=st)
+    assert not _report('pressure grid sample cape', plain[0][idx], o[0]).any()
+    assert not _report('pressure grid sample cin', plain[1][idx], o[1]).any()
+    np.testing.assert_array_equal(plain[2][idx], o[2])
