@@ -50,6 +50,13 @@ extern "C" {
                                    start's loop with every sub-step started
                                    cold (no prediction, no surrogate); for
                                    measuring what the prediction itself gains */
+#define XCAPE_INPUT_F64 0x40u  /* helicity entry points only: the profile and
+                                   surface arrays (p, t, td, u, v, ps, ts, tds,
+                                   us, vs) are double.  Heights and helicity
+                                   then use them unrounded, Bunkers their
+                                   float32 roundings, as the reference's f2py
+                                   wrappers pass float64 arrays to its
+                                   DOUBLE PRECISION and REAL routines */
 #define XCAPE_LEVEL_MAJOR 0x10u /* level-major profiles: the (nk, ncol) arrays
                                    (p on a sigma grid, t, td, u, v) hold level
                                    j of column i at [j * ncol + i] instead of
@@ -163,7 +170,8 @@ int xcape_cape_f64(const float *p, int p_shared, const int32_t *start_lev,
  * helicity, FP32 storm motion).
  *
  * Arrays as for xcape_cape_f32, plus u, v (ncol, nk) and us, vs (ncol,) in
- * m/s.  Outputs: srh_rm, srh_lm (ncol,) FP64; rm, lm, mean6 (ncol, 2) as
+ * m/s; with XCAPE_INPUT_F64 the ten input pointers point at double arrays.
+ * Outputs: srh_rm, srh_lm (ncol,) FP64; rm, lm, mean6 (ncol, 2) as
  * (u, v) pairs (the reference's (2, n2) Fortran arrays), all three NULL to
  * skip them.
  */
@@ -271,7 +279,14 @@ int xcape_mlp_adam(float *theta, const float *grad, float *m, float *v, int64_t 
                    float beta1, float beta2, float eps, float weight_decay, void *stream);
 
 /* Make theta (and norm, 8 floats) the surrogate the warm-started kernels
- * use; synchronous.  xcape_surrogate_loaded() says whether one is loaded. */
+ * use; synchronous.  xcape_surrogate_loaded() says whether one is loaded.
+ * Loads are serialised, and each packs into a buffer of its own before it is
+ * published: a warm-started call on another thread uses either the previous
+ * network or the new one, whole (which one is unspecified while a load is in
+ * progress, and iteration counts may differ between them; results stay
+ * within the solver tolerance either way).  Replaced networks stay allocated
+ * until the process ends; a ragged plan keeps the network that was loaded
+ * when it was created. */
 int xcape_surrogate_load(const float *theta, const float *norm, void *stream);
 int xcape_surrogate_loaded(void);
 
@@ -291,6 +306,18 @@ int xcape_surrogate_log_f32(const float *p, const float *t, const float *td,
                             int adiabat, float keep_fraction, uint32_t seed, int first_only,
                             float *samples, int64_t capacity, unsigned long long *count,
                             void *stream);
+
+/*
+ * The accurate mode's device arithmetic, elementwise on device arrays a, b
+ * (n,) -> y (n,), for checking its rounding on the device at hand: fn 0 the
+ * raw IEEE division kernel, 1 the compiler's IEEE division, 2 division by a
+ * per-call constant (its reciprocal rounded once), 3 the exp kernel, 4 the
+ * device library's expf, 5 the pow kernel, 6 the device library's powf.
+ * b is unused (may be NULL) for fn 3 and 4.  Asynchronous on `stream`.
+ * No reference counterpart.
+ */
+int xcape_math_check(int fn, const float *a, const float *b, int64_t n, float *y,
+                     void *stream);
 
 #ifdef __cplusplus
 }

@@ -65,6 +65,50 @@
 #undef L
 #undef LOGREC
 
+/* ---- FP32 with each exp/log/pow result moved by up to one ulp ----
+ * A stand-in for another implementation of the C library's functions (the
+ * GPU's differ from glibc's by up to an ulp): each result is kept, or moved
+ * one ulp up or down, by a hash of its argument(s) and the seed set with
+ * xo_set_perturbation (process-wide; not for concurrent use).  Columns whose
+ * outcome changes between seeds are those the reference decides by its
+ * rounding alone. */
+static uint32_t xo_pert_seed;
+
+void xo_set_perturbation(uint32_t seed) { xo_pert_seed = seed; }
+
+static float pert_f(float r, uint32_t a, uint32_t b, uint32_t tag)
+{
+    uint32_t h = (a * 2654435761u) ^ (b * 2246822519u) ^ (xo_pert_seed * 3266489917u + tag);
+    h ^= h >> 15;
+    h *= 2246822519u;
+    h ^= h >> 13;
+    switch (h % 3u) {
+    case 1: return nextafterf(r, INFINITY);
+    case 2: return nextafterf(r, -INFINITY);
+    default: return r;
+    }
+}
+static uint32_t bits_f(float x) { uint32_t u; memcpy(&u, &x, 4); return u; }
+static float pexpf(float x) { return pert_f(expf(x), bits_f(x), 0u, 1u); }
+static float plogf(float x) { return pert_f(logf(x), bits_f(x), 0u, 2u); }
+static float ppowf(float x, float y) { return pert_f(powf(x, y), bits_f(x), bits_f(y), 3u); }
+
+#define REAL float
+#define SUF _pert32
+#define EXPF pexpf
+#define LOGF plogf
+#define POWF ppowf
+#define L(x) x##f
+#define LOGREC 0
+#include "cape_column.inc"
+#undef REAL
+#undef SUF
+#undef EXPF
+#undef LOGF
+#undef POWF
+#undef L
+#undef LOGREC
+
 float xo_getqvs_f32(float p, float t) { return qvs_f32(p, t); }
 float xo_getqvi_f32(float p, float t) { return qvi_f32(p, t); }
 float xo_getthe_f32(float p, float t, float td, float q) { return the_f32(p, t, td, q); }
@@ -115,6 +159,7 @@ void xo_cape##SUFX(const R *p, const R *p1d, const R *t, const R *td,           
 
 DEFINE_LOOP(_f32, float)
 DEFINE_LOOP(_f64, double)
+DEFINE_LOOP(_pert32, float)
 
 int64_t xo_cape_log_f32(const float *p, const float *t, const float *td,
                         const float *ps, const float *ts, const float *tds,

@@ -73,6 +73,18 @@ void xo_cape_f64(const double *p, const double *p1d, const double *t, const doub
                  double *cape, double *cin, int32_t *mulvl, double *zout,
                  xo_stats *stats);
 
+/* xo_cape_f32 with every exp/log/pow result kept or moved one ulp up or
+ * down, pseudo-randomly per argument and seed (xo_set_perturbation; one
+ * process-wide seed, not for concurrent use): a stand-in for a C library
+ * whose functions differ from this one's by up to an ulp. */
+void xo_set_perturbation(uint32_t seed);
+void xo_cape_pert32(const float *p, const float *p1d, const float *t, const float *td,
+                    const float *ps, const float *ts, const float *tds,
+                    const int32_t *start_lev, int64_t nk_in, int64_t ncol,
+                    float pinc, int source, float ml_depth, int adiabat,
+                    float *cape, float *cin, int32_t *mulvl, float *zout,
+                    xo_stats *stats);
+
 /* Per-sample log of the moist-adiabat solve: one record per sub-step,
  * (p2 [Pa], theta-e of the parcel at the start of the sub-step, qt, th1,
  *  converged th2, iterations).  Returns the number of records written

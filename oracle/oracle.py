@@ -62,12 +62,14 @@ def start_levels(p1d, ps):
 
 
 def cape(p, t, td, ps, ts, tds, pinc=500., source=1, ml_depth=500., adiabat=1,
-         p1d=None, start_lev=None, fp64=False, stats=False, levels=False):
+         p1d=None, start_lev=None, fp64=False, stats=False, levels=False, perturb=None):
     """Column CAPE/CIN/MUlvl/z_out.  ``p`` is (ncol, nk) (ignored when ``p1d``
     gives shared pressure levels, in which case ``start_lev`` is required).
     ``p`` with ``start_lev``: each column from its start level of its own p
     (loopcape_pl).  With ``levels`` the LFC and EL pressures [hPa] follow
-    z_out (NaN without positive area)."""
+    z_out (NaN without positive area).  ``perturb``: a seed; every exp, log
+    and pow result is then kept or moved by one ulp (xo_cape_pert32), as a
+    C library an ulp away from this one would give them (FP32, no levels)."""
     dt = np.float64 if fp64 else np.float32
     t = _c(t, dt)
     ncol, nk = t.shape
@@ -88,13 +90,24 @@ def cape(p, t, td, ps, ts, tds, pinc=500., source=1, ml_depth=500., adiabat=1,
     st = Stats()
     lfc = np.zeros(ncol, dt)
     el = np.zeros(ncol, dt)
-    fn = lib().xo_cape_levels_f64 if fp64 else lib().xo_cape_levels_f32
     R = ctypes.c_double if fp64 else ctypes.c_float
-    fn.restype = None
-    fn(_p(p), _p(p1d), _p(t), _p(td), _p(ps), _p(ts), _p(tds), _p(start_lev),
-       ctypes.c_int64(nk), ctypes.c_int64(ncol), R(pinc), ctypes.c_int(source),
-       R(ml_depth), ctypes.c_int(adiabat), _p(cape_o), _p(cin_o), _p(mul), _p(zo),
-       _p(lfc) if levels else None, _p(el) if levels else None, ctypes.byref(st))
+    if perturb is not None:
+        if fp64 or levels:
+            raise ValueError('perturbed runs are FP32 without levels')
+        lib().xo_set_perturbation(ctypes.c_uint32(perturb))
+        fn = lib().xo_cape_pert32
+        fn.restype = None
+        fn(_p(p), _p(p1d), _p(t), _p(td), _p(ps), _p(ts), _p(tds), _p(start_lev),
+           ctypes.c_int64(nk), ctypes.c_int64(ncol), R(pinc), ctypes.c_int(source),
+           R(ml_depth), ctypes.c_int(adiabat), _p(cape_o), _p(cin_o), _p(mul), _p(zo),
+           ctypes.byref(st))
+    else:
+        fn = lib().xo_cape_levels_f64 if fp64 else lib().xo_cape_levels_f32
+        fn.restype = None
+        fn(_p(p), _p(p1d), _p(t), _p(td), _p(ps), _p(ts), _p(tds), _p(start_lev),
+           ctypes.c_int64(nk), ctypes.c_int64(ncol), R(pinc), ctypes.c_int(source),
+           R(ml_depth), ctypes.c_int(adiabat), _p(cape_o), _p(cin_o), _p(mul), _p(zo),
+           _p(lfc) if levels else None, _p(el) if levels else None, ctypes.byref(st))
     out = (cape_o, cin_o, mul, zo) + ((lfc, el) if levels else ())
     if stats:
         return out + (st.as_dict(),)

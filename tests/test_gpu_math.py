@@ -1,0 +1,127 @@
+"""The accurate mode's device arithmetic, checked on the device.
+
+The kernels divide with the raw IEEE division sequence (the compiler's own
+steps without its operand scaling), divide by per-call constants with a
+correctly rounded reciprocal, and take exp and pow from short kernels
+(phys.h).  Their claims, checked here elementwise through xcape_math_check:
+
+* the raw division equals the compiler's IEEE division bit for bit wherever
+  its operands and quotient are normal numbers (checked from 1e-30 to 1e30,
+  both signs, far beyond the physical range), and gives NaN for a zero,
+  infinite or NaN operand (which the fixed-point evaluation detects and
+  re-evaluates with IEEE division);
+* division by a constant equals IEEE division for the constants the kernels
+  use (the ice ramp's 40 K, pressure increments) -- the CPU check of the
+  same sequence over every float is in the notes of phys.h;
+* the exp kernel equals the device library's expf bit for bit on (-80, 80);
+* pow stays within the device library's powf accuracy: its error against the
+  exactly rounded x^y (from float64 on the host) is at most 1 ulp over the
+  Exner function's and theta-e's ranges, and no worse than powf's.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope='module')
+def dev():
+    import torch
+
+    from xcape_amd import _hip
+    assert _hip.device_count() >= 1, 'no HIP device visible'
+    return torch, _hip
+
+
+def _bits(x):
+    return np.asarray(x, np.float32).view(np.int32)
+
+
+def _ulp_err(got, exact):
+    """|got - exact| in ulps of the float32 nearest to the exact value."""
+    got = np.asarray(got, np.float64)
+    ref32 = exact.astype(np.float32)
+    ulp = np.spacing(np.abs(ref32)).astype(np.float64)
+    return np.abs(got - exact) / ulp
+
+
+def test_raw_division_is_ieee(dev):
+    torch, H = dev
+    rng = np.random.default_rng(11)
+    n = 1 << 22
+    mag = lambda: 10.0 ** rng.uniform(-30, 30, n)  # noqa: E731
+    sgn = lambda: rng.choice([-1.0, 1.0], n)  # noqa: E731
+    a = (sgn() * mag()).astype(np.float32)
+    b = (sgn() * mag()).astype(np.float32)
+    # plus the arithmetic's own scales: quotients near 1, physical magnitudes
+    a[: n // 4] = rng.uniform(-400, 400, n // 4).astype(np.float32)
+    b[: n // 4] = rng.uniform(1, 1.2e5, n // 4).astype(np.float32)
+    q = np.abs(a.astype(np.float64) / b.astype(np.float64))
+    normal = (q > 1e-37) & (q < 1e37)
+    a, b = a[normal], b[normal]
+    ta, tb = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
+    ieee = H.math_check(1, ta, tb).cpu().numpy()
+    raw = H.math_check(0, ta, tb).cpu().numpy()
+    assert np.array_equal(_bits(ieee), _bits(a / b))
+    mism = np.count_nonzero(_bits(raw) != _bits(ieee))
+    print(f'raw division: {mism} of {a.size} differ from IEEE', flush=True)
+    assert mism == 0
+
+
+def test_raw_division_special_operands_give_nan(dev):
+    torch, H = dev
+    a = np.array([1.0, 0.0, np.inf, 2.0, np.nan, 3.0, -np.inf], np.float32)
+    b = np.array([0.0, 0.0, 2.0, np.inf, 1.0, np.nan, -np.inf], np.float32)
+    raw = H.math_check(0, torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()).cpu().numpy()
+    assert np.isnan(raw).all(), raw
+
+
+@pytest.mark.parametrize('c', [40.0, 500.0, 100.0, 1000.0, 2500.0, 1.0, 37.5])
+def test_constant_division_is_ieee(dev, c):
+    torch, H = dev
+    rng = np.random.default_rng(int(c))
+    n = 1 << 22
+    a = (rng.choice([-1.0, 1.0], n) * 10.0 ** rng.uniform(-20, 20, n)).astype(np.float32)
+    a[: n // 2] = rng.uniform(-1e5, 1e5, n // 2).astype(np.float32)
+    ta = torch.from_numpy(a).cuda()
+    tb = torch.full_like(ta, c)
+    ieee = H.math_check(1, ta, tb).cpu().numpy()
+    q = H.math_check(2, ta, tb).cpu().numpy()
+    assert np.array_equal(_bits(q), _bits(ieee))
+
+
+def test_exp_kernel_equals_library_expf(dev):
+    torch, H = dev
+    rng = np.random.default_rng(5)
+    x = rng.uniform(-80, 80, 1 << 22).astype(np.float32)
+    x[:1 << 20] = rng.uniform(-25, 10, 1 << 20).astype(np.float32)   # the arguments used
+    tx = torch.from_numpy(x).cuda()
+    lib = H.math_check(4, tx).cpu().numpy()
+    mine = H.math_check(3, tx).cpu().numpy()
+    mism = np.count_nonzero(_bits(mine) != _bits(lib))
+    print(f'exp kernel: {mism} of {x.size} differ from expf', flush=True)
+    assert mism == 0
+    nan = H.math_check(3, torch.tensor([float('nan')], device='cuda')).cpu().numpy()
+    assert np.isnan(nan).all()
+
+
+@pytest.mark.parametrize('y,lo,hi', [(287.04 / 1005.7, 0.005, 1.2),      # Exner, p/p00
+                                     (0.2854 * (1.0 - 0.28 * 0.03), 0.8, 25.0),  # theta-e
+                                     (0.2854, 0.8, 25.0)])
+def test_pow_kernel_accuracy(dev, y, lo, hi):
+    torch, H = dev
+    rng = np.random.default_rng(3)
+    x = np.exp(rng.uniform(np.log(lo), np.log(hi), 1 << 22)).astype(np.float32)
+    yf = np.float32(y)
+    tx = torch.from_numpy(x).cuda()
+    ty = torch.full_like(tx, float(yf))
+    exact = np.power(x.astype(np.float64), np.float64(yf))
+    mine = H.math_check(5, tx, ty).cpu().numpy()
+    lib = H.math_check(6, tx, ty).cpu().numpy()
+    e_mine, e_lib = _ulp_err(mine, exact), _ulp_err(lib, exact)
+    cr = exact.astype(np.float32)
+    print(f'pow y={y:.5f}: kernel max {e_mine.max():.3f} ulp, correctly rounded '
+          f'{np.mean(mine == cr):.5f}; powf max {e_lib.max():.3f} ulp, correctly rounded '
+          f'{np.mean(lib == cr):.5f}', flush=True)
+    assert e_mine.max() <= 1.0
+    assert e_mine.max() <= max(e_lib.max(), 0.75)

@@ -40,7 +40,7 @@ def _p(a):
 
 
 def host_cape(L, p, t, td, ps, ts, tds, source, adiabat, warm, p1d=None, start=None,
-              level_major=False, pinc=500., levels=False):
+              level_major=False, pinc=500., levels=False, ml_depth=500.):
     ncol, nk = t.shape
     if level_major:
         t, td = np.ascontiguousarray(t.T), np.ascontiguousarray(td.T)
@@ -52,7 +52,7 @@ def host_cape(L, p, t, td, ps, ts, tds, source, adiabat, warm, p1d=None, start=N
     st = np.zeros(6, np.int64)
     L.host_cape(_p(p), _p(p1d), _p(start), _p(t), _p(td), _p(ps), _p(ts), _p(tds),
                 ctypes.c_int(nk), ctypes.c_int(ncol), ctypes.c_int(int(level_major)),
-                ctypes.c_float(pinc), ctypes.c_int(source), ctypes.c_float(500.),
+                ctypes.c_float(pinc), ctypes.c_int(source), ctypes.c_float(ml_depth),
                 ctypes.c_int(adiabat), ctypes.c_int(int(warm)), *[_p(o) for o in out + lv],
                 _p(st))
     if levels:
@@ -136,3 +136,32 @@ def test_lfc_el_plain_bitexact_warm_close(host_lib, source, adiabat):
     for w, r in ((warm[4], ref[4]), (warm[5], ref[5])):
         d = np.abs(w[pos] - r[pos])
         assert np.mean(d <= 1.0) >= 0.99, np.sort(d)[-5:]
+
+
+@pytest.mark.parametrize('adiabat', [1, 2, 3, 4])
+def test_edge_branches(host_lib, adiabat):
+    """The branches only the oracle had seen (tests/edge_cases.py): ML with
+    ml_depth 1 m and 1e6 m, MU with a surface above 500 hPa, the cold-surface
+    gate, NaN levels and the 100-iteration limit.  The plain method equals
+    the oracle bit for bit (NaN included) with the same count of columns
+    stopped by the limit.  The warm start stops the same columns (after its
+    fallback the sub-step is the reference's cold solve), zeroes them, and
+    agrees elsewhere within the CAPE/CIN tolerance."""
+    import edge_cases
+    arr = edge_cases.grid()
+    for src, ml in edge_cases.CASES:
+        ref = O.cape(*arr, 500., src, ml, adiabat, stats=True)
+        st_ref = ref[-1]
+        plain, sp = host_cape(host_lib, *arr, src, adiabat, False, ml_depth=ml)
+        for g, r in zip(plain, ref[:4]):
+            np.testing.assert_array_equal(g, r)
+        assert sp['nonconverged'] == st_ref['nonconverged'], (src, ml)
+        warm, sw = host_cape(host_lib, *arr, src, adiabat, True, ml_depth=ml)
+        assert sw['nonconverged'] == st_ref['nonconverged'], (src, ml)
+        np.testing.assert_array_equal(warm[2], ref[2])
+        nan = np.isnan(ref[0])
+        np.testing.assert_array_equal(np.isnan(warm[0]), nan)
+        ok = ~nan
+        assert cape_ok(warm[0][ok], ref[0][ok]).all() and cape_ok(warm[1][ok], ref[1][ok]).all()
+        stopped = (ref[0] == 0) & (ref[1] == 0)
+        assert np.all(warm[0][stopped] == 0) and np.all(warm[1][stopped] == 0)

@@ -134,23 +134,53 @@ def test_cape_bitexact_vs_reference_pressure_2d(oracle_lib, source, adiabat):
 
 
 def test_cape_edge_branches_vs_reference(oracle_lib):
-    """Branches no fixture reaches: ML with the second level above ml_depth and
-    with the whole column inside it, MU with a surface above 500 hPa, the
-    cold-surface gate, and NaN levels."""
+    """Branches no fixture reaches (tests/edge_cases.py): ML with the second
+    level above ml_depth and with the whole column inside it, MU with a
+    surface above 500 hPa, the cold-surface gate, NaN levels and the
+    100-iteration limit; every adiabat."""
+    import edge_cases
     R = _ref()
-    c = synthetic.sigma_columns(128, 30, 5)
-    p, t, td, ps, ts, tds = (c[k].copy() for k in ('p', 't', 'td', 'ps', 'ts', 'tds'))
-    ts[::7] = -2.0
-    ps[3::11] = 450.0                    # MU: first report above 500 hPa
-    p[3::11] = np.minimum(p[3::11], 440.0)
-    t[5, 10] = np.nan                    # a NaN level mid-column
-    td[6, 20] = np.nan
-    for src, ml in ((3, 1.0), (3, 1.0e6), (3, 500.), (2, 500.), (1, 500.)):
-        for adi in (1, 3, 4):
+    p, t, td, ps, ts, tds = edge_cases.grid()
+    for src, ml in edge_cases.CASES:
+        for adi in (1, 2, 3, 4):
             r = R.loopcape_ml(p.T, t.T, td.T, ps, ts, tds, 500., src, ml, adi)
             o = oracle_lib.cape(p, t, td, ps, ts, tds, 500., src, ml, adi)
             for x, y in zip(o, r):
                 assert _bitexact(x, y), (src, ml, adi)
+    # the limit is reached: the surface source stops columns 47 and 102
+    # (pseudo-liquid) and zeroes them
+    o = oracle_lib.cape(p, t, td, ps, ts, tds, 500., 1, 500., 1, stats=True)
+    assert o[-1]['nonconverged'] == 2
+    assert o[0][47] == 0 and o[1][47] == 0 and o[0][102] == 0
+
+
+def test_perturbed_library_runs(oracle_lib):
+    """The oracle's perturbed runs (exp/log/pow results moved by up to an
+    ulp, the stand-in for another C library) are active and mild where the
+    reference's solve is well conditioned: on the 64x64x40 configuration
+    nearly every column's CAPE changes in its low bits, within the CAPE/CIN
+    tolerance wherever both runs converge.  They also show where it is not:
+    in warm, moist sub-steps the map's slope is near -5, the relaxed step
+    contracts by only ~0.8, and ulp-level noise in the map keeps |f(x) - x|
+    near the 2e-4 K test, so that whether the reference converges within its
+    100 iterations -- or zeroes the column -- turns on its library's
+    rounding.  A few columns per thousand at most."""
+    from tolerances import cape_ok
+    c = synthetic.sigma_columns(4096, 40, 1000)
+    a = [c[k] for k in ('p', 't', 'td', 'ps', 'ts', 'tds')]
+    o = oracle_lib.cape(*a, 500., 1, 500., 1, stats=True)
+    assert o[-1]['nonconverged'] == 0
+    runs = [oracle_lib.cape(*a, 500., 1, 500., 1, perturb=s, stats=True) for s in (1, 2)]
+    pos = o[0] > 0
+    assert np.mean(runs[0][0][pos] != o[0][pos]) > 0.5
+    assert np.any(runs[0][0] != runs[1][0])
+    for q in runs:
+        stopped = (q[0] == 0) & (q[1] == 0) & pos
+        assert np.count_nonzero(stopped) == q[-1]['nonconverged']
+        assert np.count_nonzero(stopped) <= 4
+        ok = ~stopped
+        assert cape_ok(q[0][ok], o[0][ok]).all() and cape_ok(q[1][ok], o[1][ok]).all()
+        np.testing.assert_array_equal(q[2], o[2])
 
 
 @pytest.mark.parametrize('pinc', [100., 1000., 2500.])

@@ -22,6 +22,7 @@ XCAPE_WARM_START = 0x4
 XCAPE_NO_SURROGATE = 0x8
 XCAPE_LEVEL_MAJOR = 0x10
 XCAPE_NO_PREDICTION = 0x20
+XCAPE_INPUT_F64 = 0x40
 
 
 class XcapeError(RuntimeError):
@@ -84,6 +85,7 @@ SIGNATURES = {
     'xcape_surrogate_eval': (_I, [_P, _I64, _P, _P]),
     'xcape_surrogate_log_f32': (_I, [_P, _P, _P, _P, _P, _P, _I64, _I64, _F, _I, _F, _I, _F,
                                      _U32, _I, _P, _I64, _P, _P]),
+    'xcape_math_check': (_I, [_I, _P, _P, _I64, _P, _P]),
 }
 
 
@@ -182,6 +184,14 @@ class _Args:
             return a.to(torch.float32).contiguous()
         return np.ascontiguousarray(a, dtype=np.float32)
 
+    def f64(self, a):
+        if a is None:
+            return None
+        if self.device:
+            import torch
+            return a.to(torch.float64).contiguous()
+        return np.ascontiguousarray(a, dtype=np.float64)
+
     def i32(self, a):
         if a is None:
             return None
@@ -257,13 +267,16 @@ def cape(p, t, td, ps, ts, tds, *, pinc=500., source=1, ml_depth=500., adiabat=1
 
 
 def srh(p, t, td, u, v, ps, ts, tds, us, vs, *, depth=3000., p_shared=False, start_lev=None,
-        motions=True, level_major=False, fp64=False):
+        motions=True, level_major=False, fp64=False, inputs_f64=False):
     """Run xcape_srh_f32 (xcape_srh_f64 when ``fp64``).  Returns srh_rm,
     srh_lm (float64) and, when ``motions``, rm, lm, mean6 as (ncol, 2) (u, v)
-    pairs, float32 (float64 when ``fp64``)."""
+    pairs, float32 (float64 when ``fp64``).  ``inputs_f64``: pass the inputs
+    as float64 (XCAPE_INPUT_F64), so that the FP64 heights and helicity see
+    them unrounded."""
     A = _Args([p, t, td, u, v, ps, ts, tds, us, vs, start_lev])
-    p, t, td, u, v, ps, ts, tds, us, vs = (A.f32(a) for a in (p, t, td, u, v, ps, ts, tds,
-                                                              us, vs))
+    conv = A.f64 if inputs_f64 else A.f32
+    p, t, td, u, v, ps, ts, tds, us, vs = (conv(a) for a in (p, t, td, u, v, ps, ts, tds,
+                                                             us, vs))
     start_lev = A.i32(start_lev)
     ncol, nk = t.shape[::-1] if level_major else t.shape
     srm, slm = (A.empty(ncol, np.float64, t) for _ in range(2))
@@ -272,7 +285,8 @@ def srh(p, t, td, u, v, ps, ts, tds, us, vs, *, depth=3000., p_shared=False, sta
         rm, lm, mw = (A.empty(2 * ncol, mdt, t) for _ in range(3))
     else:
         rm = lm = mw = None
-    flags = (XCAPE_DEVICE_PTRS if A.device else 0) | (XCAPE_LEVEL_MAJOR if level_major else 0)
+    flags = (XCAPE_DEVICE_PTRS if A.device else 0) | (XCAPE_LEVEL_MAJOR if level_major else 0) \
+        | (XCAPE_INPUT_F64 if inputs_f64 else 0)
     fn = lib().xcape_srh_f64 if fp64 else lib().xcape_srh_f32
     rc = fn(_ptr(p), 1 if p_shared else 0, _ptr(start_lev), _ptr(t), _ptr(td),
             _ptr(u), _ptr(v), _ptr(ps), _ptr(ts), _ptr(tds), _ptr(us), _ptr(vs), nk, ncol,
@@ -311,3 +325,15 @@ def cape_srh(p, t, td, u, v, ps, ts, tds, us, vs, *, pinc=500., source=1, ml_dep
     if stats:
         return out + (st.as_dict(),)
     return out
+
+
+def math_check(fn, a, b=None):
+    """xcape_math_check on device tensors: the accurate mode's arithmetic
+    kernel `fn` applied elementwise (see include/xcape_hip.h)."""
+    import torch
+    a = a.to(torch.float32).contiguous()
+    b = None if b is None else b.to(torch.float32).contiguous()
+    y = torch.empty_like(a)
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    check(lib().xcape_math_check(fn, _ptr(a), _ptr(b), a.numel(), _ptr(y), stream))
+    return y

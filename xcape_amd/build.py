@@ -39,6 +39,7 @@ UNITS = {
     'mlp.hip': ['-ffp-contract=fast'],
     'surrogate_log.hip': ['-ffp-contract=off'],
     'transpose.hip': ['-ffp-contract=off'],
+    'math_check.hip': ['-ffp-contract=off'],
 }
 
 

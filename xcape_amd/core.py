@@ -302,9 +302,14 @@ def _calc_srh_numpy(*args, depth=3000, vertical_lev='sigma', output_var='srh', m
     out_sel = _OUTPUT[output_var]
 
     if method in _GPU_METHODS:
+        # float64 inputs reach the FP64 heights and helicity unrounded (the
+        # reference's f2py wrappers pass them to its DOUBLE PRECISION
+        # routines as given, and round them for the REAL Bunkers routine)
+        f64 = any(np.asarray(a).dtype == np.float64
+                  for a in (p_2d, t_2d, td_2d, u_2d, v_2d, p_s1d, t_s1d, td_s1d, u_s1d, v_s1d))
         srm, slm, rm, lm, mw = _hip.srh(p_2d, t_2d, td_2d, u_2d, v_2d, p_s1d, t_s1d, td_s1d,
                                         u_s1d, v_s1d, depth=float(depth), p_shared=p_shared,
-                                        start_lev=start)
+                                        start_lev=start, inputs_f64=f64)
     elif method == 'dummy':
         n = t_2d.shape[0]
         srm, slm = np.ones(n), np.ones(n)

@@ -110,10 +110,11 @@ XC_HD __forceinline__ T layer_dz(T thv_k, T thv_km1, T pi_k, T pi_km1)
 }
 
 // One fixed-point evaluation (:437-462): the parcel state at p2 implied by
-// the iterate thlast, and the theta it gives back.
+// the iterate thlast, and the theta it gives back.  fp_eval_ops: with the
+// operations of math mode MATH.
 template <typename T, int MATH, bool ICE>
-XC_HD __forceinline__ void fp_eval(T p2, T pi2, T logp, T t1, T th1, T qv1, T ql1, T qi1,
-                                        T qt, T thlast, T &th2, T &t2, T &qv2, T &ql2, T &qi2)
+XC_HD __forceinline__ void fp_eval_ops(T p2, T pi2, T logp, T t1, T th1, T qv1, T ql1, T qi1,
+                                       T qt, T thlast, T &th2, T &t2, T &qv2, T &ql2, T &qi2)
 {
     using F = Fn<T, MATH>;
     using K = Phys<T>;
@@ -146,6 +147,81 @@ XC_HD __forceinline__ void fp_eval(T p2, T pi2, T logp, T t1, T th1, T qv1, T ql
         arg = F::div(lhv * (ql2 - ql1), cpm * tbar) + (F::div(rm, cpm) - K::rddcp) * logp;
     }
     th2 = th1 * F::exp(arg);
+}
+
+#if defined(__HIP_DEVICE_COMPILE__)
+// The accurate mode's evaluation on the device: the operations of
+// fp_eval_ops in the same order, with the raw IEEE division kernels and
+// exp_core (phys.h).  Wherever those give the library's values -- every
+// operand in the normal range, exp arguments within (-87, 88) -- the result
+// is fp_eval_ops<float, MATH_LIB>'s bit for bit; a zero or infinite divisor
+// or other special value turns the raw division into NaN, which reaches th2
+// (every quotient feeds qv2 or the exponent), and the evaluation is then
+// repeated with the library's operations.  Divisors: the ice ramp's
+// 273.15 - 233.15 by a correctly rounded reciprocal; the two latent-heat
+// terms share one reciprocal of cpm * tbar.
+template <bool ICE>
+__device__ __forceinline__ void fp_eval_raw(float p2, float pi2, float logp, float t1, float th1,
+                                            float qv1, float ql1, float qi1, float qt,
+                                            float thlast, float &th2, float &t2, float &qv2,
+                                            float &ql2, float &qi2)
+{
+    using K = Phys<float>;
+    constexpr float RAMP = 273.15f - 233.15f;
+    constexpr float RRAMP = 1.0f / RAMP;
+    t2 = thlast * pi2;
+    const float tc = t2 - 273.15f;
+    const float esl = 611.2f * exp_core(div_raw(17.67f * tc, t2 - 29.65f));
+    const float qsl = div_raw(K::eps * esl, p2 - esl);
+    if (ICE) {
+        const float fliq = fmx(fmn(div_const(t2 - 233.15f, RAMP, RRAMP), 1.0f), 0.0f);
+        const float fice = 1.0f - fliq;
+        const float esi = 611.2f * exp_core(div_raw(21.8745584f * tc, t2 - 7.66f));
+        const float qsi = div_raw(K::eps * esi, p2 - esi);
+        qv2 = fmn(qt, fliq * qsl + fice * qsi);
+        qi2 = fmx(fice * (qt - qv2), 0.0f);
+    } else {
+        qv2 = fmn(qt, qsl);
+        qi2 = 0.0f;
+    }
+    ql2 = fmx(qt - qv2 - qi2, 0.0f);
+    const float tbar = 0.5f * (t1 + t2);
+    const float qvbar = 0.5f * (qv1 + qv2);
+    const float qlbar = 0.5f * (ql1 + ql2);
+    const float lhv = K::lv1 - K::lv2 * tbar;
+    const float rm = K::rd + K::rv * qvbar;
+    float arg;
+    if (ICE) {
+        const float qibar = 0.5f * (qi1 + qi2);
+        const float lhs = K::ls1 - K::ls2 * tbar;
+        const float cpm = K::cp + K::cpv * qvbar + K::cpl * qlbar + K::cpi * qibar;
+        float a1, a2;
+        div2_raw(lhv * (ql2 - ql1), lhs * (qi2 - qi1), cpm * tbar, a1, a2);
+        arg = a1 + a2 + (div_raw(rm, cpm) - K::rddcp) * logp;
+    } else {
+        const float cpm = K::cp + K::cpv * qvbar + K::cpl * qlbar;
+        arg = div_raw(lhv * (ql2 - ql1), cpm * tbar) + (div_raw(rm, cpm) - K::rddcp) * logp;
+    }
+    th2 = th1 * exp_core(arg);
+}
+#endif
+
+template <typename T, int MATH, bool ICE>
+XC_HD __forceinline__ void fp_eval(T p2, T pi2, T logp, T t1, T th1, T qv1, T ql1, T qi1,
+                                   T qt, T thlast, T &th2, T &t2, T &qv2, T &ql2, T &qi2)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (sizeof(T) == 4 && MATH == MATH_ACCURATE) {
+        fp_eval_raw<ICE>(p2, pi2, logp, t1, th1, qv1, ql1, qi1, qt, thlast, th2, t2, qv2, ql2,
+                         qi2);
+        if (__builtin_expect(!__builtin_isfinite(th2), 0))
+            fp_eval_ops<T, MATH_LIB, ICE>(p2, pi2, logp, t1, th1, qv1, ql1, qi1, qt, thlast,
+                                          th2, t2, qv2, ql2, qi2);
+        return;
+    }
+#endif
+    fp_eval_ops<T, MATH, ICE>(p2, pi2, logp, t1, th1, qv1, ql1, qi1, qt, thlast, th2, t2, qv2,
+                              ql2, qi2);
 }
 
 // Moist-adiabat fixed point for one sub-step from the cold start (:428-480).
@@ -702,8 +778,10 @@ struct WarmAscent {
         }
         // up to XC_ITERS_PER_TRIP fixed-point iterations (:436-479): most
         // warm-started sub-steps converge at the first or second, so two per
-        // trip halve the trips that pay for the set-up blocks around them
-#pragma unroll
+        // trip halve the trips that pay for the set-up blocks around them.
+        // Not unrolled: the unrolled pair, each with the evaluation's IEEE
+        // re-evaluation path, no longer fits the 64 registers
+#pragma unroll 1
         for (int rep = 0; rep < XC_ITERS_PER_TRIP; ++rep) {
             if (mode != M_ITER) continue;
             it = it + 1;
@@ -749,7 +827,6 @@ struct WarmAscent {
             // sub-step's last two iterates (kept from an earlier sub-step
             // when it converged at its first)
             const T x = thlast;
-            first_sub = false;
             if (it >= 2) {
                 const T dx = x - x_prev;
                 if (dx != T(0.0)) {
@@ -757,6 +834,32 @@ struct WarmAscent {
                     fpr = sl < T(-10.0) ? T(-10.0) : (sl > T(0.9) ? T(0.9) : sl);
                 }
             }
+        }
+        if (mode == M_CONV && warm) {
+            // Would the reference's own solve, from the cold start th1 with
+            // its relaxed step (contraction c = 1 - 0.3 (1 - f') per
+            // iteration), take more than about 50 iterations here?  Then it
+            // may hit its 100-iteration limit (which zeroes the column),
+            // while a predicted iterate near the fixed point need not: the
+            // plain solve decides, restarted cold.  Linearised count:
+            // |th1 - x| |c|^50 > the tolerance.  Never on ordinary soundings
+            // (the cold solve takes about 10 iterations there).
+            const T c = T(1.0) - T(0.3) * (T(1.0) - fpr);
+            const T c2 = c * c, c4 = c2 * c2, c8 = c4 * c4, c16 = c8 * c8;
+            const T c50 = c16 * c16 * c16 * c2;
+            const T e0 = thlast - th1;
+            if ((e0 < T(0.0) ? -e0 : e0) * c50 > K::converge) {
+                thlast = th1;
+                warm = false;
+                fell_back = true;
+                it = 0;
+                st.fallbacks++;
+                mode = M_ITER;
+            }
+        }
+        if (mode == M_CONV) {
+            const T x = thlast;
+            first_sub = false;
             // the rate's shape at the next sub-step's start, for the
             // pseudo-adiabats (the shape assumes the condensate falls out),
             // where the condensate is all liquid (above 0 degC; everywhere

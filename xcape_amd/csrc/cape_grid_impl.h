@@ -24,38 +24,49 @@ namespace XC_NS {
 // poff + (k - 1) * pls of p (the shared level vector on a pressure grid).
 // IDX: uint32_t when every element index fits, so that a lane carries one
 // 32-bit offset per array family instead of a 64-bit pointer per array.
-template <typename IDX>
+// TI: the element type of the profile and surface arrays (float; double for
+// the helicity chain's float64 inputs, CapeGridArgs::in_f64).
+template <typename IDX, typename TI = float>
 struct SigmaCol {
-    const float *p, *t, *td, *u, *v;
+    const TI *p, *t, *td, *u, *v;
     IDX off, poff, ls, pls;
-    float ps, ts, tds, us, vs;
+    TI ps, ts, tds, us, vs;
     __device__ __forceinline__ IDX at(int k) const { return off + (IDX)(k - 1) * ls; }
-    __device__ __forceinline__ float P(int k) const
+    __device__ __forceinline__ TI P(int k) const
     {
         return k == 0 ? ps : p[poff + (IDX)(k - 1) * pls];
     }
-    __device__ __forceinline__ float Tc(int k) const { return k == 0 ? ts : t[at(k)]; }
-    __device__ __forceinline__ float Tdc(int k) const { return k == 0 ? tds : td[at(k)]; }
-    __device__ __forceinline__ float U(int k) const { return k == 0 ? us : u[at(k)]; }
-    __device__ __forceinline__ float V(int k) const { return k == 0 ? vs : v[at(k)]; }
+    __device__ __forceinline__ TI Tc(int k) const { return k == 0 ? ts : t[at(k)]; }
+    __device__ __forceinline__ TI Tdc(int k) const { return k == 0 ? tds : td[at(k)]; }
+    __device__ __forceinline__ TI U(int k) const { return k == 0 ? us : u[at(k)]; }
+    __device__ __forceinline__ TI V(int k) const { return k == 0 ? vs : v[at(k)]; }
     // a level above the surface (k >= 1): no surface select
     __device__ __forceinline__ Raw raw_above(int k) const
     {
-        return Raw{p[poff + (IDX)(k - 1) * pls], t[at(k)], td[at(k)]};
+        return Raw{(float)p[poff + (IDX)(k - 1) * pls], (float)t[at(k)], (float)td[at(k)]};
     }
 };
+
+// An input array of CapeGridArgs as its element type TI (the arrays are
+// declared float; with in_f64 they hold doubles).
+template <typename TI>
+__device__ __forceinline__ const TI *in_arr(const float *a)
+{
+    return reinterpret_cast<const TI *>(a);
+}
 
 // First level at or above the ground, 1-based: the reference's
 // argmin(masked_less(ps - p, 0)) + 1 (core.py:286-289), with numpy's rules:
 // a NaN difference is unmasked and wins the argmin; all-masked gives 0.
-__device__ __forceinline__ int start_level(const float *p1d, int nk_in, float ps)
+template <typename TI>
+__device__ __forceinline__ int start_level(const TI *p1d, int nk_in, TI ps)
 {
     int best = -1;
-    float bv = 0.0f;
+    TI bv = TI(0);
     for (int j = 0; j < nk_in; ++j) {
-        float d = ps - p1d[j];
+        TI d = ps - p1d[j];
         if (d != d) return j + 1;
-        if (d >= 0.0f && (best < 0 || d < bv)) {
+        if (d >= TI(0) && (best < 0 || d < bv)) {
             best = j;
             bv = d;
         }
@@ -87,21 +98,21 @@ __device__ __forceinline__ void block_stats_add(const LaneStats &st, bool ran,
 
 // Column set-up shared by the kernels: the accessor of column `col` and its
 // level count (surface included).  Returns false for lanes past the grid.
-template <typename IDX>
-__device__ __forceinline__ bool column_at(const CapeGridArgs &a, int64_t col, SigmaCol<IDX> &c,
-                                          int &nk)
+template <typename IDX, typename TI = float>
+__device__ __forceinline__ bool column_at(const CapeGridArgs &a, int64_t col,
+                                          SigmaCol<IDX, TI> &c, int &nk)
 {
     if (col >= a.ncol) return false;
-    const float ps = a.ps[col];
+    const TI ps = in_arr<TI>(a.ps)[col];
     c.ls = (IDX)a.ls;
     IDX off = a.ls == 1 ? (IDX)col * (IDX)a.nk_in : (IDX)col;
     if (a.p1d) {
-        int s1 = a.start ? a.start[col] : start_level(a.p1d, (int)a.nk_in, ps);
+        int s1 = a.start ? a.start[col] : start_level(in_arr<TI>(a.p1d), (int)a.nk_in, ps);
         // a start level outside 1..nk_in cannot come from the rule above;
         // clamp caller-supplied ones so the column is never read outside
         s1 = s1 < 1 ? 1 : (s1 > (int)a.nk_in ? (int)a.nk_in : s1);
         const int s0 = s1 - 1;
-        c.p = a.p1d;
+        c.p = in_arr<TI>(a.p1d);
         c.poff = (IDX)s0;
         c.pls = 1;
         off += (IDX)s0 * c.ls;
@@ -117,18 +128,18 @@ __device__ __forceinline__ bool column_at(const CapeGridArgs &a, int64_t col, Si
         } else {
             nk = (a.nk_col ? (int)a.nk_col[col] : (int)a.nk_in) + 1;
         }
-        c.p = a.p;
+        c.p = in_arr<TI>(a.p);
         c.poff = off;
         c.pls = c.ls;
     }
     c.off = off;
-    c.t = a.t;
-    c.td = a.td;
-    c.u = a.u;
-    c.v = a.v;
+    c.t = in_arr<TI>(a.t);
+    c.td = in_arr<TI>(a.td);
+    c.u = in_arr<TI>(a.u);
+    c.v = in_arr<TI>(a.v);
     c.ps = ps;
-    c.ts = a.ts[col];
-    c.tds = a.tds[col];
+    c.ts = in_arr<TI>(a.ts)[col];
+    c.tds = in_arr<TI>(a.tds)[col];
     return true;
 }
 
@@ -482,16 +493,18 @@ __global__ void warm_reset_kernel(unsigned int *work, uint32_t ncol, uint32_t gr
 }
 
 // Storm-relative helicity chain, one column per lane.  TB: the storm-motion
-// precision (float: the reference's; double: the FP64 re-run).
-template <typename TB>
+// precision (float: the reference's; double: the FP64 re-run).  TI: the
+// input element type (double: float64 profiles, which feed the FP64 heights
+// and helicity unrounded as the reference's f2py wrappers pass them).
+template <typename TB, typename TI>
 __global__ void __launch_bounds__(256) srh_grid_kernel(CapeGridArgs a)
 {
     const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    SigmaCol<uint64_t> c{};
+    SigmaCol<uint64_t, TI> c{};
     int nk = 0;
     if (!column_at(a, col, c, nk)) return;
-    c.us = a.us[col];
-    c.vs = a.vs[col];
+    c.us = in_arr<TI>(a.us)[col];
+    c.vs = in_arr<TI>(a.vs)[col];
     SrhOutT<TB> so;
     srh_column<TB>(c, nk, a.depth, 2.0, so);
     a.srh_rm[col] = so.srh_rm;
@@ -506,6 +519,16 @@ __global__ void __launch_bounds__(256) srh_grid_kernel(CapeGridArgs a)
         mw[2 * col] = so.mean_u;
         mw[2 * col + 1] = so.mean_v;
     }
+}
+
+template <typename TB>
+hipError_t launch_srh(const CapeGridArgs &a, dim3 grid, int block, hipStream_t s)
+{
+    if (a.in_f64)
+        srh_grid_kernel<TB, double><<<grid, block, 0, s>>>(a);
+    else
+        srh_grid_kernel<TB, float><<<grid, block, 0, s>>>(a);
+    return hipGetLastError();
 }
 
 template <typename T, typename IDX, bool STATS, bool LEVELS>
@@ -631,19 +654,15 @@ hipError_t launch_t(const CapeGridArgs &a, int kind, int block, hipStream_t s)
     if constexpr (sizeof(T) == 8) {
         // the FP64 yardstick: CAPE with the reference's cold start, and the
         // helicity chain with FP64 storm motion
-        if (kind == GRID_SRH) {
-            srh_grid_kernel<double><<<grid, block, 0, s>>>(a);
-            return hipGetLastError();
-        }
-        if (kind != GRID_CAPE || a.warm) return hipErrorInvalidValue;
+        if (kind == GRID_SRH) return launch_srh<double>(a, grid, block, s);
+        if (kind != GRID_CAPE || a.warm || a.in_f64) return hipErrorInvalidValue;
         return launch_cape<T>(a, block, s);
     } else {
         switch (kind) {
         case GRID_CAPE:
+            if (a.in_f64) return hipErrorInvalidValue;
             return a.warm == 0 ? launch_cape<T>(a, block, s) : launch_warm(a, block, s);
-        case GRID_SRH:
-            srh_grid_kernel<float><<<grid, block, 0, s>>>(a);
-            return hipGetLastError();
+        case GRID_SRH: return launch_srh<float>(a, grid, block, s);
         default: return hipErrorInvalidValue;
         }
     }

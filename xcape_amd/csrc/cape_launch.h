@@ -22,6 +22,8 @@ struct CapeGridArgs {
     // column (its caller's order); nullptr: uniform / identity
     const int32_t *nk_col, *src_col, *out_index;
     int64_t ls;                  // level stride of the (nk_in, ncol) arrays: 1 or ncol
+    int in_f64;                  // the profile and surface arrays (p, p1d, t, td, u, v,
+                                 // ps, ts, tds, us, vs) hold doubles (helicity only)
     float pinc, ml_depth;
     int source, adiabat;
     void *cape, *cin, *zout;     // float or double (ncol,)

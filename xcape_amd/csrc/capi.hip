@@ -196,7 +196,7 @@ hipError_t launch_grid(const CapeGridArgs &a, bool cape, bool srh, int math, hip
 // warm-started ascent's records); 0 when the call needs none.
 int64_t grid_scratch_floats(const CapeGridArgs &a0, bool cape, bool srh, int math)
 {
-    const bool stage = a0.ls == 1 && a0.nk_in > 1;
+    const bool stage = a0.ls == 1 && a0.nk_in > 1 && !a0.in_f64;
     const bool warm = cape && a0.warm != 0 && math != 2;
     const int narr = stage ? 2 + (a0.p != nullptr) + (srh ? 2 : 0) : 0;
     return (int64_t)narr * a0.nk_in + (warm ? warm_scratch_floats(a0.nk_in) : 0);
@@ -208,7 +208,8 @@ int64_t grid_scratch_floats(const CapeGridArgs &a0, bool cape, bool srh, int mat
 hipError_t run_grid(const CapeGridArgs &a0, bool cape, bool srh, int math, hipStream_t s,
                     float *ext = nullptr, unsigned int *ext_work = nullptr)
 {
-    const bool stage = a0.ls == 1 && a0.nk_in > 1;
+    // float64 inputs (the helicity chain only) are read where they lie
+    const bool stage = a0.ls == 1 && a0.nk_in > 1 && !a0.in_f64;
     const bool warm = cape && a0.warm != 0 && math != 2;
     if (!stage && !warm) return launch_grid(a0, cape, srh, math, s);
     const bool sigma = a0.p != nullptr;
@@ -739,8 +740,12 @@ int grid_entry(const GridCall &c)
             return fail(XCAPE_EINVAL, "rm, lm and mean6 must be all NULL or all set");
     }
     const uint32_t known = XCAPE_DEVICE_PTRS | XCAPE_FAST_MATH | XCAPE_WARM_START |
-                           XCAPE_NO_SURROGATE | XCAPE_LEVEL_MAJOR | XCAPE_NO_PREDICTION;
+                           XCAPE_NO_SURROGATE | XCAPE_LEVEL_MAJOR | XCAPE_NO_PREDICTION |
+                           XCAPE_INPUT_F64;
     if (c.flags & ~known) return fail(XCAPE_EINVAL, "unknown flags 0x%x", c.flags & ~known);
+    const bool f64in = c.flags & XCAPE_INPUT_F64;
+    if (f64in && want_cape)
+        return fail(XCAPE_EINVAL, "XCAPE_INPUT_F64 is for the helicity entry points only");
     if (c.fp64 && (c.flags & (XCAPE_FAST_MATH | XCAPE_WARM_START)))
         return fail(XCAPE_EINVAL, "the FP64 entry point supports neither FAST_MATH nor WARM_START");
     if (want_srh && !c.p_shared && c.start)
@@ -774,6 +779,7 @@ int grid_entry(const GridCall &c)
     a.nk_in = c.nk;
     a.ncol = c.ncol;
     a.ls = (c.flags & XCAPE_LEVEL_MAJOR) ? c.ncol : 1;
+    a.in_f64 = f64in ? 1 : 0;
     a.pinc = (float)c.pinc;
     a.ml_depth = (float)c.ml_depth;
     a.source = c.source;
@@ -786,8 +792,10 @@ int grid_entry(const GridCall &c)
     const int math = c.fp64 ? 2 : ((c.flags & XCAPE_FAST_MATH) ? 1 : 0);
     // host arrays in column order stream through pinned staging; level-major
     // host arrays go whole through stream-ordered buffers
-    if (!dev && !(c.flags & XCAPE_LEVEL_MAJOR))
+    if (!dev && !(c.flags & XCAPE_LEVEL_MAJOR) && !f64in)
         return stream_host_grid(c, a, want_cape, want_srh, math);
+    // input floats per element (double inputs: two)
+    const size_t iw = f64in ? 2 : 1;
     if (dev) {
         a.p = c.p_shared ? nullptr : c.p;
         a.p1d = c.p_shared ? c.p : nullptr;
@@ -799,12 +807,12 @@ int grid_entry(const GridCall &c)
         a.srh_rm = c.srh_rm; a.srh_lm = c.srh_lm; a.rm = c.rm; a.lm = c.lm; a.mean6 = c.mean6;
     } else {
 #define XC_TRY(x) if ((e = (x)) != hipSuccess) return hip_fail(e, #x)
-        XC_TRY(dp_.alloc(np, s)); XC_TRY(dp_.upload(c.p, np, s));
-        XC_TRY(dt_.alloc(n2, s)); XC_TRY(dt_.upload(c.t, n2, s));
-        XC_TRY(dtd_.alloc(n2, s)); XC_TRY(dtd_.upload(c.td, n2, s));
-        XC_TRY(dps_.alloc(ncol, s)); XC_TRY(dps_.upload(c.ps, ncol, s));
-        XC_TRY(dts_.alloc(ncol, s)); XC_TRY(dts_.upload(c.ts, ncol, s));
-        XC_TRY(dtds_.alloc(ncol, s)); XC_TRY(dtds_.upload(c.tds, ncol, s));
+        XC_TRY(dp_.alloc(np * iw, s)); XC_TRY(dp_.upload(c.p, np * iw, s));
+        XC_TRY(dt_.alloc(n2 * iw, s)); XC_TRY(dt_.upload(c.t, n2 * iw, s));
+        XC_TRY(dtd_.alloc(n2 * iw, s)); XC_TRY(dtd_.upload(c.td, n2 * iw, s));
+        XC_TRY(dps_.alloc(ncol * iw, s)); XC_TRY(dps_.upload(c.ps, ncol * iw, s));
+        XC_TRY(dts_.alloc(ncol * iw, s)); XC_TRY(dts_.upload(c.ts, ncol * iw, s));
+        XC_TRY(dtds_.alloc(ncol * iw, s)); XC_TRY(dtds_.upload(c.tds, ncol * iw, s));
         if (c.start) {
             XC_TRY(dst_.alloc(ncol, s)); XC_TRY(dst_.upload(c.start, ncol, s));
         }
@@ -822,10 +830,10 @@ int grid_entry(const GridCall &c)
             }
         }
         if (want_srh) {
-            XC_TRY(du_.alloc(n2, s)); XC_TRY(du_.upload(c.u, n2, s));
-            XC_TRY(dv_.alloc(n2, s)); XC_TRY(dv_.upload(c.v, n2, s));
-            XC_TRY(dus_.alloc(ncol, s)); XC_TRY(dus_.upload(c.us, ncol, s));
-            XC_TRY(dvs_.alloc(ncol, s)); XC_TRY(dvs_.upload(c.vs, ncol, s));
+            XC_TRY(du_.alloc(n2 * iw, s)); XC_TRY(du_.upload(c.u, n2 * iw, s));
+            XC_TRY(dv_.alloc(n2 * iw, s)); XC_TRY(dv_.upload(c.v, n2 * iw, s));
+            XC_TRY(dus_.alloc(ncol * iw, s)); XC_TRY(dus_.upload(c.us, ncol * iw, s));
+            XC_TRY(dvs_.alloc(ncol * iw, s)); XC_TRY(dvs_.upload(c.vs, ncol * iw, s));
             XC_TRY(dsrm_.alloc(ncol, s)); XC_TRY(dslm_.alloc(ncol, s));
             a.u = du_.ptr; a.v = dv_.ptr; a.us = dus_.ptr; a.vs = dvs_.ptr;
             a.srh_rm = dsrm_.ptr; a.srh_lm = dslm_.ptr;

@@ -15,6 +15,10 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include <atomic>
+#include <mutex>
+#include <vector>
+
 #include "../../include/xcape_hip.h"
 #include "capi_common.h"
 #include "mlp_device.h"
@@ -232,11 +236,18 @@ int check_n(int64_t n)
 
 namespace xcape {
 
-// The loaded network (device memory, one per process and device 0's
-// context); nullptr until xcape_surrogate_load.
-SurrogateDev *g_surrogate = nullptr;
+// The loaded network (device memory, one per process); nullptr until
+// xcape_surrogate_load.  A load packs the new weights into a buffer of its
+// own, waits for the packing to finish and only then publishes the pointer,
+// so a kernel launched by another thread sees either the old network or the
+// new one, whole.  Replaced buffers are kept (13.6 KB each) until the process
+// ends: kernels already queued, and ragged plans (which keep the pointer
+// they were created with), may still read them.
+std::atomic<SurrogateDev *> g_surrogate{nullptr};
+std::mutex g_surrogate_mu;                   // serialises loads
+std::vector<SurrogateDev *> g_surrogate_old;  // replaced buffers, kept alive
 
-const void *surrogate_device() { return g_surrogate; }
+const void *surrogate_device() { return g_surrogate.load(std::memory_order_acquire); }
 
 }  // namespace xcape
 
@@ -300,24 +311,31 @@ int xcape_surrogate_load(const float *theta, const float *norm, void *stream)
     if (rc) return rc;
     hipStream_t s = (hipStream_t)stream;
     hipError_t e;
-    if (!g_surrogate && (e = hipMalloc((void **)&g_surrogate, sizeof(SurrogateDev))))
+    std::lock_guard<std::mutex> lk(g_surrogate_mu);
+    SurrogateDev *fresh = nullptr;
+    if ((e = hipMalloc((void **)&fresh, sizeof(SurrogateDev))))
         return hip_fail(e, "surrogate allocation");
-    pack_surrogate<<<1, 256, 0, s>>>(theta, norm, g_surrogate);
-    if ((e = hipGetLastError()) || (e = hipStreamSynchronize(s)))
+    pack_surrogate<<<1, 256, 0, s>>>(theta, norm, fresh);
+    if ((e = hipGetLastError()) || (e = hipStreamSynchronize(s))) {
+        (void)hipFree(fresh);
         return hip_fail(e, "surrogate load");
+    }
+    SurrogateDev *old = g_surrogate.exchange(fresh, std::memory_order_acq_rel);
+    if (old) g_surrogate_old.push_back(old);
     return XCAPE_OK;
 }
 
-int xcape_surrogate_loaded(void) { return g_surrogate != nullptr; }
+int xcape_surrogate_loaded(void) { return surrogate_device() != nullptr; }
 
 int xcape_surrogate_eval(const float *x, int64_t n, float *y, void *stream)
 {
     int rc = check_n(n);
     if (rc || (rc = ensure_device())) return rc;
-    if (!g_surrogate) return fail(XCAPE_ESTATE, "no surrogate loaded");
+    const SurrogateDev *sur = static_cast<const SurrogateDev *>(surrogate_device());
+    if (!sur) return fail(XCAPE_ESTATE, "no surrogate loaded");
     if (!x || !y) return fail(XCAPE_EINVAL, "NULL argument");
     hipStream_t s = (hipStream_t)stream;
-    surrogate_eval_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(g_surrogate, x, n, y);
+    surrogate_eval_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(sur, x, n, y);
     hipError_t e = hipGetLastError();
     return e ? hip_fail(e, "surrogate_eval") : XCAPE_OK;
 }

@@ -6,10 +6,11 @@
 // decimal values in double precision (the FP64 yardstick run).
 //
 // Math modes:
-//   MATH_ACCURATE - IEEE-rounded division and the device library's expf/logf/
-//                   powf (<= 1 ulp); translation units using it are compiled
-//                   with -ffp-contract=off so every +,-,*,/ rounds exactly as
-//                   the reference's does.
+//   MATH_ACCURATE - IEEE-rounded + - * / and exp, log, pow within about an
+//                   ulp (the device library's expf and logf, the pow kernel
+//                   below); translation units using it are compiled with
+//                   -ffp-contract=off so every + - * / rounds exactly as the
+//                   reference's does.
 //   MATH_FAST     - hardware exp2/log2 and reciprocal-based division (a few
 //                   ulp); FMA contraction allowed.
 #pragma once
@@ -22,7 +23,9 @@
 
 namespace xcape {
 
-enum { MATH_ACCURATE = 0, MATH_FAST = 1 };
+// MATH_LIB: the accurate mode's arithmetic done by the device library
+// (IEEE division, expf/logf/powf); the fallback of its faster kernels below.
+enum { MATH_ACCURATE = 0, MATH_FAST = 1, MATH_LIB = 3 };
 
 template <typename T> struct Phys;
 
@@ -58,15 +61,140 @@ template <> struct Phys<double> {
     static constexpr double converge = 0.0002;
 };
 
+// ---- device kernels of the accurate mode -----------------------------------
+//
+// Division: the IEEE quotient from the hardware reciprocal and Markstein's
+// corrections -- y = 1/b refined by one Newton step, q = a y corrected twice
+// by the exact residual a - b q (fma).  These are the steps of the compiler's
+// own IEEE division without its operand scaling and special-value fix-up, so
+// where that scaling is the identity -- a, b and a/b normal, |b| < 2^126 --
+// the result is the same correctly rounded quotient, bit for bit.  A zero,
+// infinite or NaN operand gives NaN; subnormal or huge operands give a wrong
+// value.  Callers make sure of the range (the fixed-point evaluation below
+// re-evaluates with IEEE division whenever its result is not finite).
+__device__ __forceinline__ float div_raw(float a, float b)
+{
+    const float y0 = __builtin_amdgcn_rcpf(b);
+    const float e = __builtin_fmaf(-b, y0, 1.0f);
+    const float y = __builtin_fmaf(e, y0, y0);
+    const float q0 = a * y;
+    const float r0 = __builtin_fmaf(-b, q0, a);
+    const float q1 = __builtin_fmaf(r0, y, q0);
+    const float r1 = __builtin_fmaf(-b, q1, a);
+    return __builtin_fmaf(r1, y, q1);
+}
+
+// Two quotients over one divisor, sharing the refined reciprocal (each
+// quotient's steps are div_raw's).
+__device__ __forceinline__ void div2_raw(float a1, float a2, float b, float &q1o, float &q2o)
+{
+    const float y0 = __builtin_amdgcn_rcpf(b);
+    const float e = __builtin_fmaf(-b, y0, 1.0f);
+    const float y = __builtin_fmaf(e, y0, y0);
+    float q = a1 * y;
+    float r = __builtin_fmaf(-b, q, a1);
+    q = __builtin_fmaf(r, y, q);
+    r = __builtin_fmaf(-b, q, a1);
+    q1o = __builtin_fmaf(r, y, q);
+    q = a2 * y;
+    r = __builtin_fmaf(-b, q, a2);
+    q = __builtin_fmaf(r, y, q);
+    r = __builtin_fmaf(-b, q, a2);
+    q2o = __builtin_fmaf(r, y, q);
+}
+
+// a / c for a divisor fixed per call, with rc = RN(1/c) computed once:
+// q0 = RN(a rc) is within an ulp of a/c, and one correction with the exact
+// residual gives RN(a/c) (Markstein's theorem for a correctly rounded
+// reciprocal), for finite a with a normal quotient -- checked against IEEE
+// division over every float of magnitude 1e-30 to 1e30 for the divisors 40,
+// 100, 500, 1000, 2500 and others: all agree but -0, which gives +0.
+__device__ __forceinline__ float div_const(float a, float c, float rc)
+{
+    const float q0 = a * rc;
+    const float r = __builtin_fmaf(-c, q0, a);
+    return __builtin_fmaf(r, rc, q0);
+}
+
+// exp(x): the device library's reduction -- x log2(e) as a double-float,
+// its nearest integer n split off exactly, 2^f by the hardware exp2 on
+// |f| <= 1/2, scaled by 2^n -- without the library's overflow and underflow
+// selects: the library's value for -87 < x < 88 (a NaN stays NaN).
+__device__ __forceinline__ float exp_core(float x)
+{
+    // log2(e) as the double-float the library uses (its bit patterns
+    // 0x3fb8aa3b, 0x32a5705f)
+    constexpr float L2E = 0x1.715476p+0f;
+    constexpr float L2E_LO = 0x1.4ae0bep-26f;
+    const float ph = x * L2E;
+    float pl = __builtin_fmaf(x, L2E, -ph);
+    pl = __builtin_fmaf(x, L2E_LO, pl);
+    const float n = __builtin_rintf(ph);
+    const float f = (ph - n) + pl;
+    return __builtin_ldexpf(__builtin_amdgcn_exp2f(f), (int)n);
+}
+
+// x^y for 0 < x < 2^126, |y| <= 1 (the Exner function and theta-e's power):
+// x = m 2^k with m in [sqrt(1/2), sqrt(2)); log2(m) from the hardware log2
+// (|log2 m| <= 1/2); y (k + log2 m) as a double-float (exact products by
+// fma, a Fast2Sum: |y k| >= |y log2 m| whenever k != 0); 2^t as in exp_core.
+// Its error is the hardware exp2's plus about 0.15 ulp; other operands take
+// the device library's powf.
+__device__ __forceinline__ float pow_core(float x, float y)
+{
+    if (__builtin_expect(!(x > 0.0f && x < 8.5e37f), 0)) return powf(x, y);
+    float m = __builtin_amdgcn_frexp_mantf(x);          // [1/2, 1)
+    int k = __builtin_amdgcn_frexp_expf(x);
+    if (m < 0.70710678118654752f) {
+        m = m + m;
+        k = k - 1;
+    }
+    const float l = __builtin_amdgcn_logf(m);           // log2(m)
+    const float kf = (float)k;
+    const float ah = y * kf;
+    const float al = __builtin_fmaf(y, kf, -ah);
+    const float bh = y * l;
+    const float bl = __builtin_fmaf(y, l, -bh);
+    const float s = ah + bh;
+    const float se = bh - (s - ah);
+    const float lo = se + (al + bl);
+    const float n = __builtin_rintf(s);
+    const float f = (s - n) + lo;
+    return __builtin_ldexpf(__builtin_amdgcn_exp2f(f), (int)n);
+}
+
 // ---- elementary functions per precision and mode ---------------------------
 template <typename T, int MATH> struct Fn;
 
+// Accurate mode: IEEE-rounded + - * / (the translation units compile with
+// -ffp-contract=off), so those round as the reference's do; exp and log the
+// device library's, pow the kernel above (within an ulp; the host build: the
+// C library's).  The fixed-point evaluation has a faster path of its own
+// (fp_eval in cape_column.h).
 template <> struct Fn<float, MATH_ACCURATE> {
+    static XC_HD __forceinline__ float exp(float x) { return expf(x); }
+    static XC_HD __forceinline__ float log(float x) { return logf(x); }
+    static XC_HD __forceinline__ float pow(float x, float y)
+    {
+#if defined(__HIP_DEVICE_COMPILE__)
+        return pow_core(x, y);
+#else
+        return powf(x, y);
+#endif
+    }
+    static XC_HD __forceinline__ float div(float a, float b) { return a / b; }
+    // exp(c * a / b) with c*a formed first, as the reference does
+    static XC_HD __forceinline__ float exp_ratio(float c, float a, float b)
+    {
+        return expf(c * a / b);
+    }
+};
+
+template <> struct Fn<float, MATH_LIB> {
     static XC_HD __forceinline__ float exp(float x) { return expf(x); }
     static XC_HD __forceinline__ float log(float x) { return logf(x); }
     static XC_HD __forceinline__ float pow(float x, float y) { return powf(x, y); }
     static XC_HD __forceinline__ float div(float a, float b) { return a / b; }
-    // exp(c * a / b) with c*a formed first, as the reference does
     static XC_HD __forceinline__ float exp_ratio(float c, float a, float b)
     {
         return expf(c * a / b);

@@ -76,7 +76,7 @@ __device__ void srh_column(const C &col, int nk, double depth, double hs, SrhOut
     double P_prev = Ps;
     double H = hs;
     TB z_prev = (TB)hs;
-    TB u_prev = col.U(0), v_prev = col.V(0);
+    TB u_prev = (TB)col.U(0), v_prev = (TB)col.V(0);
     const TB zsurf = (TB)hs;
     // SREH 0..depth (SREH_model_lev.f90:82-114) sums, over the levels up to
     // the first one above `depth` (its wind interpolated to `depth`),
@@ -92,7 +92,10 @@ __device__ void srh_column(const C &col, int nk, double depth, double hs, SrhOut
         const double tv = std_tv((double)col.Tc(k), (double)col.Tdc(k), Pk);
         H = H + std_dh(tv, tv_prev, Pk, P_prev);
         const TB z = (TB)H;
-        const TB uk = col.U(k), vk = col.V(k);
+        // Bunkers reads the winds in its own precision; the helicity in
+        // FP64 from the inputs as given (both as the reference's wrappers
+        // pass them)
+        const TB uk = (TB)col.U(k), vk = (TB)col.V(k);
 #pragma unroll
         for (int i = 0; i < 12; ++i) {
             if (z_prev <= lv[i] && z > lv[i]) {
@@ -103,7 +106,7 @@ __device__ void srh_column(const C &col, int nk, double depth, double hs, SrhOut
             }
         }
         if (!reached) {
-            double ut = (double)uk, vt = (double)vk;
+            double ut = (double)col.U(k), vt = (double)col.V(k);
             const bool top = H > depth;
             if (top) {
                 // INTERP1 (SREH_model_lev.f90:123-129): X1 = aglh(ktop) is
@@ -131,13 +134,13 @@ __device__ void srh_column(const C &col, int nk, double depth, double hs, SrhOut
         P_prev = Ps;
         H = hs;
         z_prev = (TB)hs;
-        u_prev = col.U(0); v_prev = col.V(0);
+        u_prev = (TB)col.U(0); v_prev = (TB)col.V(0);
         for (int k = 1; k < nk; ++k) {
             const double Pk = (double)col.P(k);
             const double tv = std_tv((double)col.Tc(k), (double)col.Tdc(k), Pk);
             H = H + std_dh(tv, tv_prev, Pk, P_prev);
             const TB z = (TB)H;
-            const TB uk = col.U(k), vk = col.V(k);
+            const TB uk = (TB)col.U(k), vk = (TB)col.V(k);
 #pragma unroll
             for (int i = 0; i < 12; ++i) {
                 if (z <= lv[i] && z_prev > lv[i]) {
@@ -153,7 +156,7 @@ __device__ void srh_column(const C &col, int nk, double depth, double hs, SrhOut
     }
 
     // ---- Bunkers (Bunkers_model_lev.f90:123-177), accumulators from zero ----
-    const TB u0 = col.U(0), v0 = col.V(0);
+    const TB u0 = (TB)col.U(0), v0 = (TB)col.V(0);
     TB mu = TB(0) + u0, mv = TB(0) + v0;
 #pragma unroll
     for (int i = 0; i < 12; ++i) { mu = mu + iu[i]; mv = mv + iv[i]; }
