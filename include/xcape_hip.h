@@ -133,6 +133,31 @@ int xcape_cape_f32(const float *p, int p_shared, const int32_t *start_lev,
                    int adiabat, float *cape, float *cin, int32_t *mulev, float *zmulev,
                    uint32_t flags, xcape_stats *stats, void *stream);
 
+/*
+ * xcape_cape_f32 for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc): the
+ * same arguments, computation and results, with device pointers only
+ * (XCAPE_DEVICE_PTRS required) and no stats (stats must be NULL).  It calls
+ * no hipMalloc, hipFree or other allocating routine: the staged copies of
+ * column-contiguous profiles and the warm start's records live in a scratch
+ * array of 256 MiB that the library holds on each device it runs on, from
+ * its first call there until the process ends.  Scratch per column: 12 nk
+ * bytes to stage column-contiguous sigma profiles (8 nk on shared pressure
+ * levels, none for level-major input), plus 16 nk + 96 bytes of warm-start
+ * records with XCAPE_WARM_START; e.g. 3932 bytes for 137 sigma levels with
+ * the warm start, so at most 68 269 such columns per call.  A call needing
+ * more than 256 MiB fails with XCAPE_EINVAL and a message giving its limit.
+ * Calls on one device share the array: calls that overlap in time on one
+ * device race and must be ordered by the caller (one stream, or a wait
+ * between them).  No counterpart in the reference.
+ */
+int xcape_cape_noalloc_f32(const float *p, int p_shared, const int32_t *start_lev,
+                           const float *t, const float *td,
+                           const float *ps, const float *ts, const float *tds,
+                           int64_t nk, int64_t ncol, float pinc, int source, float ml_depth,
+                           int adiabat, float *cape, float *cin, int32_t *mulev,
+                           float *zmulev, uint32_t flags, xcape_stats *stats, void *stream);
+
 /* As xcape_cape_f32, plus each column's level of free convection and
  * equilibrium level, p_lfc and p_el [hPa] (NaN for columns without positive
  * area).  No reference counterpart: getcape computes the crossing pressure

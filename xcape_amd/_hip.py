@@ -62,6 +62,8 @@ SIGNATURES = {
                             _P, _P, _P, _P, _U32, _P, _P]),
     'xcape_cape_levels_f32': (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _I64, _I64, _F, _I, _F, _I,
                                    _P, _P, _P, _P, _P, _P, _U32, _P]),
+    'xcape_cape_noalloc_f32': (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _I64, _I64, _F, _I, _F, _I,
+                                    _P, _P, _P, _P, _U32, _P, _P]),
     'xcape_cape_f64': (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _I64, _I64, _D, _I, _D, _I,
                             _P, _P, _P, _P, _U32, _P, _P]),
     'xcape_srh_f32': (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I64, _I64, _D,

@@ -106,7 +106,44 @@ void copy_stats(const unsigned long long *h, xcape_stats *stats)
 
 }  // namespace xcape
 
+// ---------------------------------------------------------------------------
+// Scratch of xcape_cape_noalloc_f32: one file-scope device array, of which
+// every device holds its own copy (at its own address) from the first use of
+// this library on it until the process ends.  The first NOALLOC_HEAD bytes
+// hold the warm ascent's column counters (set by a kernel at the start of
+// each launch), the rest the staged profiles and warm-start records of one
+// call.  Calls on one device share it: they must not overlap.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)256 << 20;
+constexpr size_t NOALLOC_HEAD = 256;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
 namespace {
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_noalloc_mu;
+unsigned char *g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// The current device's copy of `scratch` (looked up once per device, with the
+// device current; the host-side name is not a device address).
+int noalloc_scratch(unsigned char **out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+        return fail(XCAPE_ENODEV, "device %d beyond the %d this entry point serves", dev,
+                    NOALLOC_MAX_DEVICES);
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void *ptr = nullptr;
+        if ((e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch))) != hipSuccess)
+            return hip_fail(e, "hipGetSymbolAddress(scratch)");
+        if (!ptr) return fail(XCAPE_EHIP, "hipGetSymbolAddress(scratch) gave NULL");
+        g_noalloc_ptr[dev] = static_cast<unsigned char *>(ptr);
+    }
+    *out = g_noalloc_ptr[dev];
+    return XCAPE_OK;
+}
 
 int check_common(int64_t nk, int64_t ncol)
 {
@@ -329,6 +366,8 @@ struct GridCall {
     uint32_t flags = 0;
     xcape_stats *stats = nullptr;
     void *stream = nullptr;
+    // the no-allocation entry point's scratch (device address), or nullptr
+    unsigned char *noalloc = nullptr;
 };
 
 // ---------------------------------------------------------------------------
@@ -759,6 +798,9 @@ int grid_entry(const GridCall &c)
     if ((rc = ensure_device())) return rc;
     if (c.ncol == 0) return XCAPE_OK;
     const bool dev = c.flags & XCAPE_DEVICE_PTRS;
+    if (c.noalloc && (!dev || c.stats || !want_cape || want_srh || c.fp64 || c.p_lfc))
+        return fail(XCAPE_EINVAL, "the no-allocation entry point takes device pointers, "
+                                  "computes CAPE only and collects no stats");
     // NULL stream: HIP's default stream for device-pointer calls (the
     // caller's work on it is ordered before ours), the calling thread's own
     // stream for host-pointer calls (synchronous anyway; keeps concurrent
@@ -849,6 +891,22 @@ int grid_entry(const GridCall &c)
         XC_TRY(zero_counters(dstats_.ptr, ST_COUNT, s));
         a.stats = dstats_.ptr;
     }
+    if (c.noalloc) {
+        // the whole call's scratch in the caller-independent array: the
+        // warm ascent's counters first, then the staged profiles and records
+        const size_t per = (size_t)grid_scratch_floats(a, true, false, math) * sizeof(float);
+        const size_t need = NOALLOC_HEAD + per * (size_t)c.ncol;
+        if (need > NOALLOC_SCRATCH_BYTES)
+            return fail(XCAPE_EINVAL,
+                        "xcape_cape_noalloc_f32: %lld columns of %lld levels need %zu bytes of "
+                        "scratch; the limit is %zu (at most %zu such columns per call)",
+                        (long long)c.ncol, (long long)c.nk, need, NOALLOC_SCRATCH_BYTES,
+                        (NOALLOC_SCRATCH_BYTES - NOALLOC_HEAD) / per);
+        e = run_grid(a, want_cape, want_srh, math, s,
+                     reinterpret_cast<float *>(c.noalloc + NOALLOC_HEAD),
+                     reinterpret_cast<unsigned int *>(c.noalloc));
+        return e != hipSuccess ? hip_fail(e, "kernel launch") : XCAPE_OK;
+    }
     if ((e = run_grid(a, want_cape, want_srh, math, s)) != hipSuccess)
         return hip_fail(e, "kernel launch");
     if (!dev) {
@@ -934,6 +992,30 @@ int xcape_cape_f32(const float *p, int p_shared, const int32_t *start_lev, const
     c.adiabat = adiabat;
     c.cape = cape; c.cin = cin; c.mulev = mulev; c.zout = zmulev;
     c.flags = flags; c.stats = stats; c.stream = stream;
+    return grid_entry(c);
+}
+
+int xcape_cape_noalloc_f32(const float *p, int p_shared, const int32_t *start_lev,
+                           const float *t, const float *td, const float *ps, const float *ts,
+                           const float *tds, int64_t nk, int64_t ncol, float pinc, int source,
+                           float ml_depth, int adiabat, float *cape, float *cin,
+                           int32_t *mulev, float *zmulev, uint32_t flags, xcape_stats *stats,
+                           void *stream)
+{
+    GridCall c;
+    c.kind = GRID_CAPE;
+    c.p = p; c.p_shared = p_shared; c.start = start_lev;
+    c.t = t; c.td = td; c.ps = ps; c.ts = ts; c.tds = tds;
+    c.nk = nk; c.ncol = ncol; c.pinc = pinc; c.source = source; c.ml_depth = ml_depth;
+    c.adiabat = adiabat;
+    c.cape = cape; c.cin = cin; c.mulev = mulev; c.zout = zmulev;
+    c.flags = flags; c.stats = stats; c.stream = stream;
+    if (!(flags & XCAPE_DEVICE_PTRS))
+        return fail(XCAPE_EINVAL, "xcape_cape_noalloc_f32 takes device pointers only "
+                                  "(XCAPE_DEVICE_PTRS)");
+    if (stats) return fail(XCAPE_EINVAL, "xcape_cape_noalloc_f32 collects no stats");
+    int rc = ensure_device();
+    if (rc || (rc = noalloc_scratch(&c.noalloc))) return rc;
     return grid_entry(c);
 }
 
