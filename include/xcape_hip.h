@@ -82,6 +82,15 @@ typedef struct xcape_stats {
     int64_t first_substep_iterations; /* iterations of the ascents' first
                                          sub-steps (where no history exists) */
     int64_t reserved[1];
+    /* The warm start's loop (XCAPE_WARM_START; zero for the reference's
+     * loops).  Every trip of a lane runs some of six blocks: 0 level set-up,
+     * 1 sub-step set-up, 2 first and 3 second iteration of the trip,
+     * 4 sub-step completion, 5 layer update.  A wavefront runs a block in a
+     * trip when any of its lanes needs it, so block_lanes[b] /
+     * (64 block_waves[b]) is the block's lane utilisation. */
+    int64_t wave_trips;       /* trips of the wavefronts                       */
+    int64_t block_lanes[6];   /* lanes that ran block b                        */
+    int64_t block_waves[6];   /* wave trips in which some lane ran block b     */
 } xcape_stats;
 
 const char *xcape_version(void);
@@ -143,9 +152,9 @@ int xcape_cape_f32(const float *p, int p_shared, const int32_t *start_lev,
  * array of 256 MiB that the library holds on each device it runs on, from
  * its first call there until the process ends.  Scratch per column: 12 nk
  * bytes to stage column-contiguous sigma profiles (8 nk on shared pressure
- * levels, none for level-major input), plus 16 nk + 96 bytes of warm-start
- * records with XCAPE_WARM_START; e.g. 3932 bytes for 137 sigma levels with
- * the warm start, so at most 68 269 such columns per call.  A call needing
+ * levels, none for level-major input), plus 16 nk + 80 bytes of warm-start
+ * records with XCAPE_WARM_START; e.g. 3916 bytes for 137 sigma levels with
+ * the warm start, so at most 68 548 such columns per call.  A call needing
  * more than 256 MiB fails with XCAPE_EINVAL and a message giving its limit.
  * Calls on one device share the array: calls that overlap in time on one
  * device race and must be ordered by the caller (one stream, or a wait

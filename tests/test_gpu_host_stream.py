@@ -42,7 +42,13 @@ def _same(h, d):
     for a, b in zip(h, d):
         b = _host(b)
         if isinstance(a, dict):
-            assert a == b
+            # the work counters; the warm loop's wave-trip counts depend on
+            # how the columns were spread over the lanes (chunks differ)
+            def lanes(x):
+                d = {k: v for k, v in x.items() if k not in ('wave_trips', 'blocks')}
+                d.update({k: v['lanes'] for k, v in x.get('blocks', {}).items()})
+                return d
+            assert lanes(a) == lanes(b)
         else:
             np.testing.assert_array_equal(a, b)
 

@@ -70,19 +70,42 @@ def test_noalloc_matches_allocating_entry(env, grid, warm):
 
 
 def test_noalloc_rejects_calls_beyond_its_scratch(env):
+    """A call needing more than the scratch array fails with a message that
+    names its limit, writing nothing; a call of exactly that many columns
+    runs and equals the allocating entry point's."""
+    import re
     torch, H, S = env
     L = H.lib()
     stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
-    ncol, nk = 70_000, 137          # 3932 bytes per column: above 256 MiB
+    flags = H._warm_flags('extrapolation')
+    nk = 137
+    ncol = 80_000          # ~3.9 KB of scratch per column: above 256 MiB
     c = S.sigma_columns(ncol, nk, 73)
     args = [torch.from_numpy(c[k]).cuda() for k in ('p', 't', 'td', 'ps', 'ts', 'tds')]
-    rc, outs = _call(H, L.xcape_cape_noalloc_f32, args, 0, nk, ncol, 2, 3,
-                     H._warm_flags('extrapolation'), stream)
+    print(f'[noalloc] limit probe ncol={ncol}', flush=True)
+    rc, outs = _call(H, L.xcape_cape_noalloc_f32, args, 0, nk, ncol, 2, 3, flags, stream)
     torch.cuda.synchronize()
     assert rc == -1
-    msg = L.xcape_last_error()
-    assert b'scratch' in msg and b'68269' in msg, msg
+    msg = L.xcape_last_error().decode()
+    m = re.search(r'at most (\d+) such columns', msg)
+    assert 'scratch' in msg and m, msg
+    limit = int(m.group(1))
+    assert 60_000 < limit < ncol
     assert bool((outs[0] == -7.0).all())
+    # exactly the limit: runs, and equals xcape_cape_f32
+    sub = [a[:limit] for a in args]
+    print(f'[noalloc] at the limit ncol={limit}', flush=True)
+    rc, got = _call(H, L.xcape_cape_noalloc_f32, sub, 0, nk, limit, 2, 3, flags, stream)
+    H.check(rc)
+    rc, ref = _call(H, L.xcape_cape_f32, sub, 0, nk, limit, 2, 3, flags, stream)
+    H.check(rc)
+    torch.cuda.synchronize()
+    for g, r in zip(got, ref):
+        assert torch.equal(g, r)
+    # one more column: refused
+    sub = [a[:limit + 1] for a in args]
+    rc, _ = _call(H, L.xcape_cape_noalloc_f32, sub, 0, nk, limit + 1, 2, 3, flags, stream)
+    assert rc == -1
     # host pointers and stats are refused
     x = np.zeros((2, 3), np.float32)
     s = np.zeros(2, np.float32)

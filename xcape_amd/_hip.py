@@ -37,10 +37,21 @@ class Stats(ctypes.Structure):
     _fields_ = [('columns', ctypes.c_int64), ('levels', ctypes.c_int64),
                 ('substeps', ctypes.c_int64), ('iterations', ctypes.c_int64),
                 ('nonconverged', ctypes.c_int64), ('fallbacks', ctypes.c_int64),
-                ('first_substep_iterations', ctypes.c_int64), ('reserved', ctypes.c_int64 * 1)]
+                ('first_substep_iterations', ctypes.c_int64), ('reserved', ctypes.c_int64 * 1),
+                ('wave_trips', ctypes.c_int64), ('block_lanes', ctypes.c_int64 * 6),
+                ('block_waves', ctypes.c_int64 * 6)]
+
+    BLOCKS = ('level', 'substep', 'iteration1', 'iteration2', 'completion', 'layer')
 
     def as_dict(self):
-        return {n: int(getattr(self, n)) for n, _ in self._fields_ if n != 'reserved'}
+        d = {n: int(getattr(self, n)) for n, _ in self._fields_[:7]}
+        if self.wave_trips:
+            d['wave_trips'] = int(self.wave_trips)
+            d['blocks'] = {b: {'lanes': int(self.block_lanes[i]), 'waves': int(self.block_waves[i]),
+                               'lane_utilisation': self.block_lanes[i] / (64.0 * self.block_waves[i])
+                               if self.block_waves[i] else None}
+                           for i, b in enumerate(self.BLOCKS)}
+        return d
 
 
 _lib = None

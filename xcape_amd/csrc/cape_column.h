@@ -60,6 +60,15 @@ template <typename T> struct ColumnOut {
 struct LaneStats {
     uint32_t levels = 0, substeps = 0, iterations = 0, nonconverged = 0, fallbacks = 0;
     uint32_t first = 0;   // iterations of the ascent's first sub-step
+    // the warm start's loop: the blocks this lane ran in its last trip (bit
+    // b: level set-up, sub-step set-up, first iteration, second iteration,
+    // sub-step completion, layer update), and the counts a kernel makes of
+    // them -- lanes that ran each block, wave trips in which at least one
+    // lane did (the wave pays for the block then), and wave trips
+    uint32_t blocks = 0;
+    uint32_t blk_lanes[6] = {0, 0, 0, 0, 0, 0};
+    uint32_t blk_waves[6] = {0, 0, 0, 0, 0, 0};
+    uint32_t wave_trips = 0;
 };
 
 template <typename T> struct Env {
@@ -423,17 +432,18 @@ struct NoTrack {
 // Buoyancy of the parcel at level k and the CAPE/CIN trapezoid of the layer
 // below it (:501-547).  CIN is committed only on entering positive buoyancy.
 // p_below: the pressure of level k - 1, for the tracker.
+// e_thv, e_pi, e_p: the environment of level k.
 template <typename T, int MATH, class TRK = NoTrack>
-XC_HD __forceinline__ void layer_update(T th2, T qv2, T ql2, T qi2, const Env<T> &e,
-                                             T thv_prev, T pi_prev, T b1, T &b2, T &dz,
-                                             T &cape, T &cin, T &narea, T p_below = T(0.0),
-                                             TRK *trk = nullptr)
+XC_HD __forceinline__ void layer_update(T th2, T qv2, T ql2, T qi2, T e_thv, T e_pi, T e_p,
+                                        T thv_prev, T pi_prev, T b1, T &b2, T &dz, T &cape,
+                                        T &cin, T &narea, T p_below = T(0.0),
+                                        TRK *trk = nullptr)
 {
     using F = Fn<T, MATH>;
     using K = Phys<T>;
     const T thv2 = F::div(th2 * (T(1.0) + K::reps * qv2), T(1.0) + qv2 + ql2 + qi2);
-    b2 = F::div(K::g * (thv2 - e.thv), e.thv);
-    dz = layer_dz(e.thv, thv_prev, e.pi, pi_prev);
+    b2 = F::div(K::g * (thv2 - e_thv), e_thv);
+    dz = layer_dz(e_thv, thv_prev, e_pi, pi_prev);
     // the reference's four cases (:509-545) as selects over one division:
     // each case's expressions are evaluated as written there, and the wave
     // does not run the four branches one after the other
@@ -458,7 +468,7 @@ XC_HD __forceinline__ void layer_update(T th2, T qv2, T ql2, T qi2, const Env<T>
         narea = T(0.0);
     }
     cape = cape + fmx(T(0.0), parea);
-    if (trk) trk->add(enter, leave, parea, b1, b2, p_below, e.p);
+    if (trk) trk->add(enter, leave, parea, b1, b2, p_below, e_p);
 }
 
 // sub-step count and size for the layer p_prev -> p_k (:408-415)
@@ -469,6 +479,55 @@ XC_HD __forceinline__ int substep_count(T &dp, T pinc)
     const int nloop = 1 + ftrunc(Fn<T, MATH>::div(dp, pinc));
     dp = Fn<T, MATH>::div(dp, (T)nloop);
     return nloop;
+}
+
+// A level as the ascent enters it (:403-415, and the test of :554): its
+// environment's p, pi and thv, the sub-step size dp and count nloop of the
+// layer from the level below (substep_count of the pressure drop), and
+// whether p <= 100 hPa (where negative buoyancy ends the ascent).
+template <typename T> struct Level {
+    T p, pi, thv, dp;
+    int nloop;
+    bool top;
+};
+
+// A level as an earlier pass stores it: (dp, pi, thv, nloop | stop << 31),
+// nloop clamped to [0, 2^30] (a count below 1 -- NaN or overflowing
+// pressures -- runs no sub-step either way).
+struct LevelRec {
+    float dp, pi, thv;
+    uint32_t meta;
+};
+
+XC_HD __forceinline__ LevelRec level_pack(const Level<float> &lv)
+{
+    const int n = lv.nloop < 0 ? 0 : (lv.nloop > (1 << 30) ? (1 << 30) : lv.nloop);
+    return LevelRec{lv.dp, lv.pi, lv.thv, (uint32_t)n | (lv.top ? 0x80000000u : 0u)};
+}
+
+XC_HD __forceinline__ Level<float> level_unpack(const LevelRec &r, float p)
+{
+    Level<float> lv;
+    lv.dp = r.dp;
+    lv.pi = r.pi;
+    lv.thv = r.thv;
+    lv.nloop = (int)(r.meta & 0x7fffffffu);
+    lv.top = (r.meta >> 31) != 0u;
+    lv.p = p;
+    return lv;
+}
+
+template <typename T, int MATH>
+XC_HD __forceinline__ Level<T> level_from(const Env<T> &e, T p_below, T pinc)
+{
+    Level<T> lv;
+    lv.p = e.p;
+    lv.pi = e.pi;
+    lv.thv = e.thv;
+    lv.dp = p_below - e.p;
+    lv.nloop = substep_count<T, MATH>(lv.dp, pinc);
+    lv.top = e.p <= T(10000.0);
+    return lv;
 }
 
 // ---------------------------------------------------------------------------
@@ -542,11 +601,11 @@ XC_HD void cape_column(const C &col, int nk, T pinc, int source, T ml_depth,
         }
         T dz;
         if constexpr (LEVELS)
-            layer_update<T, MATH>(th2, qv2, ql2, qi2, e, thv_prev, pi_prev, b1, b2, dz, cape,
-                                  cin, narea, p_prev, &trk);
+            layer_update<T, MATH>(th2, qv2, ql2, qi2, e.thv, e.pi, e.p, thv_prev, pi_prev, b1,
+                                  b2, dz, cape, cin, narea, p_prev, &trk);
         else
-            layer_update<T, MATH>(th2, qv2, ql2, qi2, e, thv_prev, pi_prev, b1, b2, dz, cape,
-                                  cin, narea);
+            layer_update<T, MATH>(th2, qv2, ql2, qi2, e.thv, e.pi, e.p, thv_prev, pi_prev, b1,
+                                  b2, dz, cape, cin, narea);
         if (e.p <= T(10000.0) && b2 < T(0.0)) doit = false;
         zk = zk + dz;
         zout = zk;
@@ -598,9 +657,10 @@ XC_HD void cape_column(const C &col, int nk, T pinc, int source, T ml_depth,
 // wavefront's lanes stay busy until the whole grid is drained (a wave that
 // ran 64 fixed columns would idle every lane until its longest column ended).
 //
-// Level environments come from the accessor: env_from() of its raw values,
-// or, for an accessor that declares has_env, its env(k) (levels evaluated
-// by an earlier pass, with the same env_from), loaded one level ahead.
+// Levels come from the accessor: level_from() of env_from() of its raw
+// values, or, for an accessor that declares has_env, its lev(k) (levels
+// evaluated by an earlier pass with the same functions), loaded one level
+// ahead.
 template <class C, class = void> struct HasEnv {
     static constexpr bool value = false;
 };
@@ -640,13 +700,16 @@ struct WarmAscent {
     bool layered;   // a layer is complete: the height output is zk (else -999999)
     T th2, pi2, p2, t2, qv2, b2, zk, ql2, qi2, qt;
     T cape, cin, narea;
-    // the previous level's pi and thv (the layer update needs both levels);
-    // p of the current level stays in e.p until the next level replaces it
+    // the previous level's pi and thv (the layer update needs both levels),
+    // and the current level's environment and stop test
     T pi_prev, thv_prev;
-    Env<T> e;
-    // an accessor with evaluated environments: the next level's, loaded one
-    // level ahead so that the load is in flight while this level ascends
-    Env<T> nx;
+    T e_p, e_pi, e_thv;
+    bool e_top;
+    // an accessor with evaluated levels: the next one as stored (and its
+    // pressure, for the LFC/EL), loaded a level ahead so that the load is in
+    // flight while this level ascends
+    LevelRec nx;
+    T nx_p;
     T dp;
     // sub-step state
     T p1, t1, th1, qv1, ql1, qi1, logp, thlast;
@@ -677,7 +740,7 @@ struct WarmAscent {
         cape = T(0.0); cin = T(0.0); narea = T(0.0);
         pi_prev = pc.e.pi; thv_prev = pc.e.thv;
         k = pc.kmax;
-        e = pc.e;
+        e_p = pc.e.p; e_pi = pc.e.pi; e_thv = pc.e.thv; e_top = false;
         dp = T(0.0);
         left = 0; it = 0; nh = 0;
         p1 = T(0.0); t1 = T(0.0); th1 = T(0.0); qv1 = T(0.0); ql1 = T(0.0); qi1 = T(0.0);
@@ -695,9 +758,13 @@ struct WarmAscent {
         return k < nk - 1;
     }
 
-    // The environment of the first level above the parcel, for accessors
-    // with evaluated environments (call after init()).
-    XC_HD __forceinline__ void set_next(const Env<T> &e1) { nx = e1; }
+    // The first level above the parcel, for accessors with evaluated levels
+    // (call after init()).
+    XC_HD __forceinline__ void set_next(const LevelRec &l1, T p1)
+    {
+        nx = l1;
+        nx_p = p1;
+    }
 
     XC_HD __forceinline__ void result(ColumnOut<T> &out) const
     {
@@ -728,24 +795,31 @@ struct WarmAscent {
         // iterate-again path into an inner loop (which would make the lanes
         // that converged wait for the slowest one again).
         xc_opaque(mode);
+        st.blocks = 0;
         if (mode == M_LEVEL) {
+            st.blocks |= 1u;
             k = k + 1;
-            const T p_prev = e.p;
-            if constexpr (LEVELS) pb = p_prev;
+            if constexpr (LEVELS) pb = e_p;
+            Level<T> lv;
             if constexpr (HasEnv<C>::value) {
-                e = nx;
-                if (k + 1 < nk) nx = col.env(k + 1);
+                lv = level_unpack(nx, nx_p);
+                if (k + 1 < nk) {
+                    nx = col.rec_at(k + 1);
+                    if constexpr (LEVELS) nx_p = col.p_at(k + 1);
+                }
             } else {
-                e = env_from<T, MATH>(col.raw_above(k));
+                lv = level_from<T, MATH>(env_from<T, MATH>(col.raw_above(k)), e_p, pinc);
             }
-            dp = p_prev - e.p;
-            left = substep_count<T, MATH>(dp, pinc);
+            e_p = lv.p; e_pi = lv.pi; e_thv = lv.thv; e_top = lv.top;
+            dp = lv.dp;
+            left = lv.nloop;
             st.levels++;
             // zero sub-steps (NaN/overflow in the step count): the reference's
             // loop body does not run; straight to the buoyancy update
             mode = left >= 1 ? M_SUB : M_LAYER;
         }
         if (mode == M_SUB) {
+            st.blocks |= 2u;
             left = left - 1;
             p1 = p2; t1 = t2; th1 = th2; qv1 = qv2;
             ql1 = PSEUDO ? T(0.0) : ql2;
@@ -784,6 +858,7 @@ struct WarmAscent {
 #pragma unroll 1
         for (int rep = 0; rep < XC_ITERS_PER_TRIP; ++rep) {
             if (mode != M_ITER) continue;
+            st.blocks |= 4u << rep;
             it = it + 1;
             const T x = thlast;
             fp_eval<T, MATH, ICE>(p2, pi2, logp, t1, th1, qv1, ql1, qi1, qt, x, th2, t2, qv2,
@@ -835,6 +910,7 @@ struct WarmAscent {
                 }
             }
         }
+        if (mode == M_CONV) st.blocks |= 16u;
         if (mode == M_CONV && warm) {
             // Would the reference's own solve, from the cold start th1 with
             // its relaxed step (contraction c = 1 - 0.3 (1 - f') per
@@ -905,19 +981,20 @@ struct WarmAscent {
             mode = left > 0 ? M_SUB : M_LAYER;
         }
         if (mode == M_LAYER) {
+            st.blocks |= 32u;
             // level complete (:501-558)
             T dz;
             const T b1 = b2;
             if constexpr (LEVELS)
-                layer_update<T, MATH>(th2, qv2, ql2, qi2, e, thv_prev, pi_prev, b1, b2, dz,
-                                      cape, cin, narea, pb, &trk);
+                layer_update<T, MATH>(th2, qv2, ql2, qi2, e_thv, e_pi, e_p, thv_prev, pi_prev,
+                                      b1, b2, dz, cape, cin, narea, pb, &trk);
             else
-                layer_update<T, MATH>(th2, qv2, ql2, qi2, e, thv_prev, pi_prev, b1, b2, dz,
-                                      cape, cin, narea);
+                layer_update<T, MATH>(th2, qv2, ql2, qi2, e_thv, e_pi, e_p, thv_prev, pi_prev,
+                                      b1, b2, dz, cape, cin, narea);
             zk = zk + dz;
             layered = true;
-            pi_prev = e.pi; thv_prev = e.thv;
-            if ((e.p <= T(10000.0) && b2 < T(0.0)) || !(k < nk - 1)) return true;
+            pi_prev = e_pi; thv_prev = e_thv;
+            if ((e_top && b2 < T(0.0)) || !(k < nk - 1)) return true;
             mode = M_LEVEL;
         }
         return false;

@@ -77,10 +77,15 @@ __device__ __forceinline__ int start_level(const TI *p1d, int nk_in, TI ps)
 __device__ __forceinline__ void block_stats_add(const LaneStats &st, bool ran,
                                                 unsigned long long *stats)
 {
-    constexpr int NC = 7;
+    constexpr int NC = ST_USED;
     __shared__ unsigned long long red[NC][16];
     unsigned long long v[NC] = {(unsigned long long)ran, st.levels, st.substeps, st.iterations,
-                                st.nonconverged, st.fallbacks, st.first};
+                                st.nonconverged, st.fallbacks, st.first, st.wave_trips};
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+        v[ST_BLK_LANES + b] = st.blk_lanes[b];
+        v[ST_BLK_WAVES + b] = st.blk_waves[b];
+    }
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
@@ -150,28 +155,6 @@ __device__ __forceinline__ int64_t out_col(const CapeGridArgs &a, int64_t col)
     return a.out_index ? (int64_t)a.out_index[col] : col;
 }
 
-// First-sub-step inputs of the surrogate for a parcel: ln p2, the parcel's
-// theta-e at its start (the reference's parcel formula, getthe(p, t, t, qv),
-// :505) and its total water, for the sub-step p -> p - dp (p = pc.e.p) of the
-// first ascent level (:408-429); lp1 = ln(p2/p1) of that sub-step (0: none).
-template <typename T, class C>
-__device__ __forceinline__ void surrogate_inputs(const C &c, int nk, const Parcel<T> &pc,
-                                                 float pinc, float &f0, float &f1, float &f2,
-                                                 T &lp1)
-{
-    using F = Fn<T, XC_MATH>;
-    const Env<T> e1 = env_from<T, XC_MATH>(c.raw_above(pc.kmax + 1));
-    T dp = pc.e.p - e1.p;
-    const int nl = substep_count<T, XC_MATH>(dp, T(pinc));
-    const T p2 = pc.e.p - dp;
-    if (nl >= 1 && p2 > T(0.0)) {
-        lp1 = F::log(F::div(p2, pc.e.p));
-        f0 = (float)F::log(p2);
-        f1 = (float)thetae<T, XC_MATH>(pc.e.p, pc.t2, pc.t2, pc.qv2);
-        f2 = (float)pc.qv2;
-    }
-}
-
 // CAPE/CIN with the reference's loops and cold start (cape_column), one
 // column per lane.  STATS: accumulate the work counters into a.stats (a
 // separate instantiation, so that the counters do not occupy registers in
@@ -213,32 +196,35 @@ __global__ void __launch_bounds__(256) cape_grid_kernel(CapeGridArgs a)
 // Warm-started CAPE/CIN: two launches.
 //
 // warm_setup_kernel, one column per lane, every lane of a wave on the same
-// level: the parcel source (parcel_source), the surrogate's inputs for the
-// first sub-step (SUR), and the environment (p, pi, thv) of every level from
-// the parcel's up, written
-// level-major to a.env as 16-byte records (level k of column j at record
-// k * ncol + j, k = 0 the surface), so that a lane's level is one load of
-// one cache line.  Columns with nothing to ascend get their final outputs
-// here.
-//
-// surrogate_guess_kernel (SUR): the first sub-step's guesses, one wave-wide
-// BF16 MFMA evaluation of the surrogate per 64 columns.  A launch of its own,
-// so that the evaluation's registers do not lower the setup pass's
-// occupancy.
+// level: the parcel source (parcel_source), every level from the parcel's up
+// as the ascent enters it (Level: pi, thv, the sub-step size and count of
+// the layer below it, the 100 hPa stop test), written level-major to a.env
+// as 16-byte records (level k of column j at record k * ncol + j, k = 0 the
+// surface), so that a lane's level is one load of one cache line; and, with
+// a loaded surrogate (SUR), its first iterate for the ascent's first
+// sub-step, one wave-wide BF16 MFMA evaluation per 64 columns at the end of
+// the pass.  Columns with nothing to ascend get their final outputs here.
 //
 // warm_ascent_kernel, persistent: each lane takes a column from the a.work
 // counters, runs the WarmAscent state machine on it trip by trip, writes its
 // outputs when it ends and takes the next column, until the counter passes
 // ncol.  A level's set-up is then one load, issued a level ahead: the
-// environment's pow, exp and divisions ran in the setup pass with every lane
-// on the same level, instead of under the mask of the few lanes that reach a
-// new level in a given trip.
+// environment's pow, exp and divisions and the sub-step count's divisions
+// ran in the setup pass with every lane on the same level, instead of under
+// the mask of the few lanes that reach a new level in a given trip.
 // ---------------------------------------------------------------------------
 
-// Level environments of one column evaluated by warm_setup_kernel.
-struct EnvRec {
-    float p, pi, thv;   // of a 16-byte record (the fourth word unused)
-};
+// Level records as float4 loads and stores (the layout of LevelRec).
+__device__ __forceinline__ float4 level_rec(const Level<float> &lv)
+{
+    const LevelRec r = level_pack(lv);
+    return make_float4(r.dp, r.pi, r.thv, __uint_as_float(r.meta));
+}
+
+__device__ __forceinline__ LevelRec level_rec_of(const float4 r)
+{
+    return LevelRec{r.x, r.y, r.z, __float_as_uint(r.w)};
+}
 
 // A column's parcel as the setup pass leaves it for the ascent: 64 bytes, four
 // 16-byte loads with no dependence between them.  kmax = -1: nothing to
@@ -248,24 +234,19 @@ struct ParcelRec {
     float th2, t2;
     float qv2, b2, zk, guess;
     float e_p, e_pi, e_thv, pad0;     // the environment at kmax
-    float n_p, n_pi, n_thv, pad1;     // and at kmax + 1
+    float4 next;                      // the level record of kmax + 1
 };
 static_assert(sizeof(ParcelRec) == 64, "parcel record layout");
 
 struct EnvCol {
     static constexpr bool has_env = true;
-    const float *rec;
+    const float *rec, *p;
     uint32_t col, ls;
-    __device__ __forceinline__ Env<float> env(int k) const
+    __device__ __forceinline__ LevelRec rec_at(int k) const
     {
-        const EnvRec r =
-            *reinterpret_cast<const EnvRec *>(rec + 4 * (size_t)((uint32_t)k * ls + col));
-        Env<float> e{};
-        e.p = r.p;
-        e.pi = r.pi;
-        e.thv = r.thv;
-        return e;
+        return level_rec_of(reinterpret_cast<const float4 *>(rec)[(uint32_t)k * ls + col]);
     }
+    __device__ __forceinline__ float p_at(int k) const { return p[(uint32_t)k * ls + col]; }
 };
 
 template <bool ICE, bool PSEUDO, bool SUR, bool STATS>
@@ -273,6 +254,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))
 warm_setup_kernel(CapeGridArgs a)
 {
     using T = float;
+    using F = Fn<T, XC_MATH>;
     const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     SigmaCol<uint64_t> c{};
     int nk = 0;
@@ -280,57 +262,71 @@ warm_setup_kernel(CapeGridArgs a)
     const bool ran = in_grid && c.ts > 0.0f;
     Parcel<T> pc;
     bool ok = false;
-    T lp1 = T(0.0);
-    float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
     const uint32_t ls = (uint32_t)a.ncol, j = (uint32_t)col;
     float4 *env = reinterpret_cast<float4 *>(a.env);
+    float *envp = a.env_p;
+    const T pinc = T(a.pinc);
     // the source scan's environments are stored as it computes them; `cover`:
-    // the levels 0 .. cover-1 it has stored
+    // the levels 0 .. cover-1 it has stored, p_cov the pressure of the last
     int cover = 0;
+    T p_cov = T(NAN);
     if (ran) {
         const auto sink = [&](int k, const Env<T> &e) {
-            env[(uint32_t)k * ls + j] = make_float4(e.p, e.pi, e.thv, 0.0f);
-            if (k == cover) cover = k + 1;
+            if (k == cover) {
+                env[(uint32_t)k * ls + j] = level_rec(level_from<T, XC_MATH>(e, p_cov, pinc));
+                if (envp) envp[(uint32_t)k * ls + j] = e.p;
+                cover = k + 1;
+                p_cov = e.p;
+            }
         };
         const int source = a.src_col ? a.src_col[col] : a.source;
         ok = parcel_source<T, XC_MATH>(c, nk, source, T(a.ml_depth), pc, sink);
         a.mulvl[out_col(a, col)] = pc.mulvl;
-        if (SUR && ok && pc.kmax < nk - 1) surrogate_inputs(c, nk, pc, a.pinc, f0, f1, f2, lp1);
     }
     const bool go = ok && pc.kmax < nk - 1;
+    T lp1 = T(0.0);
+    float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
     if (go) {
         // the levels from the parcel's up that the scan did not store, by
         // absolute level so that the wave's lanes write the same row
         const int k0 = pc.kmax > cover ? pc.kmax : cover;
-        Env<T> e1{};
+        float4 r1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        T p_below = p_cov;
         for (int k = 0; k < nk; ++k) {
             if (k < k0) continue;
             const Env<T> e = env_from<T, XC_MATH>(load_raw(c, k));
-            env[(uint32_t)k * ls + j] = make_float4(e.p, e.pi, e.thv, 0.0f);
-            if (k == pc.kmax + 1) e1 = e;
+            const float4 r = level_rec(level_from<T, XC_MATH>(e, p_below, pinc));
+            env[(uint32_t)k * ls + j] = r;
+            if (envp) envp[(uint32_t)k * ls + j] = e.p;
+            if (k == pc.kmax + 1) r1 = r;
+            p_below = e.p;
         }
-        if (cover > pc.kmax + 1) {
-            // the scan stored level kmax + 1 already
-            const float4 r = env[(uint32_t)(pc.kmax + 1) * ls + j];
-            e1.p = r.x; e1.pi = r.y; e1.thv = r.z;
-        }
-        float4 *rec = reinterpret_cast<float4 *>(a.par) + 4 * col;
-        rec[0] = make_float4(__int_as_float(pc.kmax), __int_as_float(nk), pc.th2, pc.t2);
-        rec[1] = make_float4(pc.qv2, pc.b2, pc.zk, T(NAN));
-        rec[2] = make_float4(pc.e.p, pc.e.pi, pc.e.thv, 0.0f);
-        rec[3] = make_float4(e1.p, e1.pi, e1.thv, 0.0f);
+        // (the scan stored level kmax + 1 already)
+        if (cover > pc.kmax + 1) r1 = env[(uint32_t)(pc.kmax + 1) * ls + j];
+        float4 *r4 = reinterpret_cast<float4 *>(reinterpret_cast<ParcelRec *>(a.par) + col);
+        r4[0] = make_float4(__int_as_float(pc.kmax), __int_as_float(nk), pc.th2, pc.t2);
+        r4[2] = make_float4(pc.e.p, pc.e.pi, pc.e.thv, 0.0f);
+        r4[3] = r1;
         if constexpr (SUR) {
-            const int64_t n = a.ncol;
-            a.sur_in[col] = f0;
-            a.sur_in[n + col] = f1;
-            a.sur_in[2 * n + col] = f2;
-            a.sur_in[3 * n + col] = lp1;
+            // the surrogate's inputs for the ascent's first sub-step p -> p - dp
+            // (p = pc.e.p, :408-429): ln p2, the parcel's theta-e at its start
+            // (the reference's parcel formula, getthe(p, t, t, qv), :505) and
+            // its total water; lp1 = ln(p2/p1) (0: no such sub-step)
+            const Level<T> l1 = level_unpack(level_rec_of(r1), 0.0f);
+            const T p2 = pc.e.p - l1.dp;
+            if (l1.nloop >= 1 && p2 > T(0.0)) {
+                lp1 = F::log(F::div(p2, pc.e.p));
+                f0 = (float)F::log(p2);
+                f1 = (float)thetae<T, XC_MATH>(pc.e.p, pc.t2, pc.t2, pc.qv2);
+                f2 = (float)pc.qv2;
+            }
+        } else {
+            r4[1] = make_float4(pc.qv2, pc.b2, pc.zk, T(NAN));
         }
     } else if (in_grid) {
         // ts <= 0 degC: zeros; no level above the parcel (or an unknown
         // source): CAPE = CIN = 0 and the height's initial -999999
         reinterpret_cast<ParcelRec *>(a.par)[col].kmax = -1;
-        if constexpr (SUR) a.sur_in[3 * a.ncol + col] = T(0.0);
         T *cape = static_cast<T *>(a.cape), *cin = static_cast<T *>(a.cin),
           *zout = static_cast<T *>(a.zout);
         const int64_t oc = out_col(a, col);
@@ -343,31 +339,19 @@ warm_setup_kernel(CapeGridArgs a)
             a.p_el[oc] = T(NAN);
         }
     }
-    if constexpr (STATS) block_stats_add(LaneStats{}, ran, a.stats);
-}
-
-__global__ void __launch_bounds__(256) surrogate_guess_kernel(CapeGridArgs a)
-{
-    extern __shared__ uint32_t dyn_lds[];
-    const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t n = a.ncol;
-    const bool in_grid = col < n;
-    // every lane takes part in its wave's evaluation; lanes without a first
-    // sub-step have lp1 = 0 and zero inputs
-    ParcelRec *rec = reinterpret_cast<ParcelRec *>(a.par);
-    const bool go = in_grid && rec[col].kmax >= 0;
-    float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f, lp1 = 0.0f;
-    if (go) {
-        lp1 = a.sur_in[3 * n + col];
-        if (lp1 != 0.0f) {
-            f0 = a.sur_in[col];
-            f1 = a.sur_in[n + col];
-            f2 = a.sur_in[2 * n + col];
+    if constexpr (SUR) {
+        // every lane of the block takes part in its wave's evaluation (lanes
+        // without a first sub-step give zero inputs and drop the result)
+        extern __shared__ uint32_t dyn_lds[];
+        const float r = surrogate_eval_wave(static_cast<const SurrogateDev *>(a.surrogate), f0,
+                                            f1, f2,
+                                            dyn_lds + (threadIdx.x >> 6) * MLP_LDS_WORDS_PER_WAVE);
+        if (go) {
+            float4 *r4 = reinterpret_cast<float4 *>(reinterpret_cast<ParcelRec *>(a.par) + col);
+            r4[1] = make_float4(pc.qv2, pc.b2, pc.zk, lp1 != T(0.0) ? pc.th2 + r * lp1 : T(NAN));
         }
     }
-    const float r = surrogate_eval_wave(static_cast<const SurrogateDev *>(a.surrogate), f0, f1,
-                                        f2, dyn_lds + (threadIdx.x >> 6) * MLP_LDS_WORDS_PER_WAVE);
-    if (go) rec[col].guess = lp1 != 0.0f ? rec[col].th2 + r * lp1 : NAN;
+    if constexpr (STATS) block_stats_add(LaneStats{}, ran, a.stats);
 }
 
 // The ascent's columns are split into `parts` contiguous parts with a counter
@@ -450,11 +434,8 @@ warm_ascent_kernel(CapeGridArgs a)
                         // the setup pass queued only columns with a level
                         // above the parcel: init() succeeds
                         have = w.init(pc, __float_as_int(r0.y), r1.w);
-                        Env<T> e1{};
-                        e1.p = r3.x;
-                        e1.pi = r3.y;
-                        e1.thv = r3.z;
-                        w.set_next(e1);
+                        w.set_next(level_rec_of(r3),
+                                   LEVELS ? a.env_p[(size_t)(kmax + 1) * ncol + col] : 0.0f);
                     }
                     need = !have;
                 }
@@ -465,9 +446,15 @@ warm_ascent_kernel(CapeGridArgs a)
         // the wave ends when every lane has run out of columns
         if (drained) need = false;
         if (__ballot(have || need) == 0) break;
+        st.blocks = 0;
         if (have) {
-            const EnvCol ec{a.env, col, ncol};
-            if (w.trip(ec, T(a.pinc), T(a.fallback_tol), st, PREDICT)) {
+            const EnvCol ec{a.env, a.env_p, col, ncol};
+            const bool done = w.trip(ec, T(a.pinc), T(a.fallback_tol), st, PREDICT);
+            if constexpr (STATS) {
+#pragma unroll
+                for (int b = 0; b < 6; ++b) st.blk_lanes[b] += (st.blocks >> b) & 1u;
+            }
+            if (done) {
                 ColumnOut<T> o;
                 w.result(o);
                 const int64_t oc = out_col(a, col);
@@ -480,6 +467,17 @@ warm_ascent_kernel(CapeGridArgs a)
                 }
                 have = false;
                 need = true;
+            }
+        }
+        if constexpr (STATS) {
+            // the wave's view of the trip: the blocks some lane ran
+            uint32_t any = 0;
+#pragma unroll
+            for (int b = 0; b < 6; ++b) any |= (__ballot((st.blocks >> b) & 1u) != 0) << b;
+            if (lane == 0) {
+                st.wave_trips++;
+#pragma unroll
+                for (int b = 0; b < 6; ++b) st.blk_waves[b] += (any >> b) & 1u;
             }
         }
     }
@@ -597,14 +595,10 @@ template <bool ICE, bool PSEUDO, bool SUR, bool PREDICT, bool STATS, bool LEVELS
 hipError_t launch_warm_s(const CapeGridArgs &a, int block, hipStream_t s)
 {
     const dim3 sgrid((unsigned)((a.ncol + 255) / 256));
-    warm_setup_kernel<ICE, PSEUDO, SUR, STATS><<<sgrid, 256, 0, s>>>(a);
+    const size_t lds = SUR ? sizeof(uint32_t) * MLP_LDS_WORDS_PER_WAVE * 4 : 0;
+    warm_setup_kernel<ICE, PSEUDO, SUR, STATS><<<sgrid, 256, lds, s>>>(a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    if constexpr (SUR) {
-        const size_t lds = sizeof(uint32_t) * MLP_LDS_WORDS_PER_WAVE * 4;
-        surrogate_guess_kernel<<<sgrid, 256, lds, s>>>(a);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-    }
     const int grid = ascent_grid<ICE, PSEUDO, PREDICT, STATS, LEVELS>(block, a.ncol);
     // the counters start at their parts' first columns, for the grid's parts
     warm_reset_kernel<<<1, 64, 0, s>>>(a.work, (uint32_t)a.ncol, (uint32_t)grid);
@@ -636,7 +630,7 @@ hipError_t launch_warm_a(const CapeGridArgs &a, int block, hipStream_t s)
 
 inline hipError_t launch_warm(const CapeGridArgs &a, int block, hipStream_t s)
 {
-    if (!a.work || !a.env || !a.par) return hipErrorInvalidValue;
+    if (!a.work || !a.env || !a.par || (a.p_lfc && !a.env_p)) return hipErrorInvalidValue;
     switch (a.adiabat) {
     case 1: return launch_warm_a<false, true>(a, block, s);
     case 2: return launch_warm_a<false, false>(a, block, s);

@@ -8,7 +8,8 @@ namespace xcape {
 
 // Device counters one launch accumulates (one atomic per block per counter).
 enum { ST_COLUMNS = 0, ST_LEVELS, ST_SUBSTEPS, ST_ITERATIONS, ST_NONCONVERGED,
-       ST_FALLBACKS, ST_FIRST, ST_COUNT = 8 };
+       ST_FALLBACKS, ST_FIRST, ST_WAVE_TRIPS, ST_BLK_LANES, ST_BLK_WAVES = ST_BLK_LANES + 6,
+       ST_USED = ST_BLK_WAVES + 6, ST_COUNT = 24 };
 
 struct CapeGridArgs {
     const float *p;        // (ncol, nk_in) sigma grid, or nullptr
@@ -42,19 +43,23 @@ struct CapeGridArgs {
                                  // loop; 2: the same loop, every sub-step cold
     float fallback_tol;          // first-iteration residual that rejects a guess [K]
     const void *surrogate;       // device MLP weights (or nullptr)
-    // scratch of the warm-started ascent (warm != 0): the environment of
-    // every level from each parcel's up as (p, pi, thv, unused) records,
-    // (nk_in + 1, ncol) level-major (level 0 the surface; (nk_in + 1) * ncol
-    // < 2^32); one 64-byte parcel record per column (ParcelRec); the
-    // surrogate's inputs, (4, ncol); and the ascent's column counters
+    // scratch of the warm-started ascent (warm != 0): every level from each
+    // parcel's up as a 16-byte record (dp, pi, thv, nloop | stop), (nk_in + 1,
+    // ncol) level-major (level 0 the surface; (nk_in + 1) * ncol < 2^32);
+    // for the LFC/EL calls the levels' pressures in a plane of the same
+    // shape; one 64-byte parcel record per column (ParcelRec); and the
+    // ascent's column counters
     float *env;                  // 4 floats per record
+    float *env_p;                // 1 float per level (LFC/EL calls), or nullptr
     float *par;                  // 16 floats per column
-    float *sur_in;               // f0, f1, f2 and ln(p2/p1) of the first sub-step
     unsigned int *work;          // 8 counters
 };
 
-// floats of warm-start scratch per column (env, par and sur_in)
-inline int64_t warm_scratch_floats(int64_t nk_in) { return 4 * (nk_in + 1) + 20; }
+// floats of warm-start scratch per column (env, env_p when `levels`, par)
+inline int64_t warm_scratch_floats(int64_t nk_in, bool levels)
+{
+    return 4 * (nk_in + 1) + (levels ? nk_in + 1 : 0) + 16;
+}
 
 // kind of work a grid launch does
 enum { GRID_CAPE = 1, GRID_SRH = 2, GRID_CAPE_SRH = 3 };

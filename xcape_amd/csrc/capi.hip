@@ -102,6 +102,11 @@ void copy_stats(const unsigned long long *h, xcape_stats *stats)
     stats->fallbacks = (int64_t)h[ST_FALLBACKS];
     stats->first_substep_iterations = (int64_t)h[ST_FIRST];
     stats->reserved[0] = 0;
+    stats->wave_trips = (int64_t)h[ST_WAVE_TRIPS];
+    for (int b = 0; b < 6; ++b) {
+        stats->block_lanes[b] = (int64_t)h[ST_BLK_LANES + b];
+        stats->block_waves[b] = (int64_t)h[ST_BLK_WAVES + b];
+    }
 }
 
 }  // namespace xcape
@@ -236,7 +241,8 @@ int64_t grid_scratch_floats(const CapeGridArgs &a0, bool cape, bool srh, int mat
     const bool stage = a0.ls == 1 && a0.nk_in > 1 && !a0.in_f64;
     const bool warm = cape && a0.warm != 0 && math != 2;
     const int narr = stage ? 2 + (a0.p != nullptr) + (srh ? 2 : 0) : 0;
-    return (int64_t)narr * a0.nk_in + (warm ? warm_scratch_floats(a0.nk_in) : 0);
+    return (int64_t)narr * a0.nk_in +
+           (warm ? warm_scratch_floats(a0.nk_in, a0.p_lfc != nullptr) : 0);
 }
 
 // ext/ext_work: caller-owned scratch (grid_scratch_floats per column of the
@@ -260,7 +266,7 @@ hipError_t run_grid(const CapeGridArgs &a0, bool cape, bool srh, int math, hipSt
         if (srh) { iu = narr; src0[narr++] = a0.u; iv = narr; src0[narr++] = a0.v; }
     }
     const int64_t nk = a0.nk_in;
-    const int64_t wfl = warm ? warm_scratch_floats(nk) : 0;
+    const int64_t wfl = warm ? warm_scratch_floats(nk, a0.p_lfc != nullptr) : 0;
     const int64_t per_col = ((int64_t)narr * nk + wfl) * (int64_t)sizeof(float);
     int64_t chunk = ((int64_t)16 << 30) / per_col;
     if (chunk < ((int64_t)1 << 20)) chunk = (int64_t)1 << 20;
@@ -321,7 +327,7 @@ hipError_t run_grid(const CapeGridArgs &a0, bool cape, bool srh, int math, hipSt
             const size_t lv = (size_t)(nk + 1) * (size_t)n;
             a.env = w;
             a.par = w + 4 * lv;
-            a.sur_in = a.par + 16 * (size_t)n;
+            a.env_p = a0.p_lfc ? a.par + 16 * (size_t)n : nullptr;
             a.work = work;
         }
         if (cape) {

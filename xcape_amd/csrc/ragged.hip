@@ -467,7 +467,7 @@ int xcape_ragged_plan_create(const int64_t *offsets_host, const int32_t *source_
         size_t per_seg_floats[NSEG] = {}, total = 0;
         for (int g = 0; g < NSEG; ++g) {
             const int64_t n = pl->seg_end[g] - pl->seg_begin[g];
-            per_seg_floats[g] = n > 0 ? (size_t)warm_scratch_floats(maxnk) * (size_t)n : 0;
+            per_seg_floats[g] = n > 0 ? (size_t)warm_scratch_floats(maxnk, false) * (size_t)n : 0;
             total += per_seg_floats[g];
         }
         if ((e = hipMalloc((void **)&pl->d_warm, sizeof(float) * (total ? total : 1))) ||
@@ -498,7 +498,7 @@ int xcape_ragged_plan_create(const int64_t *offsets_host, const int32_t *source_
             const size_t lv = (size_t)(maxnk + 1) * (size_t)n;
             w.env = ws;
             w.par = ws + 4 * lv;
-            w.sur_in = w.par + 16 * (size_t)n;
+            w.env_p = nullptr;
             w.work = pl->d_work + 8 * g;
             off += per_seg_floats[g];
         }
