@@ -197,13 +197,11 @@ def pmc_summary(fetch_dir, write_dir, out, ncol, fast_math):
     kernels = {
         'plain': lambda n: 'cape_grid_kernel<float' in n,
         'warm_setup': lambda n: 'warm_setup_kernel<true, true, true, false>' in n,
-        'warm_surrogate': lambda n: 'surrogate_guess_kernel' in n,
         'warm_ascent': lambda n: 'warm_ascent_kernel<true, true, true, false, false>' in n,
     }
     res = {'ncol': ncol, 'fast_math': fast_math,
            'kernels': {'plain': 'cape_grid_kernel<float, unsigned int, true, true, false, false>',
                        'warm': 'warm_setup_kernel<true, true, true, false> + '
-                               'surrogate_guess_kernel + '
                                'warm_ascent_kernel<true, true, true, false, false>'}}
     gib = float(1 << 30)
     for label, d, counter in (('fetch', fetch_dir, 'FETCH_SIZE'), ('write', write_dir, 'WRITE_SIZE')):
@@ -216,7 +214,7 @@ def pmc_summary(fetch_dir, write_dir, out, ncol, fast_math):
             v = first(disp, pred)
             if v is not None:
                 res[f'{mode}_{label}_bytes'] = v * 1024.0 * factor
-        parts = [f'warm_{x}_{label}_bytes' for x in ('setup', 'surrogate', 'ascent')]
+        parts = [f'warm_{x}_{label}_bytes' for x in ('setup', 'ascent')]
         if all(x in res for x in parts):
             res[f'warm_{label}_bytes'] = sum(res[x] for x in parts)
     for mode in ('plain', 'warm'):
@@ -511,9 +509,9 @@ def main():
                                'motion (xcape_cape_srh_f32)',
                    'ncol': ncol, 'nlev': NLEV, 'warm_start': warm,
                    'fast_math': bool(args.fast_math), 'parallelism': 'single GPU'},
-        'roofline': {'bound': 'mfma', 'pipe': 'VALU FP32 (the ascent has no matrix work)',
-                     'kernel': ('warm_setup_kernel + surrogate_guess_kernel + '
-                                'warm_ascent_kernel (the warm-started CAPE-only call)' if warm else
+        'roofline': {'bound': 'valu', 'pipe': 'VALU FP32 (the ascent has no matrix work)',
+                     'kernel': ('warm_setup_kernel + warm_ascent_kernel (the warm-started '
+                                'CAPE-only call)' if warm else
                                 'cape_grid_kernel (the plain CAPE-only call)'),
                      'achieved': achieved, 'peak': FP32_PEAK_TFLOPS, 'unit': 'TFLOP/s',
                      'frac': achieved / FP32_PEAK_TFLOPS, 'traffic': traffic,
