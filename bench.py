@@ -17,9 +17,18 @@ Prints one JSON line:
                 measured with HIP events on the launch stream
   attribution   the CAPE-only times of the reference loops, the warm start's
                 loop with cold starts, extrapolation only, and with the surrogate
-  accuracy      CAPE/CIN/MUlev/zMUlev of a column sample against the FP32 oracle
-                and against the FP64 re-run of the GPU kernel; SRH and Bunkers
-                against the oracle
+  accuracy      CAPE/CIN/MUlev/zMUlev of a column sample against the FP32 oracle;
+                every output (CAPE, CIN, MUlev, zMUlev, LFC, EL, SRH, Bunkers)
+                against the FP64 re-run of the GPU kernels (fp64_rerun); the
+                CAPE outputs of every column the CPU baseline finished against
+                the reference's (vs_reference_cpu_sample)
+  configs       the 64x64x40 sigma grid (surface, pseudo-liquid) and the
+                1024x512x37 pressure-level grid (surface, mixed-layer,
+                most-unstable): plain and warm-started calls, their rooflines,
+                the reference on the host cores and the outputs against it
+  ragged_200k   the ragged batch: both plans autotuned (every config timed is
+                listed), rooflines, the reference grouped by (level count,
+                source, adiabat) and the outputs against it
   cpu_baseline  the reference Fortran (compiled from source, oracle/_ref) on the
                 host cores over a bounded sample of the same workload, or the C
                 restatement of the oracle when that build is absent
@@ -52,9 +61,16 @@ W_ENV, W_LEVEL, W_SUBSTEP, W_ITER_ICE, W_ITER_LIQ, W_MU = 26, 35, 6, 74, 69, 26
 PMC_FILE = os.path.join(ROOT, 'profiles', 'cape_warm_combined_2048x1024x137_pmc.json')
 
 
-def algorithmic_ops(stats, nk, mu_levels):
-    w_it = W_ITER_ICE if ADIABAT in (3, 4) else W_ITER_LIQ
-    return (W_ENV * (nk + 1) * stats['columns'] + W_MU * mu_levels + W_LEVEL * stats['levels']
+def algorithmic_ops(stats, nk, mu_levels, w_it=None, env_levels=None):
+    """Unit-weighted FP32 operations of the reference method's work counted
+    in `stats`: the environment of env_levels levels (default (nk + 1) per
+    column), the most-unstable scan over mu_levels, and the ascent's levels,
+    sub-steps and iterations (w_it per iteration; default that of ADIABAT)."""
+    if w_it is None:
+        w_it = W_ITER_ICE if ADIABAT in (3, 4) else W_ITER_LIQ
+    if env_levels is None:
+        env_levels = (nk + 1) * stats['columns']
+    return (W_ENV * env_levels + W_MU * mu_levels + W_LEVEL * stats['levels']
             + W_SUBSTEP * stats['substeps'] + w_it * stats['iterations'])
 
 
@@ -64,60 +80,124 @@ def bytes_per_column(nk):
     return 4 * (5 * nk + 5) + 4 * 4 + 2 * 8 + 3 * 8
 
 
+def reference_kind():
+    from oracle import ref_driver as R
+    return 'reference' if R.available() else 'port'
+
+
+def run_tasks(tasks, threads, seconds, fn):
+    """fn(columns) on `threads` host threads over `tasks` (arrays of column
+    indices), in order, until `seconds` of wall time have passed (ctypes
+    releases the GIL).  Returns (columns done, wall seconds, {task number:
+    (columns, fn's result)})."""
+    import threading
+    from concurrent.futures import ThreadPoolExecutor
+    nxt = [0]
+    lock = threading.Lock()
+    res = {}
+    deadline = time.perf_counter() + seconds
+
+    def worker(_):
+        done = 0
+        while time.perf_counter() < deadline:
+            with lock:
+                i = nxt[0]
+                nxt[0] += 1
+            if i >= len(tasks):
+                break
+            res[i] = (tasks[i], fn(tasks[i]))
+            done += len(tasks[i])
+        return done
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        n = sum(ex.map(worker, range(threads)))
+    return n, time.perf_counter() - t0, res
+
+
+def ranges(ncol, chunk):
+    return [np.arange(i0, min(i0 + chunk, ncol)) for i0 in range(0, ncol, chunk)]
+
+
+def gathered(res):
+    """The columns and CAPE outputs of run_tasks' results, by column."""
+    keys = sorted(res)
+    if not keys:
+        return np.zeros(0, np.int64), [np.zeros(0)] * 4
+    idx = np.concatenate([res[i][0] for i in keys])
+    outs = [np.concatenate([np.asarray(res[i][1][j]) for i in keys]) for j in range(4)]
+    order = np.argsort(idx, kind='stable')
+    return idx[order], [o[order] for o in outs]
+
+
 def cpu_baseline(host, threads, seconds):
     """Reference Fortran (or the C oracle) on `threads` host threads over
     chunks of the same input columns, until `seconds` of wall time have
-    passed; columns/s of the chunks completed (ctypes releases the GIL)."""
-    import threading
-    from concurrent.futures import ThreadPoolExecutor
-
+    passed; columns/s of the chunks completed.  Also returns the CAPE
+    outputs of the columns done (the reference side of
+    accuracy.vs_reference_cpu_sample; Bunkers runs in its timing build,
+    whose outputs are not kept)."""
     from oracle import oracle as O
     from oracle import ref_driver as R
 
-    kind = 'reference' if R.available() else 'port'
-    ncol = host['t'].shape[0]
-    chunk = 1024
-    nxt = [0]
-    lock = threading.Lock()
-    deadline = time.perf_counter() + seconds
+    kind = reference_kind()
 
-    def run_chunk(a):
+    def run_chunk(cols):
+        a = {k: v[cols[0]:cols[-1] + 1] for k, v in host.items()}
         if kind == 'reference':
-            R.loopcape_ml(a['p'].T, a['t'].T, a['td'].T, a['ps'], a['ts'], a['tds'], PINC,
-                          SOURCE, 500., ADIABAT)
+            out = R.loopcape_ml(a['p'].T, a['t'].T, a['td'].T, a['ps'], a['ts'], a['tds'], PINC,
+                                SOURCE, 500., ADIABAT)
             h, hs = R.stdheight_ml(a['p'].T, a['t'].T, a['td'].T, a['ps'], a['ts'], a['tds'], 2.)
             rm, lm, mw = R.bunkers_ml_timing(a['u'].T, a['v'].T, h.astype(np.float32), a['us'],
                                              a['vs'], hs.astype(np.float32))
             R.sreh_ml(a['u'].T, a['v'].T, h, a['us'], a['vs'], hs, rm[0], rm[1], lm[0], lm[1],
                       DEPTH)
         else:
-            O.cape(a['p'], a['t'], a['td'], a['ps'], a['ts'], a['tds'], PINC, SOURCE, 500.,
-                   ADIABAT)
+            out = O.cape(a['p'], a['t'], a['td'], a['ps'], a['ts'], a['tds'], PINC, SOURCE, 500.,
+                         ADIABAT)
             O.srh(a['p'], a['t'], a['td'], a['u'], a['v'], a['ps'], a['ts'], a['tds'], a['us'],
                   a['vs'], DEPTH)
+        return out
 
-    def worker(_):
-        done = 0
-        while time.perf_counter() < deadline:
-            with lock:
-                i0 = nxt[0]
-                nxt[0] += chunk
-            if i0 >= ncol:
-                break
-            idx = slice(i0, min(i0 + chunk, ncol))
-            run_chunk({k: v[idx] for k, v in host.items()})
-            done += idx.stop - idx.start
-        return done
-
-    t0 = time.perf_counter()
-    with ThreadPoolExecutor(max_workers=threads) as ex:
-        n = sum(ex.map(worker, range(threads)))
-    dt = time.perf_counter() - t0
+    n, dt, res = run_tasks(ranges(host['t'].shape[0], 1024), threads, seconds, run_chunk)
     src = 'reference Fortran compiled from source' if kind == 'reference' else 'C restatement'
     return {'value': n / dt, 'unit': 'columns/s', 'cores': threads, 'kind': kind,
             'sample': f'{n} columns of the same inputs (137 sigma levels, MU, pseudo-ice, '
                       f'pinc 500, + stdheight/Bunkers/SREH 0-3 km), {src}, {dt:.1f} s wall '
-                      f'on {threads} threads'}
+                      f'on {threads} threads'}, res
+
+
+def vs_reference(got, ref):
+    """A GPU run's (cape, cin, mulev, zmulev) against the reference's on the
+    same columns (host arrays): columns outside max(0.5 J/kg, 0.1 %) for CAPE
+    and for CIN, those of them whose reference CAPE is below 0.05 J/kg (a
+    sliver of positive area whose CIN commit hangs on a near-zero buoyancy
+    crossing) counted apart, columns that one run stops at the iteration
+    limit (CAPE = CIN = 0) and the other does not, MUlev equality and the
+    zMUlev error."""
+    cape, cin, mul, z = (np.asarray(x) for x in got)
+    rc, rci, rm, rz = (np.asarray(x) for x in ref)
+    res = {'columns': int(len(rc))}
+    sliver = rc < 0.05
+    stop_g = (cape == 0) & (cin == 0)
+    stop_r = (rc == 0) & (rci == 0)
+    for name, g, r in (('cape', cape, rc), ('cin', cin, rci)):
+        g, r = g.astype(np.float64), r.astype(np.float64)
+        d = np.abs(g - r)
+        both_nan = np.isnan(g) & np.isnan(r)
+        out = ~both_nan & ~(d <= np.maximum(0.5, 1e-3 * np.abs(r)))
+        res[name] = {'outside_tol': int(out.sum()),
+                     'outside_tol_sliver': int((out & sliver).sum()),
+                     'outside_tol_stop_flip': int((out & (stop_g != stop_r)).sum()),
+                     'max_abs_err': float(np.nanmax(np.where(both_nan, 0.0, d)))
+                     if len(d) else 0.0}
+    res['zero_cape_and_cin'] = {'gpu': int(stop_g.sum()), 'reference': int(stop_r.sum()),
+                      'flips': int((stop_g != stop_r).sum())}
+    res['mulev'] = {'frac_equal': float(np.mean(mul == rm)) if len(rm) else 1.0,
+                    'unequal': int((mul != rm).sum())}
+    dz = np.abs(z.astype(np.float64) - rz)
+    res['zmulev_max_abs_err'] = float(np.nanmax(dz)) if len(dz) else 0.0
+    return res
 
 
 def full_grid_agreement(outs, ref):
@@ -162,6 +242,161 @@ def accuracy(outs, inp, sample, fp64_outs):
                                                np.abs(g[8] - srh[4]).max()))}
     res['columns'] = sample
     return res
+
+
+def fp64_rerun(outs, inp, sample, warm, fast_math):
+    """Every output of the reported method on the first `sample` columns
+    against the FP64 re-run of the GPU kernels on the same FP32 inputs:
+    CAPE, CIN, MUlev, zMUlev (xcape_cape_f64), LFC and EL
+    (xcape_cape_levels_f64 against xcape_cape_levels_f32 with the same
+    method), SRH and Bunkers motion (xcape_srh_f64)."""
+    from xcape_amd import _hip
+    s = {k: v[:sample] for k, v in inp.items()}
+    ca = [s[k] for k in ('p', 't', 'td', 'ps', 'ts', 'tds')]
+    kw = dict(pinc=PINC, source=SOURCE, adiabat=ADIABAT)
+    f64 = [x.cpu().numpy() for x in _hip.cape(*ca, fp64=True, levels=True, **kw)]
+    f32 = [x.cpu().numpy() for x in _hip.cape(*ca, warm_start=warm, fast_math=fast_math,
+                                                levels=True, **kw)]
+    srh64 = [x.cpu().numpy() for x in _hip.srh(*(s[k] for k in ('p', 't', 'td', 'u', 'v', 'ps',
+                                                                 'ts', 'tds', 'us', 'vs')),
+                                                 depth=DEPTH, fp64=True)]
+    g = [x[:sample].cpu().numpy() for x in outs]
+    res = {'columns': sample}
+    for i, name in enumerate(('cape', 'cin')):
+        d = np.abs(g[i].astype(np.float64) - f64[i])
+        tol = np.maximum(0.5, 1e-3 * np.abs(f64[i]))
+        res[name] = {'max_abs_err': float(np.nanmax(d)), 'frac_within_tol': float(np.mean(d <= tol))}
+    res['mulev'] = {'frac_equal': float(np.mean(g[2] == f64[2]))}
+    res['zmulev'] = {'max_abs_err': float(np.nanmax(np.abs(g[3] - f64[3])))}
+    for i, name in ((4, 'lfc_hpa'), (5, 'el_hpa')):
+        fin = np.isfinite(f32[i]) & np.isfinite(f64[i])
+        d = np.abs(f32[i][fin].astype(np.float64) - f64[i][fin])
+        res[name] = {'max_abs_err': float(d.max()) if d.size else 0.0,
+                     'frac_within_1hpa': float(np.mean(d <= 1.0)) if d.size else 1.0,
+                     'presence_mismatch': int((np.isfinite(f32[i]) != np.isfinite(f64[i])).sum())}
+    res['srh'] = {'max_abs_err': float(max(np.abs(g[4] - srh64[0]).max(),
+                                           np.abs(g[5] - srh64[1]).max()))}
+    res['bunkers'] = {'max_abs_err': float(max(np.abs(g[j] - srh64[j - 4]).max()
+                                               for j in (6, 7, 8)))}
+    return res
+
+
+def roofline_entry(ops, ms, bytes_col, ncol):
+    """VALU (unit-weighted FP32 ops) and HBM (compulsory bytes) fractions of
+    a call taking `ms`."""
+    achieved = ops / (ms / 1e3) / 1e12
+    gbs = bytes_col * ncol / (ms / 1e3) / 1e9
+    return {'bound': 'valu', 'achieved': achieved, 'peak': FP32_PEAK_TFLOPS, 'unit': 'TFLOP/s',
+            'frac': achieved / FP32_PEAK_TFLOPS, 'ops_per_launch': ops,
+            'hbm_gbs_algorithmic': gbs, 'hbm_frac': gbs / HBM_PEAK_GBS,
+            'bytes_per_column': bytes_col}
+
+
+def cape_config(name, dev, cols_host, kw, p_shared, env_levels, mu_levels, w_it, bytes_col,
+                ref_fn, threads, seconds, steps, warmup, log):
+    """One grid configuration: the plain and warm-started CAPE calls on
+    column-contiguous device arrays `dev` (the whole call timed with HIP
+    events: staging transpose and kernels), their work counters and
+    roofline, the reference on the host cores over chunks of the same
+    columns (`ref_fn(columns)`, bounded by `seconds`), and each method's
+    outputs against the reference's on the columns it finished."""
+    import torch
+
+    from xcape_amd import _hip
+    stream = torch.cuda.current_stream()
+    ncol = dev[1].shape[0]
+    res = {'workload': name, 'columns': ncol}
+    outs = {}
+    for m, warm in (('plain', False), ('warm', True)):
+        st = _hip.cape(*dev, warm_start=warm, stats=True, p_shared=p_shared, **kw)[-1]
+        out, ms, _ = timed(lambda: _hip.cape(*dev, warm_start=warm, p_shared=p_shared, **kw),
+                           steps, warmup, stream)
+        outs[m] = [x.cpu().numpy() for x in out]
+        res[m] = {'ms': ms, 'columns_per_s': ncol / (ms / 1e3), 'iterations': st['iterations'],
+                  'substeps': st['substeps'], 'fallbacks': st['fallbacks'],
+                  'nonconverged': st['nonconverged']}
+        if m == 'plain':
+            ops = algorithmic_ops(st, None, mu_levels, w_it, env_levels)
+        res[m]['roofline'] = roofline_entry(ops, ms, bytes_col, ncol)
+        log(f'[bench] {name.split(":")[0]} {m}: {ms:.3f} ms')
+    kind = reference_kind()
+    n, dt, rres = run_tasks(ranges(ncol, 1024), threads, seconds, ref_fn)
+    idx, ref = gathered(rres)
+    res['cpu_baseline'] = {'value': n / dt, 'unit': 'columns/s', 'cores': threads, 'kind': kind,
+                           'sample': f'{n} of the {ncol} columns, {dt:.2f} s wall on '
+                                     f'{threads} threads'}
+    for m in ('plain', 'warm'):
+        res[m]['vs_reference'] = vs_reference([o[idx] for o in outs[m]], ref)
+        res[m]['speedup_vs_cpu'] = res[m]['columns_per_s'] / res['cpu_baseline']['value']
+    return res
+
+
+def c0_section(args, threads, log):
+    """C0: 64 x 64 columns x 40 sigma levels, surface parcel, pseudo-liquid,
+    pinc 500 Pa (the reference's CPU-runnable case), whole grid."""
+    import torch
+
+    from oracle import oracle as O
+    from oracle import ref_driver as R
+    from xcape_amd import synthetic
+    ncol, nk = 64 * 64, 40
+    c = synthetic.sigma_columns(ncol, nk, 1000)
+    keys = ('p', 't', 'td', 'ps', 'ts', 'tds')
+    dev = [torch.from_numpy(c[k]).cuda() for k in keys]
+    kw = dict(pinc=PINC, source=1, ml_depth=500., adiabat=1, fast_math=args.fast_math)
+    ref = R.available()
+
+    def ref_fn(cols):
+        a = [c[k][cols] for k in keys]
+        if ref:
+            return R.loopcape_ml(a[0].T, a[1].T, a[2].T, a[3], a[4], a[5], PINC, 1, 500., 1)
+        return O.cape(*a, PINC, 1, 500., 1)
+    return cape_config('C0: 64x64 columns x 40 sigma levels, surface, pseudo-liquid, pinc 500 Pa',
+                       dev, c, kw, False, ncol * (nk + 1), 0, W_ITER_LIQ, 3 * nk * 4 + 12 + 16,
+                       ref_fn, threads, 5.0, 50, 5, log)
+
+
+def c1_sections(args, threads, log):
+    """C1: 1024 x 512 columns x 37 shared pressure levels (1000-100 hPa),
+    levels below ground masked by each column's start level, pseudo-liquid,
+    pinc 500 Pa; surface, mixed-layer (500 m) and most-unstable runs, the
+    reference's loopcape_pl1d (start levels from the reference's rule,
+    core.py:286-289, inside its timing)."""
+    import torch
+
+    from oracle import oracle as O
+    from oracle import ref_driver as R
+    from xcape_amd import core, synthetic
+    ncol = 1024 * 512
+    t0 = time.perf_counter()
+    c = synthetic.pressure_columns(ncol, 1001)
+    log(f'[bench] C1 inputs {time.perf_counter() - t0:.1f} s')
+    p1d = c['p'].astype(np.float32)
+    nk = len(p1d)
+    dev = [torch.from_numpy(p1d).cuda()] + [torch.from_numpy(c[k]).cuda()
+                                             for k in ('t', 'td', 'ps', 'ts', 'tds')]
+    start = core.pressure_start_levels(p1d.astype(np.float64), c['ps'].astype(np.float64))
+    env_levels = int((nk - start + 2).sum())
+    n500 = int((p1d >= 500.0).sum())
+    mu_levels = int(np.maximum(0, n500 - (start - 1)).sum()) + ncol
+    ref = R.available()
+    out = {}
+    for name, source in (('surface', 1), ('mixed_layer_500m', 3), ('most_unstable', 2)):
+        kw = dict(pinc=PINC, source=source, ml_depth=500., adiabat=1, fast_math=args.fast_math)
+
+        def ref_fn(cols, source=source):
+            ps = c['ps'][cols]
+            st = core.pressure_start_levels(p1d.astype(np.float64), ps.astype(np.float64))
+            if ref:
+                return R.loopcape_pl1d(c['t'][cols].T, c['td'][cols].T, p1d, ps, c['ts'][cols],
+                                       c['tds'][cols], PINC, source, 500., 1, st)
+            return O.cape(None, c['t'][cols], c['td'][cols], ps, c['ts'][cols], c['tds'][cols],
+                          PINC, source, 500., 1, p1d=p1d, start_lev=st)
+        out[name] = cape_config(f'C1: 1024x512 columns x 37 pressure levels, {name}, '
+                                f'pseudo-liquid, pinc 500 Pa', dev, c, kw, True, env_levels,
+                                mu_levels if source == 2 else 0, W_ITER_LIQ,
+                                2 * nk * 4 + 12 + 16, ref_fn, threads, 4.0, 10, 2, log)
+    return out
 
 
 def pmc_summary(fetch_dir, write_dir, out, ncol, fast_math):
@@ -223,44 +458,88 @@ def pmc_summary(fetch_dir, write_dir, out, ncol, fast_math):
                                                     res[f'{mode}_write_bytes'])
     with open(out, 'w') as f:
         json.dump(res, f, indent=1)
-    print(json.dumps(res))
+    return res
 
 
-def ragged_section(fast_math, log, ncol=200_000):
-    """The ragged-batch workload (200 000 columns, level counts 30/40/60/91/137,
-    SB/ML/MU sources, pseudo-liquid/pseudo-ice): autotuned persistent plans,
-    graph-replayed, plain and warm-started; accuracy on a column sample."""
+def ragged_section(fast_math, threads, log, ncol=200_000):
+    """C2, the ragged-batch workload: 200 000 columns, level counts
+    30/40/60/91/137, SB/ML/MU sources, pseudo-liquid/pseudo-ice.  Both plans
+    autotuned (plain: chunk x LDS tile depth x columns per workgroup; warm:
+    the persistent ascent's columns per workgroup; every config timed is
+    reported), graph-replayed, with the roofline of the reference method's
+    work and the reference on the host cores, one loopcape_ml call per (level
+    count, source, adiabat) group of 1024-column chunks, on a bounded sample,
+    whose outputs each plan's are compared with."""
     import torch
 
     from oracle import oracle as O
+    from oracle import ref_driver as R
     from xcape_amd import ragged, synthetic
     t0 = time.perf_counter()
     c = synthetic.ragged_columns(ncol, 1002)
     b = ragged.RaggedBatch.from_dict(c)
-    res = {'columns': ncol, 'generate_s': time.perf_counter() - t0}
+    res = {'workload': 'C2: 200000 ragged columns (30/40/60/91/137 levels), SB/ML/MU, '
+                       'pseudo-liquid/pseudo-ice, pinc 500 Pa',
+           'columns': ncol, 'generate_s': time.perf_counter() - t0}
+    off = c['offsets']
+    nlev = c['nlev'].astype(np.int64)
+    live = c['ts'] > 0
+    env_levels = int(((nlev + 1) * live).sum())
+    p500 = np.add.reduceat((c['p'] >= 500.0).astype(np.int64), off[:-1])
+    mu_levels = int(((p500 + 1) * live * (c['source'] == 2)).sum())
+    ice = float(np.mean(c['adiabat'][live] >= 3))
+    w_it = ice * W_ITER_ICE + (1 - ice) * W_ITER_LIQ
+    bytes_col = float(np.mean(3 * 4 * nlev)) + 12 + 8 + 8 + 16
+    outs = {}
+    ops = None
     for name, warm in (('plain', False), ('warm', True)):
         best, times = ragged.autotune(b, fast_math=fast_math, use_cache=False, warm_start=warm)
         plan = ragged.RaggedPlan(b, fast_math=fast_math, warm_start=warm, **best)
         ms = ragged._time_plan(plan, reps=10)
         st = plan.run(stats=True)
-        plan.close()
+        plan.run()
         torch.cuda.synchronize()
+        outs[name] = [x.cpu().numpy() for x in (b.cape, b.cin, b.mulev, b.zmulev)]
+        plan.close()
+        if ops is None:
+            ops = algorithmic_ops(st, None, mu_levels, w_it, env_levels)
         res[name] = {'best': best, 'ms': ms, 'columns_per_s': ncol / (ms / 1e3),
-                     'configs_swept': len(times), 'iterations': st['iterations'],
-                     'fallbacks': st['fallbacks']}
-        log(f'[bench] ragged {name}: {best} {ms:.3f} ms')
-    # accuracy of the warm plan on the first 2000 columns (grouped oracle)
-    sel = np.arange(2000)
-    got = b.cape.cpu().numpy()[sel]
-    off = c['offsets']
-    err = 0.0
-    for i in sel:
-        s0, s1 = off[i], off[i + 1]
-        r = O.cape(c['p'][None, s0:s1], c['t'][None, s0:s1], c['td'][None, s0:s1],
-                   c['ps'][i:i + 1], c['ts'][i:i + 1], c['tds'][i:i + 1], 500.,
-                   int(c['source'][i]), 500., int(c['adiabat'][i]))
-        err = max(err, abs(float(got[i]) - float(r[0][0])))
-    res['warm_cape_max_abs_err_2000'] = err
+                     'configs_swept': len(times), 'sweep_ms': times,
+                     'iterations': st['iterations'], 'fallbacks': st['fallbacks'],
+                     'roofline': roofline_entry(ops, ms, bytes_col, ncol)}
+        log(f'[bench] ragged {name}: {best} {ms:.3f} ms ({len(times)} configs)')
+    res['roofline_note'] = ('ops: the plain plan\'s counted work with the iteration weight '
+                            'averaged over the batch\'s adiabat mix')
+    # the reference, grouped as its callers must group a mixed batch
+    kind = reference_kind()
+    keys = np.stack([nlev, c['source'], c['adiabat']], 1)
+    tasks = []
+    for key in np.unique(keys, axis=0):
+        cols = np.nonzero((keys == key).all(1))[0]
+        for j in range(0, len(cols), 1024):
+            tasks.append(cols[j:j + 1024])
+    # interleave the groups so that a bounded run samples every one
+    rng = np.random.default_rng(0)
+    tasks = [tasks[i] for i in rng.permutation(len(tasks))]
+
+    def ref_fn(cols):
+        nk = int(nlev[cols[0]])
+        rows = off[cols][:, None] + np.arange(nk)[None, :]
+        src, adi = int(c['source'][cols[0]]), int(c['adiabat'][cols[0]])
+        a = (c['p'][rows], c['t'][rows], c['td'][rows], c['ps'][cols], c['ts'][cols],
+             c['tds'][cols])
+        if kind == 'reference':
+            return R.loopcape_ml(a[0].T, a[1].T, a[2].T, a[3], a[4], a[5], PINC, src, 500., adi)
+        return O.cape(*a, PINC, src, 500., adi)
+    n, dt, rres = run_tasks(tasks, threads, 5.0, ref_fn)
+    idx, ref = gathered(rres)
+    res['cpu_baseline'] = {'value': n / dt, 'unit': 'columns/s', 'cores': threads, 'kind': kind,
+                           'sample': f'{n} of the {ncol} columns in 1024-column chunks of '
+                                     f'(level count, source, adiabat) groups, {dt:.2f} s wall on '
+                                     f'{threads} threads'}
+    for name in ('plain', 'warm'):
+        res[name]['vs_reference'] = vs_reference([o[idx] for o in outs[name]], ref)
+        res[name]['speedup_vs_cpu'] = res[name]['columns_per_s'] / res['cpu_baseline']['value']
     res['speedup_warm_vs_plain'] = res['plain']['ms'] / res['warm']['ms']
     return res
 
@@ -353,6 +632,8 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-ragged', action='store_true',
                     help='skip the ragged-batch (autotuned persistent kernel) section')
+    ap.add_argument('--no-configs', action='store_true',
+                    help='skip the 64x64x40 sigma and 1024x512x37 pressure-level sections')
     ap.add_argument('--ncol', type=int, default=NCOL_X * NCOL_Y)
     ap.add_argument('--profile-pass', action='store_true',
                     help='one launch of each kernel variant, for counter runs')
@@ -362,8 +643,18 @@ def main():
     ap.add_argument('--host-only', action='store_true',
                     help='only the host-array (PCIe-inclusive) section, for traces')
     args = ap.parse_args()
+    # the JSON line is the only output on stdout: whatever native code
+    # prints (the reference Fortran's convergence messages, which its
+    # runtime flushes at exit) goes to stderr with the progress lines
+    sys.stdout.flush()
+    json_fd = os.dup(1)
+    os.dup2(2, 1)
+
+    def emit(obj):
+        os.write(json_fd, (json.dumps(obj) + '\n').encode())
+
     if args.pmc_summary:
-        pmc_summary(*args.pmc_summary, args.pmc_out, args.ncol, bool(args.fast_math))
+        emit(pmc_summary(*args.pmc_summary, args.pmc_out, args.ncol, bool(args.fast_math)))
         return
     if args.gpus != 1:
         print(f'bench.py runs on one GPU; --gpus {args.gpus} is not supported '
@@ -399,8 +690,8 @@ def main():
     stream = torch.cuda.current_stream()
     sur = load_surrogate(args.surrogate, log)
     if args.host_only:
-        print(json.dumps({'host_arrays_pcie_inclusive': host_array_section(a, ncol, args, log),
-                          'surrogate': sur}))
+        emit({'host_arrays_pcie_inclusive': host_array_section(a, ncol, args, log),
+              'surrogate': sur})
         return
 
     def step(warm, stats=False):
@@ -456,6 +747,8 @@ def main():
     fp64 = _hip.cape(*sl, pinc=PINC, source=SOURCE, adiabat=ADIABAT, fp64=True)
     acc = accuracy(outs, inp, sample, fp64)
     acc_plain = accuracy(outs_p, inp, sample, fp64)
+    acc['fp64_rerun'] = fp64_rerun(outs, inp, sample, warm, bool(args.fast_math))
+    acc_plain['fp64_rerun'] = fp64_rerun(outs_p, inp, sample, False, bool(args.fast_math))
     # every column of the reported method against the reference method in the
     # accurate math mode, on the GPU (the oracle is too slow for the full grid)
     ref_full = _hip.cape(*cape_lm, level_major=True, pinc=PINC, source=SOURCE, adiabat=ADIABAT)
@@ -473,20 +766,36 @@ def main():
         if pmc.get('ncol') == ncol and pmc.get('fast_math') == bool(args.fast_math):
             traffic = pmc.get(f'{dom}_hbm_bytes_per_launch')
 
+    threads = len(os.sched_getaffinity(0))
+    configs = None
+    if not args.no_configs:
+        log('[bench] 64x64x40 sigma grid')
+        configs = {'C0': c0_section(args, threads, log)}
+        log('[bench] 1024x512x37 pressure-level grid')
+        configs['C1'] = c1_sections(args, threads, log)
     rag = None
     if not args.no_ragged:
         log('[bench] ragged batches')
-        rag = ragged_section(bool(args.fast_math), log)
+        rag = ragged_section(bool(args.fast_math), threads, log)
 
     host_path = host_array_section(a, ncol, args, log)
 
     cpu = None
     if not args.no_cpu:
-        threads = len(os.sched_getaffinity(0))
         nhost = min(ncol, args.cpu_sample)
         host = {k: v[:nhost].cpu().numpy() for k, v in inp.items()}
         log(f'[bench] CPU baseline on {threads} threads, {args.cpu_seconds} s')
-        cpu = cpu_baseline(host, threads, args.cpu_seconds)
+        cpu, ref_res = cpu_baseline(host, threads, args.cpu_seconds)
+        del host
+        # the combined step's CAPE outputs against the reference's on every
+        # column the baseline finished
+        idx, ref = gathered(ref_res)
+        sel = torch.from_numpy(idx).to(outs[0].device)
+        acc['vs_reference_cpu_sample'] = vs_reference(
+            [x[sel].cpu().numpy() for x in outs[:4]], ref)
+        acc_plain['vs_reference_cpu_sample'] = vs_reference(
+            [x[sel].cpu().numpy() for x in outs_p[:4]], ref)
+        log(f"[bench] vs reference: {acc['vs_reference_cpu_sample']}")
     del a, inp
 
     line = {
@@ -529,11 +838,12 @@ def main():
         'kernel_counts': counts,
         'surrogate': sur,
         'accuracy': acc,
+        'configs': configs,
         'ragged_200k': rag,
         'host_arrays_pcie_inclusive': host_path,
         'cpu_baseline': cpu,
     }
-    print(json.dumps(line))
+    emit(line)
 
 
 if __name__ == '__main__':

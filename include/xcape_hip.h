@@ -193,6 +193,16 @@ int xcape_cape_f64(const float *p, int p_shared, const int32_t *start_lev,
                    int adiabat, double *cape, double *cin, int32_t *mulev, double *zmulev,
                    uint32_t flags, xcape_stats *stats, void *stream);
 
+/* xcape_cape_levels_f32 evaluated in FP64, as xcape_cape_f64 (the FP64
+ * re-run's LFC and EL).  No reference counterpart. */
+int xcape_cape_levels_f64(const float *p, int p_shared, const int32_t *start_lev,
+                          const float *t, const float *td,
+                          const float *ps, const float *ts, const float *tds,
+                          int64_t nk, int64_t ncol, double pinc, int source, double ml_depth,
+                          int adiabat, double *cape, double *cin, int32_t *mulev,
+                          double *zmulev, double *p_lfc, double *p_el, uint32_t flags,
+                          void *stream);
+
 /*
  * Storm-relative helicity 0..depth for the Bunkers right and left movers,
  * plus the storm motions and the 0-6 km mean wind.  Replaces the chain
@@ -259,7 +269,11 @@ int xcape_cape_srh_f32(const float *p, int p_shared, const int32_t *start_lev,
  * count); every other array is a device pointer the plan keeps using.
  *
  * chunk: columns a wavefront takes from the work queue per grab (64, 128 or
- * 256); tile: levels per column staged in LDS (0..64); waves_per_block 1..4.
+ * 256); tile: levels per column staged in LDS (0..64); cols_per_block:
+ * columns per workgroup, 16, 32, 64, 128 or 256 (one column per lane; 16 and
+ * 32 are one wavefront with the other lanes idle, for the tuner's sweep).
+ * Warm-started plans use neither chunk nor tile, and take 64, 128 or 256
+ * columns per workgroup of their persistent ascent.
  * flags: XCAPE_FAST_MATH and/or XCAPE_WARM_START, or 0.
  */
 typedef struct xcape_ragged_plan xcape_ragged_plan;
@@ -271,7 +285,7 @@ int xcape_ragged_plan_create(const int64_t *offsets_host, const int32_t *source_
                              const float *td, const float *ps, const float *ts,
                              const float *tds, float pinc, float ml_depth, float *cape,
                              float *cin, int32_t *mulev, float *zmulev, int chunk, int tile,
-                             int waves_per_block, uint32_t flags, xcape_ragged_plan **out);
+                             int cols_per_block, uint32_t flags, xcape_ragged_plan **out);
 
 /* Run the plan on `stream`: the first call captures the launch into a
  * hipGraph, later calls replay it.  With stats != NULL the run is a direct

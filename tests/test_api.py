@@ -145,6 +145,20 @@ def test_invalid_arguments_rejected_without_device():
                            P(s), P(s), P(s), P(s), 3, 2, 3000., P(d), P(d), P(w), P(w), P(w),
                            0, None) == -1
     assert b'CAPE calls only' in L.xcape_last_error()
+    # ragged plans: columns per workgroup from the tuner's set, 64 or more
+    # for warm-started plans
+    off = np.array([0, 3, 5], np.int64)
+    one = np.ones(2, np.int32)
+    p5 = np.zeros(5, np.float32)
+    h = ctypes.c_void_p()
+
+    def plan(cols, flags=0):
+        return L.xcape_ragged_plan_create(P(off), P(one), P(one), 2, P(off), P(one), P(one),
+                                          P(p5), P(p5), P(p5), P(s), P(s), P(s), 500., 500.,
+                                          P(o), P(o), P(m), P(o), 64, 0, cols, flags,
+                                          ctypes.byref(h))
+    assert plan(48) == -1 and b'cols_per_block' in L.xcape_last_error()
+    assert plan(16, _hip.XCAPE_WARM_START) == -1 and b'warm' in L.xcape_last_error()
     assert L.xcape_version().startswith(b'xcape-hip')
 
 

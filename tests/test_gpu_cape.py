@@ -220,7 +220,7 @@ def test_warm_grid_matches_ragged_warm(hip, source, adiabat):
              t=c['t'].ravel(), td=c['td'].ravel(), ps=c['ps'], ts=c['ts'], tds=c['tds'],
              source=np.full(n, source, np.int32), adiabat=np.full(n, adiabat, np.int32))
     b = RaggedBatch.from_dict(d)
-    plan = RaggedPlan(b, pinc=500., ml_depth=500., chunk=64, tile=0, waves=4, warm_start=True)
+    plan = RaggedPlan(b, pinc=500., ml_depth=500., chunk=64, tile=0, cols=256, warm_start=True)
     st = plan.run(stats=True)
     torch.cuda.synchronize()
     r = [x.cpu().numpy() for x in (b.cape, b.cin, b.mulev, b.zmulev)]
@@ -379,6 +379,32 @@ def test_lfc_el_pressure_grid_and_small(hip, oracle_lib):
     assert np.all(np.abs(w[4][pos] - r[4][pos]) <= 1.0)
 
 
+def test_lfc_el_fp64(hip, oracle_lib):
+    """xcape_cape_levels_f64 (the FP64 re-run's LFC and EL) against the
+    oracle's FP64 run: the same columns with positive area, LFC and EL within
+    1e-3 hPa (the device's and the host's double exp/log/pow differ by an
+    ulp, far below the FP32 spread); its CAPE, CIN and levels equal
+    xcape_cape_f64's."""
+    c = synthetic.sigma_columns(2000, 80, 63)
+    args = (c['p'], c['t'], c['td'], c['ps'], c['ts'], c['tds'])
+    for source, adiabat in ((1, 1), (3, 3)):
+        kw = dict(pinc=500., source=source, adiabat=adiabat)
+        g = hip.cape(*args, fp64=True, levels=True, **kw)
+        base = hip.cape(*args, fp64=True, **kw)
+        for x, y in zip(g[:4], base):
+            np.testing.assert_array_equal(x, y)
+        assert g[4].dtype == np.float64
+        o = oracle_lib.cape(*args, fp64=True, levels=True, **kw)
+        pos = g[0] > 0
+        assert pos.sum() > 100
+        np.testing.assert_array_equal(pos, o[0] > 0)
+        np.testing.assert_array_equal(np.isfinite(g[4]), pos)
+        for x, y in ((g[4], o[4]), (g[5], o[5])):
+            d = np.abs(x[pos] - y[pos])
+            print(f'fp64 levels s{source} a{adiabat}: max {d.max():.3g}', flush=True)
+            assert np.all(d <= 1e-3)
+
+
 def test_cin_disagreements_only_where_cape_vanishes(hip):
     """On the 2048x1024x137 grid, every variant's CAPE agrees with the plain
     accurate kernel within max(0.5 J/kg, 0.1 %) everywhere.  CIN may not
@@ -440,19 +466,26 @@ def test_pressure_per_column_with_start_levels(hip, oracle_lib, warm):
                 c['tds'], start_lev=st)
 
 
-def test_pressure_grid_full_size(hip, oracle_lib):
+@pytest.fixture(scope='module')
+def pressure_grid():
+    return synthetic.pressure_columns(1024 * 512, 2024)
+
+
+@pytest.mark.parametrize('source', [1, 3, 2])
+def test_pressure_grid_full_size(hip, oracle_lib, pressure_grid, source):
     """The pressure-level configuration at its full size: 1024x512 columns x
-    37 fixed levels, surface parcel, pseudo-liquid, the shared-levels path with
-    start levels from the device's own rule.  Size-independent checks on every
-    column (outputs finite and non-negative, the warm start
-    within tolerance of the plain method for CAPE, CIN only where CAPE
-    vanishes, host arrays streamed equal to device arrays), and a random
-    sample of 2000 columns against the oracle."""
+    37 fixed levels, pseudo-liquid, the shared-levels path with start levels
+    from the device's own rule, for each of its three runs: surface,
+    mixed-layer (500 m) and most-unstable.  Size-independent checks on every
+    column (outputs finite and non-negative, MUlev inside the column, the
+    warm start within tolerance of the plain method for CAPE, CIN only where
+    CAPE vanishes, the same MUlev, host arrays streamed equal to device
+    arrays), and a random sample of 2000 columns against the oracle."""
     import torch
     n = 1024 * 512
-    c = synthetic.pressure_columns(n, 2024)
+    c = pressure_grid
     args = (c['p'], c['t'], c['td'], c['ps'], c['ts'], c['tds'])
-    kw = dict(pinc=500., source=1, adiabat=1, p_shared=True)
+    kw = dict(pinc=500., source=source, ml_depth=500., adiabat=1, p_shared=True)
     dev = [torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in args]
     plain = [x.cpu().numpy() for x in hip.cape(*dev, **kw)]
     warm = [x.cpu().numpy() for x in hip.cape(*dev, warm_start=True, **kw)]
@@ -464,13 +497,21 @@ def test_pressure_grid_full_size(hip, oracle_lib):
     assert (plain[0] >= 0).all() and (plain[1] >= 0).all()
     assert cape_ok(warm[0], plain[0]).all()
     off = ~cape_ok(warm[1], plain[1])
-    print('pressure grid: CIN outside tolerance', int(off.sum()), flush=True)
+    print(f'pressure grid source {source}: CIN outside tolerance', int(off.sum()), flush=True)
     assert off.sum() <= n // 10000
     assert (np.maximum(warm[0][off], plain[0][off]) < 0.05).all()
+    np.testing.assert_array_equal(warm[2], plain[2])
+    live = c['ts'] > 0
+    if source == 2:
+        # the most-unstable level: 1 (the surface) .. the column's last level
+        assert (plain[2][live] >= 1).all() and (plain[2][live] <= len(c['p']) + 1).all()
+    else:
+        # no most-unstable search: the reference's -999999
+        assert (plain[2][live] == -999999).all()
     idx = np.sort(np.random.default_rng(5).choice(n, 2000, replace=False))
     st = oracle_lib.start_levels(c['p'], c['ps'][idx])
     o = oracle_lib.cape(None, c['t'][idx], c['td'][idx], c['ps'][idx], c['ts'][idx],
-                        c['tds'][idx], 500., 1, 500., 1, p1d=c['p'], start_lev=st)
+                        c['tds'][idx], 500., source, 500., 1, p1d=c['p'], start_lev=st)
     assert not _report('pressure grid sample cape', plain[0][idx], o[0]).any()
     assert not _report('pressure grid sample cin', plain[1][idx], o[1]).any()
     np.testing.assert_array_equal(plain[2][idx], o[2])

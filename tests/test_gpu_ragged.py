@@ -39,9 +39,11 @@ def _oracle_ragged(o, d):
     return out
 
 
-@pytest.mark.parametrize('cfg', [dict(chunk=64, tile=16, waves=4),
-                                 dict(chunk=128, tile=0, waves=2),
-                                 dict(chunk=256, tile=32, waves=1)])
+@pytest.mark.parametrize('cfg', [dict(chunk=64, tile=16, cols=256),
+                                 dict(chunk=128, tile=0, cols=128),
+                                 dict(chunk=256, tile=32, cols=64),
+                                 dict(chunk=64, tile=8, cols=16),
+                                 dict(chunk=128, tile=0, cols=32)])
 def test_ragged_matches_oracle(batch_np, oracle_lib, cfg):
     import torch
 
@@ -70,7 +72,7 @@ def test_graph_replays_match_direct_launch(batch_np, warm):
 
     from xcape_amd.ragged import RaggedBatch, RaggedPlan
     b = RaggedBatch.from_dict(batch_np)
-    plan = RaggedPlan(b, chunk=64, tile=16, waves=4, warm_start=warm)
+    plan = RaggedPlan(b, chunk=64, tile=16, cols=256, warm_start=warm)
     plan.run(stats=True)              # direct launch
     torch.cuda.synchronize()
     ref = [x.clone() for x in (b.cape, b.cin, b.mulev, b.zmulev)]
@@ -90,11 +92,11 @@ def test_autotuner_picks_a_timed_config(batch_np, tmp_path, monkeypatch):
     from xcape_amd import ragged
     monkeypatch.setattr(ragged, 'CACHE', str(tmp_path / 'tune.json'))
     b = ragged.RaggedBatch.from_dict(batch_np)
-    best, times = ragged.autotune(b, chunks=(64, 128), tiles=(0, 16), waves=(2, 4), reps=1)
+    best, times = ragged.autotune(b, chunks=(64, 128), tiles=(0, 16), cols=(32, 128), reps=1)
     assert len(times) == 8 and best is not None
     assert min(times.values()) == times[f"chunk={best['chunk']},tile={best['tile']},"
-                                        f"waves={best['waves']}"]
-    best2, _ = ragged.autotune(b, chunks=(64, 128), tiles=(0, 16), waves=(2, 4), reps=1)
+                                        f"cols={best['cols']}"]
+    best2, _ = ragged.autotune(b, chunks=(64, 128), tiles=(0, 16), cols=(32, 128), reps=1)
     assert best2 == best              # served from the cache
 
 
@@ -103,7 +105,7 @@ def test_ragged_warm_start(batch_np, oracle_lib):
 
     from xcape_amd.ragged import RaggedBatch, RaggedPlan
     b = RaggedBatch.from_dict(batch_np)
-    plan = RaggedPlan(b, chunk=64, tile=16, waves=4, warm_start=True)
+    plan = RaggedPlan(b, chunk=64, tile=16, cols=256, warm_start=True)
     st = plan.run(stats=True)
     torch.cuda.synchronize()
     got = [x.cpu().numpy() for x in (b.cape, b.cin, b.mulev)]
@@ -113,6 +115,14 @@ def test_ragged_warm_start(batch_np, oracle_lib):
     np.testing.assert_array_equal(got[2], ref[2])
     assert st['iterations'] < 0.45 * 9.5 * st['substeps']
     plan.close()
+    # the ascent's workgroup size (the warm plans' tuning knob) moves no result
+    for cols in (64, 128):
+        p2 = RaggedPlan(b, cols=cols, warm_start=True)
+        p2.run()
+        torch.cuda.synchronize()
+        for x, y in zip((b.cape, b.cin, b.mulev), got):
+            np.testing.assert_array_equal(x.cpu().numpy(), y)
+        p2.close()
 
 
 def test_ragged_full_size(hip, oracle_lib):
@@ -127,7 +137,7 @@ def test_ragged_full_size(hip, oracle_lib):
     d = synthetic.ragged_columns(200_000, 2003)
     n = len(d['nlev'])
     b = RaggedBatch.from_dict(d)
-    plan = RaggedPlan(b, chunk=64, tile=0, waves=1)
+    plan = RaggedPlan(b, chunk=64, tile=0, cols=64)
     plan.run()
     torch.cuda.synchronize()
     plain = [x.cpu().numpy().copy() for x in (b.cape, b.cin, b.mulev, b.zmulev)]

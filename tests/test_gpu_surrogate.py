@@ -213,3 +213,37 @@ def test_warm_start_with_surrogate_parity_and_gain(torch, trained):
         assert st_s['substeps'] == st_e['substeps']
         assert st_s['first_substep_iterations'] < 0.5 * st_e['first_substep_iterations']
         assert st_s['iterations'] < st_e['iterations']
+
+
+def test_committed_surrogate_gain_full_size(torch):
+    """The committed network on the combined workload at its full size,
+    2048x1024 columns x 137 sigma levels, most-unstable, pseudo-ice (inputs
+    generated on the device): against extrapolation alone, the same
+    sub-steps, fewer than half the iterations on the first sub-steps it
+    serves, fewer iterations in all; and CAPE within the tolerance of
+    extrapolation's on every column, CIN too apart from slivers of positive
+    area below 0.05 J/kg (there the reference's CIN hangs on a near-zero
+    buoyancy crossing, and either run is as close to the reference)."""
+    from xcape_amd import _hip, surrogate, synthetic
+    surrogate.Surrogate.load().activate()
+    n = 2048 * 1024
+    inp = synthetic.sigma_columns_torch(n, 137, 4004, device='cuda')
+    dev = [inp[k] for k in ('p', 't', 'td', 'ps', 'ts', 'tds')]
+    kw = dict(pinc=500., source=2, adiabat=3, stats=True)
+    out_s = _hip.cape(*dev, warm_start=True, **kw)
+    out_e = _hip.cape(*dev, warm_start='extrapolation', **kw)
+    st_s, st_e = out_s[-1], out_e[-1]
+    print('surrogate', {k: st_s[k] for k in ('iterations', 'first_substep_iterations',
+                                              'fallbacks', 'substeps')},
+          'extrapolation', {k: st_e[k] for k in ('iterations', 'first_substep_iterations',
+                                                  'fallbacks', 'substeps')}, flush=True)
+    assert st_s['substeps'] == st_e['substeps']
+    assert st_s['first_substep_iterations'] < 0.5 * st_e['first_substep_iterations']
+    assert st_s['iterations'] < st_e['iterations']
+    cs, ce = out_s[0].cpu().numpy(), out_e[0].cpu().numpy()
+    assert cape_ok(cs, ce).all()
+    off = ~cape_ok(out_s[1].cpu().numpy(), out_e[1].cpu().numpy())
+    print('CIN outside tolerance', int(off.sum()), flush=True)
+    assert off.sum() <= n // 100000
+    assert (np.maximum(cs[off], ce[off]) < 0.05).all()
+    np.testing.assert_array_equal(out_s[2].cpu().numpy(), out_e[2].cpu().numpy())

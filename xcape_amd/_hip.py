@@ -77,6 +77,8 @@ SIGNATURES = {
                                     _P, _P, _P, _P, _U32, _P, _P]),
     'xcape_cape_f64': (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _I64, _I64, _D, _I, _D, _I,
                             _P, _P, _P, _P, _U32, _P, _P]),
+    'xcape_cape_levels_f64': (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _I64, _I64, _D, _I, _D, _I,
+                                   _P, _P, _P, _P, _P, _P, _U32, _P]),
     'xcape_srh_f32': (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I64, _I64, _D,
                            _P, _P, _P, _P, _P, _U32, _P]),
     'xcape_srh_f64': (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I64, _I64, _D,
@@ -241,15 +243,16 @@ def _warm_flags(warm_start):
 def cape(p, t, td, ps, ts, tds, *, pinc=500., source=1, ml_depth=500., adiabat=1,
          p_shared=False, start_lev=None, fast_math=False, warm_start=False, fp64=False,
          stats=False, level_major=False, levels=False):
-    """Run xcape_cape_f32 (or _f64, or with ``levels`` xcape_cape_levels_f32).
+    """Run xcape_cape_f32 (or _f64, or with ``levels`` xcape_cape_levels_f32 /
+    _f64).
     t, td: (ncol, nk); p: (ncol, nk) or, with p_shared, (nk,); a
     ``start_lev`` with (ncol, nk) p starts each column at its own level of
     its own p (the reference's loopcape_pl); with
     ``level_major`` the profile arrays are (nk, ncol).  Returns cape, cin,
     mulev, zmulev (then p_lfc, p_el [hPa] with ``levels``; then a stats dict
     with ``stats``)."""
-    if levels and (fp64 or stats):
-        raise ValueError('levels are computed by the float32 method without stats')
+    if levels and stats:
+        raise ValueError('levels are computed without stats')
     A = _Args([p, t, td, ps, ts, tds, start_lev])
     p, t, td, ps, ts, tds = (A.f32(a) for a in (p, t, td, ps, ts, tds))
     start_lev = A.i32(start_lev)
@@ -261,8 +264,9 @@ def cape(p, t, td, ps, ts, tds, *, pinc=500., source=1, ml_depth=500., adiabat=1
         | _warm_flags(warm_start) | (XCAPE_LEVEL_MAJOR if level_major else 0)
     st = Stats()
     if levels:
-        lfc_o, el_o = (A.empty(ncol, np.float32, t) for _ in range(2))
-        rc = lib().xcape_cape_levels_f32(
+        lfc_o, el_o = (A.empty(ncol, odt, t) for _ in range(2))
+        fn = lib().xcape_cape_levels_f64 if fp64 else lib().xcape_cape_levels_f32
+        rc = fn(
             _ptr(p), 1 if p_shared else 0, _ptr(start_lev), _ptr(t), _ptr(td), _ptr(ps),
             _ptr(ts), _ptr(tds), nk, ncol, pinc, source, ml_depth, adiabat, _ptr(cape_o),
             _ptr(cin_o), _ptr(mul_o), _ptr(z_o), _ptr(lfc_o), _ptr(el_o), flags, A.stream)

@@ -185,8 +185,8 @@ __global__ void __launch_bounds__(256) cape_grid_kernel(CapeGridArgs a)
         a.mulvl[col] = o.mulvl;
         zout[col] = o.zout;
         if constexpr (LEVELS) {
-            a.p_lfc[col] = (float)o.p_lfc;
-            a.p_el[col] = (float)o.p_el;
+            static_cast<T *>(a.p_lfc)[col] = o.p_lfc;
+            static_cast<T *>(a.p_el)[col] = o.p_el;
         }
     }
     if constexpr (STATS) block_stats_add(st, ran, a.stats);
@@ -335,8 +335,8 @@ warm_setup_kernel(CapeGridArgs a)
         zout[oc] = ran ? T(-999999.0) : T(0.0);
         if (!ran) a.mulvl[oc] = 0;
         if (a.p_lfc) {
-            a.p_lfc[oc] = T(NAN);
-            a.p_el[oc] = T(NAN);
+            static_cast<T *>(a.p_lfc)[oc] = T(NAN);
+            static_cast<T *>(a.p_el)[oc] = T(NAN);
         }
     }
     if constexpr (SUR) {
@@ -462,8 +462,8 @@ warm_ascent_kernel(CapeGridArgs a)
                 cin[oc] = o.cin;
                 zout[oc] = o.zout;
                 if constexpr (LEVELS) {
-                    a.p_lfc[oc] = o.p_lfc;
-                    a.p_el[oc] = o.p_el;
+                    static_cast<float *>(a.p_lfc)[oc] = o.p_lfc;
+                    static_cast<float *>(a.p_el)[oc] = o.p_el;
                 }
                 have = false;
                 need = true;
@@ -549,14 +549,12 @@ hipError_t launch_cape_s(const CapeGridArgs &a, int block, hipStream_t s)
 template <typename T>
 hipError_t launch_cape(const CapeGridArgs &a, int block, hipStream_t s)
 {
-    if (a.p_lfc && (sizeof(T) != 4 || a.stats)) return hipErrorInvalidValue;
+    if (a.p_lfc && a.stats) return hipErrorInvalidValue;
     if (a.stats) return launch_cape_s<T, uint64_t, true, false>(a, block, s);
     const bool fits = (uint64_t)a.ncol * (uint64_t)a.nk_in < ((uint64_t)1 << 32);
-    if constexpr (sizeof(T) == 4) {
-        if (a.p_lfc)
-            return fits ? launch_cape_s<T, uint32_t, false, true>(a, block, s)
-                        : launch_cape_s<T, uint64_t, false, true>(a, block, s);
-    }
+    if (a.p_lfc)
+        return fits ? launch_cape_s<T, uint32_t, false, true>(a, block, s)
+                    : launch_cape_s<T, uint64_t, false, true>(a, block, s);
     return fits ? launch_cape_s<T, uint32_t, false, false>(a, block, s)
                 : launch_cape_s<T, uint64_t, false, false>(a, block, s);
 }

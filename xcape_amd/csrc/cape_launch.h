@@ -29,7 +29,7 @@ struct CapeGridArgs {
     int source, adiabat;
     void *cape, *cin, *zout;     // float or double (ncol,)
     int32_t *mulvl;              // (ncol,)
-    float *p_lfc, *p_el;         // (ncol,) LFC and EL [hPa], or nullptr (float only)
+    void *p_lfc, *p_el;          // (ncol,) LFC and EL [hPa] (T, as cape), or nullptr
     unsigned long long *stats;   // ST_COUNT counters or nullptr
     // storm-relative helicity in the same pass (fused kernel) or alone
     const float *u, *v;          // (ncol, nk_in)

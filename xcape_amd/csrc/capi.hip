@@ -335,7 +335,10 @@ hipError_t run_grid(const CapeGridArgs &a0, bool cape, bool srh, int math, hipSt
             a.cin = static_cast<char *>(a.cin) + c0 * esz;
             a.zout = static_cast<char *>(a.zout) + c0 * esz;
             a.mulvl += c0;
-            if (a.p_lfc) { a.p_lfc += c0; a.p_el += c0; }
+            if (a.p_lfc) {
+                a.p_lfc = static_cast<char *>(a.p_lfc) + c0 * esz;
+                a.p_el = static_cast<char *>(a.p_el) + c0 * esz;
+            }
         }
         if (srh) {
             a.us += c0; a.vs += c0;
@@ -366,7 +369,7 @@ struct GridCall {
     int source = 1, adiabat = 1;
     void *cape = nullptr, *cin = nullptr, *zout = nullptr;
     int32_t *mulev = nullptr;
-    float *p_lfc = nullptr, *p_el = nullptr;   // LFC/EL outputs (float CAPE calls only)
+    void *p_lfc = nullptr, *p_el = nullptr;    // LFC/EL outputs (float, or double when fp64)
     double *srh_rm = nullptr, *srh_lm = nullptr;
     void *rm = nullptr, *lm = nullptr, *mean6 = nullptr;  // float, or double when fp64
     uint32_t flags = 0;
@@ -626,8 +629,8 @@ int stream_host_grid(const GridCall &c, const CapeGridArgs &tmpl, bool want_cape
         o_z = out.add(nullptr, c.zout, osz);
         o_mul = out.add(nullptr, c.mulev, 4);
         if (c.p_lfc) {
-            o_lfc = out.add(nullptr, c.p_lfc, 4);
-            o_el = out.add(nullptr, c.p_el, 4);
+            o_lfc = out.add(nullptr, c.p_lfc, osz);
+            o_el = out.add(nullptr, c.p_el, osz);
         }
     }
     if (want_srh) {
@@ -740,7 +743,7 @@ int stream_host_grid(const GridCall &c, const CapeGridArgs &tmpl, bool want_cape
         a.u = fin(i_u); a.v = fin(i_v); a.us = fin(i_us); a.vs = fin(i_vs);
         a.cape = fout(o_cape); a.cin = fout(o_cin); a.zout = fout(o_z);
         a.mulvl = static_cast<int32_t *>(fout(o_mul));
-        a.p_lfc = static_cast<float *>(fout(o_lfc)); a.p_el = static_cast<float *>(fout(o_el));
+        a.p_lfc = fout(o_lfc); a.p_el = fout(o_el);
         a.srh_rm = static_cast<double *>(fout(o_srm)); a.srh_lm = static_cast<double *>(fout(o_slm));
         a.rm = fout(o_rm); a.lm = fout(o_lm); a.mean6 = fout(o_mw);
         a.stats = (c.stats && want_cape) ? dstats : nullptr;
@@ -819,8 +822,7 @@ int grid_entry(const GridCall &c)
 
     DevBuf<float> dp_, dt_, dtd_, dps_, dts_, dtds_, du_, dv_, dus_, dvs_;
     DevBuf<int32_t> dst_, dmul_;
-    DevBuf<unsigned char> dcape_, dcin_, dz_, drm_, dlm_, dmw_;
-    DevBuf<float> dlfc_, del_;
+    DevBuf<unsigned char> dcape_, dcin_, dz_, drm_, dlm_, dmw_, dlfc_, del_;
     DevBuf<double> dsrm_, dslm_;
     DevBuf<unsigned long long> dstats_;
     CapeGridArgs a{};
@@ -873,7 +875,7 @@ int grid_entry(const GridCall &c)
             XC_TRY(dz_.alloc(ncol * osz, s)); XC_TRY(dmul_.alloc(ncol, s));
             a.cape = dcape_.ptr; a.cin = dcin_.ptr; a.zout = dz_.ptr; a.mulvl = dmul_.ptr;
             if (c.p_lfc) {
-                XC_TRY(dlfc_.alloc(ncol, s)); XC_TRY(del_.alloc(ncol, s));
+                XC_TRY(dlfc_.alloc(ncol * osz, s)); XC_TRY(del_.alloc(ncol * osz, s));
                 a.p_lfc = dlfc_.ptr; a.p_el = del_.ptr;
             }
         }
@@ -922,8 +924,8 @@ int grid_entry(const GridCall &c)
             XC_TRY(hipMemcpyAsync(c.zout, dz_.ptr, ncol * osz, hipMemcpyDeviceToHost, s));
             XC_TRY(dmul_.download(c.mulev, ncol, s));
             if (c.p_lfc) {
-                XC_TRY(dlfc_.download(c.p_lfc, ncol, s));
-                XC_TRY(del_.download(c.p_el, ncol, s));
+                XC_TRY(hipMemcpyAsync(c.p_lfc, dlfc_.ptr, ncol * osz, hipMemcpyDeviceToHost, s));
+                XC_TRY(hipMemcpyAsync(c.p_el, del_.ptr, ncol * osz, hipMemcpyDeviceToHost, s));
             }
         }
         if (want_srh) {
@@ -1034,6 +1036,27 @@ int xcape_cape_levels_f32(const float *p, int p_shared, const int32_t *start_lev
 {
     GridCall c;
     c.kind = GRID_CAPE;
+    c.p = p; c.p_shared = p_shared; c.start = start_lev;
+    c.t = t; c.td = td; c.ps = ps; c.ts = ts; c.tds = tds;
+    c.nk = nk; c.ncol = ncol; c.pinc = pinc; c.source = source; c.ml_depth = ml_depth;
+    c.adiabat = adiabat;
+    c.cape = cape; c.cin = cin; c.mulev = mulev; c.zout = zmulev;
+    c.p_lfc = p_lfc; c.p_el = p_el;
+    if (ncol > 0 && (!p_lfc || !p_el)) return fail(XCAPE_EINVAL, "NULL LFC/EL output array");
+    c.flags = flags; c.stream = stream;
+    return grid_entry(c);
+}
+
+int xcape_cape_levels_f64(const float *p, int p_shared, const int32_t *start_lev,
+                          const float *t, const float *td, const float *ps, const float *ts,
+                          const float *tds, int64_t nk, int64_t ncol, double pinc, int source,
+                          double ml_depth, int adiabat, double *cape, double *cin,
+                          int32_t *mulev, double *zmulev, double *p_lfc, double *p_el,
+                          uint32_t flags, void *stream)
+{
+    GridCall c;
+    c.kind = GRID_CAPE;
+    c.fp64 = true;
     c.p = p; c.p_shared = p_shared; c.start = start_lev;
     c.t = t; c.td = td; c.ps = ps; c.ts = ts; c.tds = tds;
     c.nk = nk; c.ncol = ncol; c.pinc = pinc; c.source = source; c.ml_depth = ml_depth;

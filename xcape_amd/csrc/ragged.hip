@@ -109,6 +109,7 @@ struct RaggedArgs {
     unsigned long long *stats;
     int chunk;                     // columns a wavefront takes per grab
     int tile;                      // levels staged in LDS (0 = none)
+    int cpw;                       // columns per wave pass: 64, or 16 / 32 (other lanes idle)
 };
 
 // Sorted column j of the staged batch: level k >= 1 at [(k-1) * ls + j] of
@@ -180,9 +181,9 @@ __global__ void __launch_bounds__(256) ragged_kernel(RaggedArgs a)
         base = __shfl(base, 0, 64);
         const int64_t b0 = a.seg_begin + (int64_t)base;
         if (b0 >= a.seg_end) break;
-        for (int sub = 0; sub < a.chunk; sub += 64) {
+        for (int sub = 0; sub < a.chunk; sub += a.cpw) {
             const int64_t j = b0 + sub + lane;
-            const bool active = j < a.seg_end;
+            const bool active = lane < a.cpw && j < a.seg_end;
             const int nk_in = active ? a.nk_sorted[j] : 0;
             const int tile = TILED ? min(a.tile, nk_in) : 0;
             // levels 0..tile-1: the wave's lanes read consecutive columns
@@ -319,8 +320,8 @@ hipError_t enqueue(const xcape_ragged_plan *pl, bool stats, hipStream_t s,
         if (pl->warm) {
             CapeGridArgs w = pl->warm_args[g];
             w.stats = stats ? pl->d_stats : nullptr;
-            e = pl->fast ? launch_cape_grid_fast(w, GRID_CAPE, 256, sg)
-                         : launch_cape_grid_accurate(w, false, GRID_CAPE, 256, sg);
+            e = pl->fast ? launch_cape_grid_fast(w, GRID_CAPE, pl->block, sg)
+                         : launch_cape_grid_accurate(w, false, GRID_CAPE, pl->block, sg);
         } else {
             RaggedArgs a = pl->args;
             a.seg_begin = pl->seg_begin[g];
@@ -367,7 +368,7 @@ int xcape_ragged_plan_create(const int64_t *offsets_host, const int32_t *source_
                              const float *td, const float *ps, const float *ts,
                              const float *tds, float pinc, float ml_depth, float *cape,
                              float *cin, int32_t *mulev, float *zmulev, int chunk, int tile,
-                             int waves_per_block, uint32_t flags, xcape_ragged_plan **out)
+                             int cols_per_block, uint32_t flags, xcape_ragged_plan **out)
 {
     if (!out) return fail(XCAPE_EINVAL, "out is NULL");
     *out = nullptr;
@@ -382,8 +383,15 @@ int xcape_ragged_plan_create(const int64_t *offsets_host, const int32_t *source_
         return fail(XCAPE_EINVAL, "chunk=%d not in {64, 128, 256}", chunk);
     if (tile < 0 || tile > MAX_TILE)
         return fail(XCAPE_EINVAL, "tile=%d outside 0..%d", tile, MAX_TILE);
-    if (waves_per_block < 1 || waves_per_block > 4)
-        return fail(XCAPE_EINVAL, "waves_per_block=%d outside 1..4", waves_per_block);
+    if (cols_per_block != 16 && cols_per_block != 32 && cols_per_block != 64 &&
+        cols_per_block != 128 && cols_per_block != 256)
+        return fail(XCAPE_EINVAL, "cols_per_block=%d not in {16, 32, 64, 128, 256}",
+                    cols_per_block);
+    if ((flags & XCAPE_WARM_START) && cols_per_block < 64)
+        return fail(XCAPE_EINVAL, "warm-started plans take 64, 128 or 256 columns per block "
+                                  "(cols_per_block=%d)", cols_per_block);
+    // a block of fewer than 64 columns is one wave with the other lanes idle
+    const int waves_per_block = cols_per_block < 64 ? 1 : cols_per_block / 64;
     if (flags & ~(uint32_t)(XCAPE_FAST_MATH | XCAPE_DEVICE_PTRS | XCAPE_WARM_START |
                             XCAPE_NO_SURROGATE))
         return fail(XCAPE_EINVAL, "unsupported flags 0x%x", flags);
@@ -461,6 +469,7 @@ int xcape_ragged_plan_create(const int64_t *offsets_host, const int32_t *source_
     a.ncol = ncol; a.pinc = pinc; a.ml_depth = ml_depth;
     a.cape = cape; a.cin = cin; a.zout = zmulev; a.mulvl = mulev;
     a.chunk = chunk; a.tile = tile;
+    a.cpw = cols_per_block < 64 ? cols_per_block : 64;
     if (pl->warm) {
         // each segment as a grid of its own for the warm kernels: its columns
         // start seg_begin into each level row of the staged copies
@@ -516,7 +525,7 @@ int xcape_ragged_plan_create(const int64_t *offsets_host, const int32_t *source_
             pl->block, pl->lds_bytes);
         if (e != hipSuccess || per_cu < 1) per_cu = 1;
         // enough blocks to fill every CU once, never more than the work needs
-        const int64_t waves_needed = (n + chunk - 1) / chunk;
+        const int64_t waves_needed = (n + chunk - 1) / chunk;   // (chunk: columns per grab)
         const int64_t blocks_needed = (waves_needed + waves_per_block - 1) / waves_per_block;
         pl->seg_grid[g] =
             (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)ncu * per_cu, blocks_needed));

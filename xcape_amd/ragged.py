@@ -25,8 +25,11 @@ from . import _hip
 
 CHUNKS = (64, 128, 256)
 TILES = (0, 8, 16, 32)
-WAVES_PER_BLOCK = (1, 2, 4)
-DEFAULT = dict(chunk=64, tile=16, waves=4)
+# columns per workgroup, one per lane: 16 and 32 leave lanes of the
+# wavefront idle, 64 / 128 / 256 are one, two and four full wavefronts
+COLS_PER_BLOCK = (16, 32, 64, 128, 256)
+WARM_COLS_PER_BLOCK = (64, 128, 256)
+DEFAULT = dict(chunk=64, tile=16, cols=256)
 CACHE = os.environ.get('XCAPE_AUTOTUNE_CACHE',
                        os.path.join(os.path.expanduser('~'), '.cache', 'xcape_amd',
                                     'autotune.json'))
@@ -67,10 +70,10 @@ class RaggedBatch:
 
 
 class RaggedPlan:
-    def __init__(self, batch, pinc=500., ml_depth=500., chunk=64, tile=16, waves=4,
+    def __init__(self, batch, pinc=500., ml_depth=500., chunk=64, tile=16, cols=256,
                  fast_math=False, warm_start=False):
         self.batch = batch
-        self.cfg = dict(chunk=chunk, tile=tile, waves=waves)
+        self.cfg = dict(chunk=chunk, tile=tile, cols=cols)
         b = batch
         P = _hip._ptr
         h = ctypes.c_void_p()
@@ -80,7 +83,7 @@ class RaggedPlan:
             P(b.offsets_host), P(b.source_host), P(b.adiabat_host), b.ncol,
             P(b.offsets), P(b.source), P(b.adiabat), P(b.p), P(b.t), P(b.td), P(b.ps), P(b.ts),
             P(b.tds), pinc, ml_depth, P(b.cape), P(b.cin), P(b.mulev), P(b.zmulev),
-            chunk, tile, waves, flags, ctypes.byref(h))
+            chunk, tile, cols, flags, ctypes.byref(h))
         _hip.check(rc)
         self._h = h
 
@@ -140,10 +143,11 @@ def _save_cache(c):
 
 
 def autotune(batch, pinc=500., ml_depth=500., fast_math=False, chunks=CHUNKS, tiles=TILES,
-             waves=WAVES_PER_BLOCK, reps=3, use_cache=True, log=None, warm_start=False):
-    """Time every (chunk, tile, waves-per-block) plan on `batch` and return
-    (best config, {config: ms}).  Results are cached per device, batch
-    signature and method settings."""
+             cols=None, reps=3, use_cache=True, log=None, warm_start=False):
+    """Time every (chunk, tile, columns-per-workgroup) plan on `batch` and
+    return (best config, {config: ms}).  Warm-started plans have no chunk or
+    tile: their sweep is over the persistent ascent's columns per workgroup.
+    Results are cached per device, batch signature and method settings."""
     import torch
     dev = torch.cuda.get_device_name(0)
     key = f'{dev}|{batch.signature()}|{pinc}|{ml_depth}|{int(fast_math)}|{int(warm_start)}'
@@ -153,22 +157,25 @@ def autotune(batch, pinc=500., ml_depth=500., fast_math=False, chunks=CHUNKS, ti
         return ent['best'], ent['times']
     times = {}
     best, best_ms = None, float('inf')
+    if cols is None:
+        cols = WARM_COLS_PER_BLOCK if warm_start else COLS_PER_BLOCK
     if warm_start:
         # the warm kernels take neither a chunk nor a tile
-        chunks, tiles, waves = (DEFAULT['chunk'],), (0,), (DEFAULT['waves'],)
+        chunks, tiles = (DEFAULT['chunk'],), (0,)
+        cols = tuple(c for c in cols if c >= 64)
     for ch in chunks:
         for tl in tiles:
-            for wv in waves:
-                plan = RaggedPlan(batch, pinc, ml_depth, ch, tl, wv, fast_math, warm_start)
+            for cb in cols:
+                plan = RaggedPlan(batch, pinc, ml_depth, ch, tl, cb, fast_math, warm_start)
                 t0 = time.perf_counter()
                 ms = _time_plan(plan, reps)
                 plan.close()
-                name = f'chunk={ch},tile={tl},waves={wv}'
+                name = f'chunk={ch},tile={tl},cols={cb}'
                 times[name] = ms
                 if log:
                     log(f'{name}: {ms:.3f} ms ({time.perf_counter() - t0:.1f} s)')
                 if ms < best_ms:
-                    best_ms, best = ms, dict(chunk=ch, tile=tl, waves=wv)
+                    best_ms, best = ms, dict(chunk=ch, tile=tl, cols=cb)
     if use_cache:
         cache[key] = {'best': best, 'times': times}
         _save_cache(cache)
