@@ -73,8 +73,10 @@
  * outcome changes between seeds are those the reference decides by its
  * rounding alone. */
 static uint32_t xo_pert_seed;
+static float xo_pert_iter_amp;
 
 void xo_set_perturbation(uint32_t seed) { xo_pert_seed = seed; }
+void xo_set_iterate_perturbation(float amp) { xo_pert_iter_amp = amp; }
 
 static float pert_f(float r, uint32_t a, uint32_t b, uint32_t tag)
 {
@@ -92,6 +94,15 @@ static uint32_t bits_f(float x) { uint32_t u; memcpy(&u, &x, 4); return u; }
 static float pexpf(float x) { return pert_f(expf(x), bits_f(x), 0u, 1u); }
 static float plogf(float x) { return pert_f(logf(x), bits_f(x), 0u, 2u); }
 static float ppowf(float x, float y) { return pert_f(powf(x, y), bits_f(x), bits_f(y), 3u); }
+/* a shift in [-amp, amp] by a hash of the iterate and the seed */
+static float iter_delta(float x)
+{
+    uint32_t h = bits_f(x) * 2654435761u ^ (xo_pert_seed * 3266489917u + 4u);
+    h ^= h >> 15;
+    h *= 2246822519u;
+    h ^= h >> 13;
+    return xo_pert_iter_amp * ((float)(h & 0xffffu) / 32767.5f - 1.0f);
+}
 
 #define REAL float
 #define SUF _pert32
@@ -100,7 +111,13 @@ static float ppowf(float x, float y) { return pert_f(powf(x, y), bits_f(x), bits
 #define POWF ppowf
 #define L(x) x##f
 #define LOGREC 0
+#define ITERPERT 1
+#define ITERPERT_AMP xo_pert_iter_amp
+#define ITERPERT_DELTA iter_delta
 #include "cape_column.inc"
+#undef ITERPERT
+#undef ITERPERT_AMP
+#undef ITERPERT_DELTA
 #undef REAL
 #undef SUF
 #undef EXPF

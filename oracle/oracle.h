@@ -76,8 +76,14 @@ void xo_cape_f64(const double *p, const double *p1d, const double *t, const doub
 /* xo_cape_f32 with every exp/log/pow result kept or moved one ulp up or
  * down, pseudo-randomly per argument and seed (xo_set_perturbation; one
  * process-wide seed, not for concurrent use): a stand-in for a C library
- * whose functions differ from this one's by up to an ulp. */
+ * whose functions differ from this one's by up to an ulp.  With
+ * xo_set_iterate_perturbation(amp > 0), also every converged sub-step
+ * accepts another point of its tolerance band: the last iterate moved by up
+ * to amp [K] (pseudo-randomly per iterate and seed), evaluated, and kept
+ * when its residual passes the same 2e-4 K test -- the freedom any solver
+ * of the reference's test has, a warm-started one included. */
 void xo_set_perturbation(uint32_t seed);
+void xo_set_iterate_perturbation(float amp);
 void xo_cape_pert32(const float *p, const float *p1d, const float *t, const float *td,
                     const float *ps, const float *ts, const float *tds,
                     const int32_t *start_lev, int64_t nk_in, int64_t ncol,

@@ -62,14 +62,18 @@ def start_levels(p1d, ps):
 
 
 def cape(p, t, td, ps, ts, tds, pinc=500., source=1, ml_depth=500., adiabat=1,
-         p1d=None, start_lev=None, fp64=False, stats=False, levels=False, perturb=None):
+         p1d=None, start_lev=None, fp64=False, stats=False, levels=False, perturb=None,
+         perturb_iterate=0.0):
     """Column CAPE/CIN/MUlvl/z_out.  ``p`` is (ncol, nk) (ignored when ``p1d``
     gives shared pressure levels, in which case ``start_lev`` is required).
     ``p`` with ``start_lev``: each column from its start level of its own p
     (loopcape_pl).  With ``levels`` the LFC and EL pressures [hPa] follow
     z_out (NaN without positive area).  ``perturb``: a seed; every exp, log
     and pow result is then kept or moved by one ulp (xo_cape_pert32), as a
-    C library an ulp away from this one would give them (FP32, no levels)."""
+    C library an ulp away from this one would give them (FP32, no levels);
+    with ``perturb_iterate`` > 0 every converged sub-step also accepts
+    another point of its tolerance band (the last iterate moved by up to that
+    many K, kept when it passes the same convergence test)."""
     dt = np.float64 if fp64 else np.float32
     t = _c(t, dt)
     ncol, nk = t.shape
@@ -95,6 +99,7 @@ def cape(p, t, td, ps, ts, tds, pinc=500., source=1, ml_depth=500., adiabat=1,
         if fp64 or levels:
             raise ValueError('perturbed runs are FP32 without levels')
         lib().xo_set_perturbation(ctypes.c_uint32(perturb))
+        lib().xo_set_iterate_perturbation(ctypes.c_float(perturb_iterate))
         fn = lib().xo_cape_pert32
         fn.restype = None
         fn(_p(p), _p(p1d), _p(t), _p(td), _p(ps), _p(ts), _p(tds), _p(start_lev),

@@ -3,7 +3,7 @@ goldens.  Calls go through the C-ABI (xcape_amd._hip -> libxcape_hip.so)."""
 import numpy as np
 import pytest
 
-from tests.tolerances import cape_ok, cape_tol
+from tests.tolerances import cape_ok, cape_tol, ill_conditioned_and_matched
 from xcape_amd import synthetic
 
 pytestmark = pytest.mark.gpu
@@ -496,10 +496,23 @@ def test_pressure_grid_full_size(hip, oracle_lib, pressure_grid, source):
     assert np.all(np.isfinite(plain[0])) and np.all(np.isfinite(plain[1]))
     assert (plain[0] >= 0).all() and (plain[1] >= 0).all()
     assert cape_ok(warm[0], plain[0]).all()
-    off = ~cape_ok(warm[1], plain[1])
-    print(f'pressure grid source {source}: CIN outside tolerance', int(off.sum()), flush=True)
-    assert off.sum() <= n // 10000
-    assert (np.maximum(warm[0][off], plain[0][off]) < 0.05).all()
+    off = np.nonzero(~cape_ok(warm[1], plain[1]))[0]
+    print(f'pressure grid source {source}: CIN outside tolerance', off.tolist(), flush=True)
+    assert len(off) <= n // 10000
+    if len(off):
+        # where the warm start and the plain method disagree on CIN, the
+        # reference's own CIN is ill-conditioned (committed at a near-zero
+        # buoyancy crossing: it turns on the last bit of the C library or on
+        # the iterate accepted within the convergence tolerance), and each
+        # of them is within tolerance of one of the oracle's runs
+        def run(cols, **kw):
+            return oracle_lib.cape(None, c['t'][cols], c['td'][cols], c['ps'][cols],
+                                   c['ts'][cols], c['tds'][cols], 500., source, 500., 1,
+                                   p1d=c['p'], start_lev=oracle_lib.start_levels(
+                                       c['p'], c['ps'][cols]), **kw)
+        ill, matched = ill_conditioned_and_matched(
+            run, off, [(warm[0][off], warm[1][off]), (plain[0][off], plain[1][off])])
+        assert ill.all() and matched.all(), (off, ill, matched)
     np.testing.assert_array_equal(warm[2], plain[2])
     live = c['ts'] > 0
     if source == 2:

@@ -19,7 +19,7 @@ import pytest
 sys.path.insert(0, os.path.join(os.path.dirname(__file__), '..'))
 from oracle import mlp_oracle as mo  # noqa: E402
 from oracle import oracle as orc  # noqa: E402
-from tolerances import cape_ok  # noqa: E402
+from tolerances import cape_ok, ill_conditioned_and_matched  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
@@ -221,9 +221,11 @@ def test_committed_surrogate_gain_full_size(torch):
     generated on the device): against extrapolation alone, the same
     sub-steps, fewer than half the iterations on the first sub-steps it
     serves, fewer iterations in all; and CAPE within the tolerance of
-    extrapolation's on every column, CIN too apart from slivers of positive
-    area below 0.05 J/kg (there the reference's CIN hangs on a near-zero
-    buoyancy crossing, and either run is as close to the reference)."""
+    extrapolation's on every column, CIN too apart from columns where the
+    reference's own CIN is ill-conditioned (committed at a near-zero
+    buoyancy crossing: it turns on the last bit of the C library or on the
+    iterate accepted within the convergence tolerance), on which each run is
+    within the tolerance of one of the oracle's FP64 or perturbed runs."""
     from xcape_amd import _hip, surrogate, synthetic
     surrogate.Surrogate.load().activate()
     n = 2048 * 1024
@@ -242,8 +244,18 @@ def test_committed_surrogate_gain_full_size(torch):
     assert st_s['iterations'] < st_e['iterations']
     cs, ce = out_s[0].cpu().numpy(), out_e[0].cpu().numpy()
     assert cape_ok(cs, ce).all()
-    off = ~cape_ok(out_s[1].cpu().numpy(), out_e[1].cpu().numpy())
-    print('CIN outside tolerance', int(off.sum()), flush=True)
-    assert off.sum() <= n // 100000
-    assert (np.maximum(cs[off], ce[off]) < 0.05).all()
+    ks, ke = out_s[1].cpu().numpy(), out_e[1].cpu().numpy()
+    off = np.nonzero(~cape_ok(ks, ke))[0]
+    print('CIN outside tolerance', off.tolist(), flush=True)
+    assert len(off) <= n // 100000
+    if len(off):
+        sel = torch.from_numpy(off).cuda()
+        host = [x[sel].cpu().numpy() for x in dev]
+
+        def run(cols, **kw):
+            j = np.searchsorted(off, cols)
+            return orc.cape(*(a[j] for a in host), pinc=500., source=2, adiabat=3, **kw)
+        ill, matched = ill_conditioned_and_matched(run, off, [(cs[off], ks[off]),
+                                                              (ce[off], ke[off])])
+        assert ill.all() and matched.all(), (off, ill, matched)
     np.testing.assert_array_equal(out_s[2].cpu().numpy(), out_e[2].cpu().numpy())

@@ -183,6 +183,32 @@ def test_perturbed_library_runs(oracle_lib):
         np.testing.assert_array_equal(q[2], o[2])
 
 
+def test_iterate_perturbed_runs(oracle_lib):
+    """The runs that also accept another point of each converged sub-step's
+    tolerance band: with amplitude 0 they are the library-perturbed runs bit
+    for bit; with 1e-4 K they move CAPE in many columns, by far less than the
+    tolerance; with 1 K, far beyond the band, the alternatives that fail the
+    convergence test are dropped, so the results stay as close (the band is
+    enforced by the test, not assumed)."""
+    from tolerances import cape_ok
+    c = synthetic.sigma_columns(2048, 40, 1000)
+    a = [c[k] for k in ('p', 't', 'td', 'ps', 'ts', 'tds')]
+    for src, adi in ((1, 1), (2, 3)):
+        lib = oracle_lib.cape(*a, 500., src, 500., adi, perturb=5)
+        z = oracle_lib.cape(*a, 500., src, 500., adi, perturb=5, perturb_iterate=0.0)
+        for x, y in zip(lib, z):
+            np.testing.assert_array_equal(x, y)
+        it = oracle_lib.cape(*a, 500., src, 500., adi, perturb=5, perturb_iterate=1e-4)
+        pos = lib[0] > 0
+        assert np.mean(it[0][pos] != lib[0][pos]) > 0.1
+        ok = ~((it[0] == 0) & (it[1] == 0) & pos)
+        assert np.mean(cape_ok(it[0][ok], lib[0][ok]) & cape_ok(it[1][ok], lib[1][ok])) > 0.999
+        np.testing.assert_array_equal(it[2], lib[2])
+        far = oracle_lib.cape(*a, 500., src, 500., adi, perturb=5, perturb_iterate=1.0)
+        ok = ~((far[0] == 0) & (far[1] == 0) & pos)
+        assert np.mean(cape_ok(far[0][ok], lib[0][ok]) & cape_ok(far[1][ok], lib[1][ok])) > 0.999
+
+
 @pytest.mark.parametrize('pinc', [100., 1000., 2500.])
 def test_cape_bitexact_vs_reference_soundings(oracle_lib, soundings, pinc):
     R = _ref()
