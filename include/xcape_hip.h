@@ -134,6 +134,13 @@ int xcape_release_host_staging(void);
  * t, td: (ncol, nk); ps, ts, tds: (ncol,).
  * Outputs (ncol,): cape, cin [J/kg], mulev (1-based, surface = 1, -999999 if
  * not most-unstable), zmulev [m].  Columns with ts <= 0 degC give zeros.
+ * Accuracy: CAPE and CIN within max(0.5 J/kg, 0.1 %) of the reference's
+ * FP32 results, with one exception: CIN where the reference's own CIN is
+ * ill-conditioned.  getcape commits the accumulated CIN when the parcel
+ * first enters positive buoyancy (CAPE_CODE_model_lev.f90:509-523); where a
+ * layer's buoyancy is within rounding of zero, an ulp of exp/log/pow or the
+ * iterate accepted inside the 2e-4 K band decides whether it is committed,
+ * and CIN may then differ by up to the whole negative area (CAPE does not).
  */
 int xcape_cape_f32(const float *p, int p_shared, const int32_t *start_lev,
                    const float *t, const float *td,

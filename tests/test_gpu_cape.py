@@ -405,20 +405,23 @@ def test_lfc_el_fp64(hip, oracle_lib):
             assert np.all(d <= 1e-3)
 
 
-def test_cin_disagreements_only_where_cape_vanishes(hip):
+def test_cin_disagreements_only_where_the_reference_is_ill_conditioned(hip, oracle_lib):
     """On the 2048x1024x137 grid, every variant's CAPE agrees with the plain
     accurate kernel within max(0.5 J/kg, 0.1 %) everywhere.  CIN may not
-    agree only where the positive area is a sliver: the reference commits
+    agree only where the reference's own CIN is ill-conditioned: it commits
     the accumulated CIN on the first entry into positive buoyancy
     (CAPE_CODE_model_lev.f90:509-523), so a layer whose buoyancy is within
-    rounding of zero commits it under one rounding and not under another.
-    Such columns have CAPE ~ 0 in every variant, and they are rare."""
+    rounding of zero commits it under one rounding, or one accepted iterate,
+    and not under another.  There the oracle's own runs (FP64, perturbed
+    library, perturbed accepted iterates) disagree beyond the tolerance, and
+    each variant is within tolerance of one of them.  Such columns are
+    rare."""
     import torch
     ncol = 2048 * 1024
     inp = synthetic.sigma_columns_torch(ncol, 137, 4004, device='cuda')
+    cols = [inp[k] for k in ('p', 't', 'td', 'ps', 'ts', 'tds')]
     lm = [inp[k].t().contiguous() for k in ('p', 't', 'td')] + \
          [inp[k] for k in ('ps', 'ts', 'tds')]
-    del inp
     kw = dict(pinc=500., source=2, adiabat=3, level_major=True)
     ref = hip.cape(*lm, **kw)
     for extra in (dict(warm_start=True), dict(fast_math=True),
@@ -428,12 +431,21 @@ def test_cin_disagreements_only_where_cape_vanishes(hip):
         dcape = (got[0].double() - ref[0].double()).abs()
         dcin = (got[1].double() - ref[1].double()).abs()
         assert bool((dcape <= tol[0]).all()), extra
-        off = dcin > tol[1]
-        n_off = int(off.sum())
-        print(extra, 'CIN outside tolerance:', n_off, flush=True)
-        assert n_off <= ncol // 10000, extra
-        if n_off:
-            assert float(torch.maximum(got[0][off], ref[0][off]).max()) < 0.05, extra
+        off = torch.nonzero(dcin > tol[1]).flatten()
+        print(extra, 'CIN outside tolerance:', off.tolist(), flush=True)
+        assert len(off) <= ncol // 10000, extra
+        if len(off):
+            host = [x[off].cpu().numpy() for x in cols]
+            o = off.cpu().numpy()
+
+            def run(c, **kw):
+                j = np.searchsorted(o, c)
+                return oracle_lib.cape(*(a[j] for a in host), pinc=500., source=2, adiabat=3,
+                                       **kw)
+            ill, matched = ill_conditioned_and_matched(
+                run, o, [(got[0][off].cpu().numpy(), got[1][off].cpu().numpy()),
+                         (ref[0][off].cpu().numpy(), ref[1][off].cpu().numpy())])
+            assert ill.all() and matched.all(), (extra, o, ill, matched)
     torch.cuda.synchronize()
 
 

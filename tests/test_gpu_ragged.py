@@ -3,7 +3,7 @@ replays that keep matching the direct launch."""
 import numpy as np
 import pytest
 
-from tests.tolerances import cape_ok
+from tests.tolerances import cape_ok, ill_conditioned_and_matched
 from xcape_amd import synthetic
 
 pytestmark = pytest.mark.gpu
@@ -128,9 +128,12 @@ def test_ragged_warm_start(batch_np, oracle_lib):
 def test_ragged_full_size(hip, oracle_lib):
     """The ragged configuration at its full size, 200 000 columns: the plain
     plan equals, column for column, the grid kernel run on each (level count,
-    source, adiabat) group; the warm plan agrees with it within tolerance
-    (CIN only where CAPE vanishes: the reference's CIN is ill-conditioned
-    there); a random sample of 2000 columns against the oracle."""
+    source, adiabat) group; the warm plan agrees with it within tolerance,
+    CIN apart from columns where the reference's own CIN is ill-conditioned
+    (its oracle runs in FP64 and with perturbed library or accepted iterates
+    disagree beyond the tolerance), on which both plans are within tolerance
+    of one of those runs; a random sample of 2000 columns against the
+    oracle."""
     import torch
 
     from xcape_amd.ragged import RaggedBatch, RaggedPlan
@@ -157,10 +160,23 @@ def test_ragged_full_size(hip, oracle_lib):
         for j in range(4):
             np.testing.assert_array_equal(plain[j][idx], g[j])
     assert cape_ok(warm[0], plain[0]).all()
-    off = ~cape_ok(warm[1], plain[1])
-    print('ragged full size: CIN outside tolerance', int(off.sum()), flush=True)
-    assert off.sum() <= n // 10000
-    assert (np.maximum(warm[0][off], plain[0][off]) < 0.05).all()
+    off = np.nonzero(~cape_ok(warm[1], plain[1]))[0]
+    print('ragged full size: CIN outside tolerance', off.tolist(), flush=True)
+    assert len(off) <= n // 10000
+    if len(off):
+        def run(cols, **kw):
+            outs = []
+            for i in cols:
+                rows = np.arange(d['offsets'][i], d['offsets'][i + 1])
+                outs.append(oracle_lib.cape(d['p'][None, rows], d['t'][None, rows],
+                                            d['td'][None, rows], d['ps'][i:i + 1],
+                                            d['ts'][i:i + 1], d['tds'][i:i + 1], pinc=500.,
+                                            source=int(d['source'][i]), ml_depth=500.,
+                                            adiabat=int(d['adiabat'][i]), **kw))
+            return [np.concatenate([o[j] for o in outs]) for j in range(2)]
+        ill, matched = ill_conditioned_and_matched(
+            run, off, [(warm[0][off], warm[1][off]), (plain[0][off], plain[1][off])])
+        assert ill.all() and matched.all(), (off, ill, matched)
     sample = np.sort(np.random.default_rng(3).choice(n, 2000, replace=False))
     sub = {k: d[k][sample] for k in ('nlev', 'source', 'adiabat', 'ps', 'ts', 'tds')}
     skeys = np.stack([sub['nlev'], sub['source'], sub['adiabat']], 1)
