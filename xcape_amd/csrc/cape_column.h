@@ -272,8 +272,12 @@ template <typename T> struct Parcel {
 
 // An observer of the level environments the source scans evaluate:
 // sink(k, env) for each level k they compute (a caller that also needs the
-// environments keeps them instead of computing them again).
+// environments keeps them instead of computing them again).  With
+// every_level, the most-unstable scan also evaluates (and passes on) the
+// levels above 500 hPa, which it reads for their pressure anyway: a caller
+// that needs every level's environment then reads each level once.
 struct NoEnvSink {
+    static constexpr bool every_level = false;
     template <typename E> XC_HD __forceinline__ void operator()(int, const E &) const {}
 };
 
@@ -302,21 +306,24 @@ XC_HD __forceinline__ bool parcel_source(const C &col, int nk, int source, T ml_
                 const Raw cur = nx;
                 if (k + 1 < nk) nx = load_raw(col, k + 1);
                 T pk = T(100.0) * T(cur.p);
-                if (pk >= T(50000.0)) {
+                const bool scan = pk >= T(50000.0);
+                if (scan || Sink::every_level) {
                     Env<T> e = env_from<T, MATH>(cur);
                     sink(k, e);
-                    if (k > 0 && k < kskip) zrun = zrun + layer_dz(e.thv, ep.thv, e.pi, ep.pi);
-                    ep = e;
-                    T the = thetae<T, MATH>(e.p, e.t, e.td, e.q);
-                    if (the > maxthe) {
-                        pc.mulvl = k + 1;
-                        maxthe = the;
-                        kmax = k;
-                        zmax = zrun;
+                    if (scan) {
+                        if (k > 0 && k < kskip)
+                            zrun = zrun + layer_dz(e.thv, ep.thv, e.pi, ep.pi);
+                        ep = e;
+                        T the = thetae<T, MATH>(e.p, e.t, e.td, e.q);
+                        if (the > maxthe) {
+                            pc.mulvl = k + 1;
+                            maxthe = the;
+                            kmax = k;
+                            zmax = zrun;
+                        }
                     }
-                } else if (kskip == nk) {
-                    kskip = k;
                 }
+                if (!scan && kskip == nk) kskip = k;
             }
             if (kmax < kskip) {
                 zk = zmax;

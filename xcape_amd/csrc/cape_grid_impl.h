@@ -249,6 +249,30 @@ struct EnvCol {
     __device__ __forceinline__ float p_at(int k) const { return p[(uint32_t)k * ls + col]; }
 };
 
+// The setup pass's sink of the source scan's environments: each level's
+// record, stored as the scan evaluates the levels 0, 1, ... in order (*cover:
+// the levels stored so far, *p_cov the pressure of the last).  It asks the
+// most-unstable scan for every level, so that the pass reads each level of
+// the profiles once.
+struct RecordSink {
+    static constexpr bool every_level = true;
+    float4 *env;
+    float *envp;
+    uint32_t ls, j;
+    float pinc;
+    int *cover;
+    float *p_cov;
+    __device__ __forceinline__ void operator()(int k, const Env<float> &e) const
+    {
+        if (k == *cover) {
+            env[(uint32_t)k * ls + j] = level_rec(level_from<float, XC_MATH>(e, *p_cov, pinc));
+            if (envp) envp[(uint32_t)k * ls + j] = e.p;
+            *cover = k + 1;
+            *p_cov = e.p;
+        }
+    }
+};
+
 template <bool ICE, bool PSEUDO, bool SUR, bool STATS>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
 warm_setup_kernel(CapeGridArgs a)
@@ -271,14 +295,7 @@ warm_setup_kernel(CapeGridArgs a)
     int cover = 0;
     T p_cov = T(NAN);
     if (ran) {
-        const auto sink = [&](int k, const Env<T> &e) {
-            if (k == cover) {
-                env[(uint32_t)k * ls + j] = level_rec(level_from<T, XC_MATH>(e, p_cov, pinc));
-                if (envp) envp[(uint32_t)k * ls + j] = e.p;
-                cover = k + 1;
-                p_cov = e.p;
-            }
-        };
+        const RecordSink sink{env, envp, ls, j, pinc, &cover, &p_cov};
         const int source = a.src_col ? a.src_col[col] : a.source;
         ok = parcel_source<T, XC_MATH>(c, nk, source, T(a.ml_depth), pc, sink);
         a.mulvl[out_col(a, col)] = pc.mulvl;
@@ -377,8 +394,11 @@ __host__ __device__ __forceinline__ uint32_t work_parts(uint32_t grid)
 
 // PREDICT: the warm start (false: the same loop with every sub-step cold).
 // LEVELS: also the LFC and EL.
+// Held to 72 VGPRs, 7 waves per SIMD: the ascent is bound by VALU issue (4
+// waves per SIMD are within 6 % of 8), and at 64 VGPRs the compiler spilled
+// scalar registers to VGPR lanes and shuffled more (7 waves ran 1.5 % faster).
 template <bool ICE, bool PSEUDO, bool PREDICT, bool STATS, bool LEVELS = false>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 7)))
 warm_ascent_kernel(CapeGridArgs a)
 {
     using T = float;
