@@ -540,3 +540,27 @@ def test_pressure_grid_full_size(hip, oracle_lib, pressure_grid, source):
     assert not _report('pressure grid sample cape', plain[0][idx], o[0]).any()
     assert not _report('pressure grid sample cin', plain[1][idx], o[1]).any()
     np.testing.assert_array_equal(plain[2][idx], o[2])
+
+
+def test_staging_of_unaligned_and_even_level_inputs(hip):
+    """The staging transpose copies 16 bytes per lane when the profiles are
+    16-byte aligned and the level count odd, 4 bytes otherwise: device
+    inputs starting 4 bytes into an allocation, and an even level count,
+    give the results of aligned copies of the same columns."""
+    import torch
+    for nk in (137, 40):
+        c = synthetic.sigma_columns(3001, nk, 77)
+        args = (c['p'], c['t'], c['td'], c['ps'], c['ts'], c['tds'])
+        kw = dict(pinc=500., source=2, adiabat=3, warm_start=True)
+        ref = [x.cpu() for x in hip.cape(*[torch.from_numpy(a).cuda() for a in args], **kw)]
+        # the same columns at an odd float offset into a larger buffer
+        shifted = []
+        for a in args[:3]:
+            buf = torch.empty(a.size + 1, dtype=torch.float32, device='cuda')
+            view = buf[1:].view(a.shape)
+            view.copy_(torch.from_numpy(a))
+            assert view.data_ptr() % 16 != 0
+            shifted.append(view)
+        got = hip.cape(*shifted, *[torch.from_numpy(a).cuda() for a in args[3:]], **kw)
+        for x, y in zip(got, ref):
+            assert torch.equal(x.cpu(), y), nk

@@ -75,6 +75,38 @@ def test_sigma_synthetic_srh(hip, oracle_lib, depth):
     _cmp_srh(g, o, f'sigma depth {depth}')
 
 
+def test_levels_above_the_outputs(hip, oracle_lib):
+    """Past the helicity's top and the last Bunkers level (6 km) the kernel
+    only checks that heights keep rising, and walks a column again in full
+    where they do not.  Columns that make it do so, against the oracle's
+    full walk: an upper level's pressure set near the surface's (the heights
+    drop below 6 km and climb through the Bunkers levels again, whose last
+    brackets then win), a NaN temperature at the top, and the upper levels'
+    pressures above the surface's (the last level below the surface: the
+    reference's decreasing orientation).  The untouched columns too."""
+    c = synthetic.sigma_columns(1024, 137, 1005, winds=True)
+    p, t = c['p'].copy(), c['t'].copy()
+    p[0:100, 120] = c['ps'][0:100] * 0.9
+    t[100:200, 136] = np.nan
+    p[200:300, 100:] = c['ps'][200:300, None] * 20.0
+    args = [p, t] + [c[k] for k in ('td', 'u', 'v', 'ps', 'ts', 'tds', 'us', 'vs')]
+    g = hip.srh(*args, depth=3000.)
+    o = oracle_lib.srh(*args, 3000.)
+    for x, y in zip(g, o):
+        np.testing.assert_array_equal(np.isnan(x), np.isnan(y))
+    np.testing.assert_almost_equal(g[0], o[0], 5)
+    np.testing.assert_almost_equal(g[1], o[1], 5)
+    for x, y in zip(g[2:], o[2:]):
+        fin = np.isfinite(y)
+        np.testing.assert_array_equal(x[fin], y[fin])
+    # the first group's Bunkers levels were crossed again above the dropped
+    # level, and the third group's column turned over: both change the motion
+    ref = oracle_lib.srh(c['p'], c['t'], *args[2:], 3000.)
+    assert np.any(o[2][0:100] != ref[2][0:100])
+    assert np.any(o[2][200:300] != ref[2][200:300])
+    _cmp_srh(g, o, 'levels above the outputs')
+
+
 def test_pressure_synthetic_srh(hip, oracle_lib):
     c = synthetic.pressure_columns(2048, 1002, winds=True)
     st = oracle_lib.start_levels(c['p'], c['ps'])

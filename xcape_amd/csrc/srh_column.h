@@ -46,6 +46,22 @@ __device__ __forceinline__ double std_tv(double T, double Td, double P)
     return Tin * ((w + eps) / (eps * (1.0 + w)));
 }
 
+// std_tv's value to within about 1e-3 relative, in FP32 with the hardware
+// exp2/log2/reciprocal: only its sign is used (a level's height increment is
+// positive when its pressure is below the last one's and both virtual
+// temperatures are positive)
+__device__ __forceinline__ float std_tv_sign(float T, float Td, float P)
+{
+    const float eps = 0.6219800858985514f;
+    const float Tin = T + 273.15f;
+    const float Tdin = Td + 273.15f;
+    const float lg = __builtin_amdgcn_logf(Tdin) * 0.6931471805599453f;
+    const float ex = (53.49f - 6808.0f * __builtin_amdgcn_rcpf(Tdin)) - 5.09f * lg;
+    const float E = 6.112f * __builtin_amdgcn_exp2f(ex * 1.4426950408889634f);
+    const float w = eps * E * __builtin_amdgcn_rcpf(P - E);
+    return Tin * (w + eps) * __builtin_amdgcn_rcpf(eps * (1.0f + w));
+}
+
 // H(k) = H(k-1) + (R ((Tv(k) + Tv(k-1))/2) / g) ln(P(k)/P(k-1)) (:147, :155)
 __device__ __forceinline__ double std_dh(double tv, double tv_prev, double P, double P_prev)
 {
@@ -72,11 +88,7 @@ __device__ void srh_column(const C &col, int nk, double depth, double hs, SrhOut
     // ---- pass 1: heights and Bunkers brackets (increasing orientation),
     // and the helicity sum's storm-motion-free part ----
     const double Ps = (double)col.P(0);
-    double tv_prev = std_tv((double)col.Tc(0), (double)col.Tdc(0), Ps);
-    double P_prev = Ps;
-    double H = hs;
-    TB z_prev = (TB)hs;
-    TB u_prev = (TB)col.U(0), v_prev = (TB)col.V(0);
+    const double tv0 = std_tv((double)col.Tc(0), (double)col.Tdc(0), Ps);
     const TB zsurf = (TB)hs;
     // SREH 0..depth (SREH_model_lev.f90:82-114) sums, over the levels up to
     // the first one above `depth` (its wind interpolated to `depth`),
@@ -85,52 +97,90 @@ __device__ void srh_column(const C &col, int nk, double depth, double hs, SrhOut
     // and the telescoping sums of (v - v') and (u - u'), v_top - v_0 and
     // u_top - u_0, so c enters at the end and the heights are computed once.
     // FP64; the split moves the sum by rounding only.
-    double s0 = 0.0, z1prev = hs, ut_prev = (double)col.U(0), vt_prev = (double)col.V(0);
-    bool reached = false;
-    for (int k = 1; k < nk; ++k) {
-        const double Pk = (double)col.P(k);
-        const double tv = std_tv((double)col.Tc(k), (double)col.Tdc(k), Pk);
-        H = H + std_dh(tv, tv_prev, Pk, P_prev);
-        const TB z = (TB)H;
-        // Bunkers reads the winds in its own precision; the helicity in
-        // FP64 from the inputs as given (both as the reference's wrappers
-        // pass them)
-        const TB uk = (TB)col.U(k), vk = (TB)col.V(k);
+    //
+    // No output depends on a level above both the helicity's top and the
+    // last Bunkers level (6 km) but through the orientation test after the
+    // walk (the height of the column's last level against the surface's):
+    // past them the first walk only checks that the heights keep rising
+    // (each level's pressure below the last one's, its virtual temperature
+    // positive, to a 1 K margin), which settles that test, and skips their
+    // FP64 logs and exp.  Where the check fails the column is walked again
+    // in full, so the outputs are those of the full walk in every case.
+    double tv_prev, P_prev, H, s0, z1prev, ut_prev, vt_prev;
+    TB z_prev, u_prev, v_prev;
+    bool reached, rising = false;
+    for (int walk = 0; walk < 2; ++walk) {
 #pragma unroll
-        for (int i = 0; i < 12; ++i) {
-            if (z_prev <= lv[i] && z > lv[i]) {
-                const TB w2 = (lv[i] - z_prev) / (z - z_prev);
-                const TB w1 = (TB)(1.0 - (double)w2);
-                iu[i] = w1 * u_prev + w2 * uk;
-                iv[i] = w1 * v_prev + w2 * vk;
+        for (int i = 0; i < 12; ++i) { iu[i] = TB(-999999.0); iv[i] = TB(-999999.0); }
+        tv_prev = tv0; P_prev = Ps; H = hs;
+        z_prev = zsurf; u_prev = (TB)col.U(0); v_prev = (TB)col.V(0);
+        s0 = 0.0; z1prev = hs; ut_prev = (double)col.U(0); vt_prev = (double)col.V(0);
+        reached = false;
+        rising = false;
+        bool failed = false;
+        float p_last = 0.0f;
+        for (int k = 1; k < nk; ++k) {
+            if (rising) {
+                const float pk = col.P(k);
+                if (pk < p_last && std_tv_sign(col.Tc(k), col.Tdc(k), pk) > 1.0f) {
+                    p_last = pk;
+                    continue;
+                }
+                failed = true;
+                break;
+            }
+            const double Pk = (double)col.P(k);
+            const double tv = std_tv((double)col.Tc(k), (double)col.Tdc(k), Pk);
+            H = H + std_dh(tv, tv_prev, Pk, P_prev);
+            const TB z = (TB)H;
+            // Bunkers reads the winds in its own precision; the helicity in
+            // FP64 from the inputs as given (both as the reference's wrappers
+            // pass them)
+            const TB uk = (TB)col.U(k), vk = (TB)col.V(k);
+#pragma unroll
+            for (int i = 0; i < 12; ++i) {
+                if (z_prev <= lv[i] && z > lv[i]) {
+                    const TB w2 = (lv[i] - z_prev) / (z - z_prev);
+                    const TB w1 = (TB)(1.0 - (double)w2);
+                    iu[i] = w1 * u_prev + w2 * uk;
+                    iv[i] = w1 * v_prev + w2 * vk;
+                }
+            }
+            if (!reached) {
+                double ut = (double)col.U(k), vt = (double)col.V(k);
+                const bool top = H > depth;
+                if (top) {
+                    // INTERP1 (SREH_model_lev.f90:123-129): X1 = aglh(ktop) is
+                    // lowered by 0.01 (a REAL literal) when it equals aglh(ktop-1)
+                    double x1 = H;
+                    if (z1prev == x1) x1 = x1 - XC_SPD(0.01);
+                    ut = ut + ((ut_prev - ut) * ((depth - x1) / (z1prev - x1)));
+                    vt = vt + ((vt_prev - vt) * ((depth - x1) / (z1prev - x1)));
+                }
+                s0 = s0 + ((ut * (vt - vt_prev)) - (vt * (ut - ut_prev)));
+                reached = top;
+                // past the top, (ut_prev, vt_prev) keep its interpolated wind
+                z1prev = H; ut_prev = ut; vt_prev = vt;
+            }
+            z_prev = z; u_prev = uk; v_prev = vk;
+            tv_prev = tv; P_prev = Pk;
+            // every output's levels behind and this level above the surface:
+            // from here on the first walk checks the rest only
+            if (walk == 0 && reached && z > lv[11] && z > zsurf && tv > 1.0) {
+                rising = true;
+                p_last = (float)Pk;
             }
         }
-        if (!reached) {
-            double ut = (double)col.U(k), vt = (double)col.V(k);
-            const bool top = H > depth;
-            if (top) {
-                // INTERP1 (SREH_model_lev.f90:123-129): X1 = aglh(ktop) is
-                // lowered by 0.01 (a REAL literal) when it equals aglh(ktop-1)
-                double x1 = H;
-                if (z1prev == x1) x1 = x1 - XC_SPD(0.01);
-                ut = ut + ((ut_prev - ut) * ((depth - x1) / (z1prev - x1)));
-                vt = vt + ((vt_prev - vt) * ((depth - x1) / (z1prev - x1)));
-            }
-            s0 = s0 + ((ut * (vt - vt_prev)) - (vt * (ut - ut_prev)));
-            reached = top;
-            // past the top, (ut_prev, vt_prev) keep its interpolated wind
-            z1prev = H; ut_prev = ut; vt_prev = vt;
-        }
-        z_prev = z; u_prev = uk; v_prev = vk;
-        tv_prev = tv; P_prev = Pk;
+        if (!failed) break;
     }
-    if (zsurf > z_prev) {
+    // (a walk that went on rising ends above the surface: the test is false)
+    if (!rising && zsurf > z_prev) {
         // heights decrease with k (IP = 1, IM = 0 in the reference): the
         // bracket is z(k) <= h < z(k-1); the top-down search keeps the
         // highest k, i.e. the last bracket of this bottom-up walk
 #pragma unroll
         for (int i = 0; i < 12; ++i) { iu[i] = TB(-999999.0); iv[i] = TB(-999999.0); }
-        tv_prev = std_tv((double)col.Tc(0), (double)col.Tdc(0), Ps);
+        tv_prev = tv0;
         P_prev = Ps;
         H = hs;
         z_prev = (TB)hs;

@@ -57,7 +57,12 @@ __global__ void __launch_bounds__(256) transpose_levels_kernel(TransposeArgs a)
 constexpr int SLAB = 64;          // columns per workgroup of the slab kernel
 constexpr int SLAB_MAX_PITCH = 255;
 
-// blockIdx.x: 64-column slab, blockIdx.y: array; LDS: SLAB * pitch floats
+// blockIdx.x: 64-column slab, blockIdx.y: array; LDS: SLAB * pitch floats.
+// VEC4 (16-byte aligned sources, nk odd, pitch == nk): the run goes into LDS
+// as it lies, 16 bytes per lane on both sides (column c's level k at
+// c * nk + k; the run of slab j starts 256 nk bytes into the array), and the
+// odd nk keeps the column-wise reads below on distinct banks.
+template <bool VEC4>
 __global__ void __launch_bounds__(256) transpose_slab_kernel(TransposeArgs a, int pitch)
 {
     extern __shared__ float slab[];
@@ -68,17 +73,26 @@ __global__ void __launch_bounds__(256) transpose_slab_kernel(TransposeArgs a, in
     const int nc = (int)(a.ncol - c0 < SLAB ? a.ncol - c0 : SLAB);
     const int total = nc * nk;
     const float *s = src + c0 * a.nk;
-    // element e of the run is level k of column c: walk (c, k) along with e
-    int c = threadIdx.x / nk, k = threadIdx.x - c * nk;
-    const int dc = 256 / nk, dk = 256 - dc * nk;
+    if constexpr (VEC4) {
+        const int total4 = total >> 2;
+        const float4 *s4 = reinterpret_cast<const float4 *>(s);
+        float4 *l4 = reinterpret_cast<float4 *>(slab);
 #pragma unroll 4
-    for (int e = threadIdx.x; e < total; e += 256) {
-        slab[c * pitch + k] = s[e];
-        k += dk;
-        c += dc;
-        if (k >= nk) {
-            k -= nk;
-            ++c;
+        for (int q = threadIdx.x; q < total4; q += 256) l4[q] = s4[q];
+        for (int e = (total4 << 2) + (int)threadIdx.x; e < total; e += 256) slab[e] = s[e];
+    } else {
+        // element e of the run is level k of column c: walk (c, k) along with e
+        int c = threadIdx.x / nk, k = threadIdx.x - c * nk;
+        const int dc = 256 / nk, dk = 256 - dc * nk;
+#pragma unroll 4
+        for (int e = threadIdx.x; e < total; e += 256) {
+            slab[c * pitch + k] = s[e];
+            k += dk;
+            c += dc;
+            if (k >= nk) {
+                k -= nk;
+                ++c;
+            }
         }
     }
     __syncthreads();
@@ -110,8 +124,14 @@ hipError_t launch_transpose_levels(const float *const *src, float *const *dst, i
     if (pitch <= SLAB_MAX_PITCH) {
         const int64_t sx = (ncol + SLAB - 1) / SLAB;
         if (sx > 0x7fffffff) return hipErrorInvalidValue;
-        transpose_slab_kernel<<<dim3((unsigned)sx, (unsigned)narr), 256,
-                                sizeof(float) * SLAB * (size_t)pitch, s>>>(a, (int)pitch);
+        bool vec4 = (nk & 1) != 0;
+        for (int i = 0; i < narr; ++i) vec4 = vec4 && (reinterpret_cast<uintptr_t>(src[i]) % 16 == 0);
+        const dim3 grid((unsigned)sx, (unsigned)narr);
+        const size_t lds = sizeof(float) * SLAB * (size_t)pitch;
+        if (vec4)
+            transpose_slab_kernel<true><<<grid, 256, lds, s>>>(a, (int)pitch);
+        else
+            transpose_slab_kernel<false><<<grid, 256, lds, s>>>(a, (int)pitch);
     } else {
         transpose_levels_kernel<<<dim3((unsigned)gx, (unsigned)gy, (unsigned)narr), 256, 0, s>>>(
             a);
