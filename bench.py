@@ -280,19 +280,26 @@ def full_grid_agreement(outs, ref):
 
 def accuracy(outs, inp, sample, fp64_outs):
     """Outputs of the first `sample` columns against the FP32 oracle (CAPE,
-    CIN, MUlev, zMUlev, SRH, Bunkers) and against the FP64 re-run of the GPU
-    kernel (CAPE, CIN, MUlev, zMUlev)."""
+    CIN, MUlev, zMUlev, SRH, Bunkers), against the oracle's FP64 run (CAPE,
+    CIN) and against the FP64 re-run of the GPU kernel (CAPE, CIN, MUlev,
+    zMUlev)."""
     from oracle import oracle as O
     a = {k: v[:sample].cpu().numpy() for k, v in inp.items()}
     o = O.cape(a['p'], a['t'], a['td'], a['ps'], a['ts'], a['tds'], PINC, SOURCE, 500., ADIABAT)
+    o64 = O.cape(*(a[k].astype(np.float64) for k in ('p', 't', 'td', 'ps', 'ts', 'tds')), PINC,
+                 SOURCE, 500., ADIABAT, fp64=True)
     g = [x[:sample].cpu().numpy() for x in outs]
     f64 = [x[:sample].cpu().numpy() for x in fp64_outs]
     res = {}
     for i, name in enumerate(('cape', 'cin')):
         d = np.abs(g[i].astype(np.float64) - o[i])
         tol = np.maximum(0.5, 1e-3 * np.abs(o[i].astype(np.float64)))
+        d64 = np.abs(g[i].astype(np.float64) - o64[i])
         res[name] = {'max_abs_err': float(d.max()), 'frac_within_tol': float(np.mean(d <= tol)),
-                     'max_abs_err_vs_fp64_gpu': float(np.abs(g[i] - f64[i]).max())}
+                     'max_abs_err_vs_fp64_gpu': float(np.abs(g[i] - f64[i]).max()),
+                     'max_abs_err_vs_fp64_oracle': float(d64.max()),
+                     'frac_within_tol_vs_fp64_oracle':
+                         float(np.mean(d64 <= np.maximum(0.5, 1e-3 * np.abs(o64[i]))))}
     res['mulev'] = {'frac_equal': float(np.mean(g[2] == o[2])),
                     'frac_equal_fp64_gpu': float(np.mean(g[2] == f64[2]))}
     res['zmulev'] = {'max_abs_err': float(np.abs(g[3].astype(np.float64) - o[3]).max()),
