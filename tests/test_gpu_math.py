@@ -16,7 +16,11 @@ correctly rounded reciprocal, and take exp and pow from short kernels
 * the exp kernel equals the device library's expf bit for bit on (-80, 80);
 * pow stays within the device library's powf accuracy: its error against the
   exactly rounded x^y (from float64 on the host) is at most 1 ulp over the
-  Exner function's and theta-e's ranges, and no worse than powf's.
+  Exner function's and theta-e's ranges, and no worse than powf's;
+* the helicity chain's FP64 log (xcape_math_check_f64) is within an ulp of
+  ln x (long double on the host) over the doubles' whole positive range, the
+  heights' arguments (level pressure ratios, dewpoints in K) included, and
+  gives the C library's ln 0, ln inf and NaN.
 """
 import numpy as np
 import pytest
@@ -125,3 +129,40 @@ def test_pow_kernel_accuracy(dev, y, lo, hi):
           f'{np.mean(lib == cr):.5f}', flush=True)
     assert e_mine.max() <= 1.0
     assert e_mine.max() <= max(e_lib.max(), 0.75)
+
+
+def _ulp_err64(got, x):
+    """|got - ln x| in ulps of the double nearest to ln x (long double)."""
+    exact = np.log(x.astype(np.longdouble))
+    ulp = np.spacing(np.abs(exact.astype(np.float64))).astype(np.longdouble)
+    return np.asarray(np.abs(got.astype(np.longdouble) - exact) / ulp, np.float64)
+
+
+def test_log_kernel_f64_within_an_ulp(dev):
+    torch, H = dev
+    rng = np.random.default_rng(17)
+    n = 1 << 21
+    x = 10.0 ** rng.uniform(-307, 308, n)
+    q = n // 8
+    x[:q] = rng.uniform(0.85, 1.0, q)                   # level pressure ratios
+    x[q:2 * q] = rng.uniform(1.0, 1.2, q)
+    x[2 * q:3 * q] = rng.uniform(150.0, 350.0, q)      # dewpoints [K]
+    # the reduction's edges: around sqrt(1/2) and 1 at every exponent
+    e = rng.integers(-1000, 1000, q).astype(np.int32)
+    edge = rng.choice([np.sqrt(0.5), 1.0, 2.0 * np.sqrt(0.5)], q)
+    x[3 * q:4 * q] = np.ldexp(edge, e) * (1.0 + rng.integers(-4, 5, q) * 2.0 ** -52)
+    x[4 * q:4 * q + 1000] = 2.0 ** -1074 * rng.integers(1, 1 << 40, 1000)   # subnormal
+    tx = torch.from_numpy(x).cuda()
+    mine = H.math_check_f64(0, tx).cpu().numpy()
+    lib = H.math_check_f64(1, tx).cpu().numpy()
+    e_mine, e_lib = _ulp_err64(mine, x), _ulp_err64(lib, x)
+    cr = np.log(x.astype(np.longdouble)).astype(np.float64)
+    print(f'log f64 kernel: max {e_mine.max():.3f} ulp, correctly rounded '
+          f'{np.mean(mine == cr):.5f}; library max {e_lib.max():.3f} ulp, correctly rounded '
+          f'{np.mean(lib == cr):.5f}', flush=True)
+    assert e_mine.max() < 1.0
+    sp = np.array([0.0, -0.0, np.inf, -np.inf, -1.0, np.nan, 1.0, 2.0])
+    got = H.math_check_f64(0, torch.from_numpy(sp).cuda()).cpu().numpy()
+    want = np.array([-np.inf, -np.inf, np.inf, np.nan, np.nan, np.nan, 0.0, np.log(2.0)])
+    np.testing.assert_array_equal(got, want)
+

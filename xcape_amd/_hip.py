@@ -101,6 +101,7 @@ SIGNATURES = {
     'xcape_surrogate_log_f32': (_I, [_P, _P, _P, _P, _P, _P, _I64, _I64, _F, _I, _F, _I, _F,
                                      _U32, _I, _P, _I64, _P, _P]),
     'xcape_math_check': (_I, [_I, _P, _P, _I64, _P, _P]),
+    'xcape_math_check_f64': (_I, [_I, _P, _I64, _P, _P]),
 }
 
 
@@ -353,4 +354,15 @@ def math_check(fn, a, b=None):
     y = torch.empty_like(a)
     stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
     check(lib().xcape_math_check(fn, _ptr(a), _ptr(b), a.numel(), _ptr(y), stream))
+    return y
+
+
+def math_check_f64(fn, a):
+    """xcape_math_check_f64 on a device tensor: the helicity chain's FP64
+    log kernel (fn 0) or the device library's log (fn 1), elementwise."""
+    import torch
+    a = a.to(torch.float64).contiguous()
+    y = torch.empty_like(a)
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    check(lib().xcape_math_check_f64(fn, _ptr(a), a.numel(), _ptr(y), stream))
     return y

@@ -5,6 +5,7 @@
 // fn: 0 div_raw(a, b)   1 a / b (the compiler's IEEE division)
 //     2 div_const(a, b, RN(1/b))   3 exp_core(a)   4 expf(a)
 //     5 pow_core(a, b)   6 powf(a, b)
+// xcape_math_check_f64: fn 0 log_core_f64(a)   1 log(a) (FP64)
 #include <hip/hip_runtime.h>
 
 #include "../../include/xcape_hip.h"
@@ -34,6 +35,13 @@ __global__ void math_check_kernel(int fn, const float *a, const float *b, int64_
     y[i] = r;
 }
 
+__global__ void math_check_f64_kernel(int fn, const double *a, int64_t n, double *y)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    y[i] = fn == 0 ? log_core_f64(a[i]) : log(a[i]);
+}
+
 }  // namespace
 
 extern "C" int xcape_math_check(int fn, const float *a, const float *b, int64_t n, float *y,
@@ -49,4 +57,17 @@ extern "C" int xcape_math_check(int fn, const float *a, const float *b, int64_t 
     math_check_kernel<<<(unsigned)((n + 255) / 256), 256, 0, (hipStream_t)stream>>>(fn, a, b, n, y);
     const hipError_t e = hipGetLastError();
     return e ? hip_fail(e, "math_check") : XCAPE_OK;
+}
+
+extern "C" int xcape_math_check_f64(int fn, const double *a, int64_t n, double *y, void *stream)
+{
+    if (fn < 0 || fn > 1) return fail(XCAPE_EINVAL, "fn=%d not in 0..1", fn);
+    if (n < 0 || n > ((int64_t)1 << 31)) return fail(XCAPE_EINVAL, "n=%lld out of range", (long long)n);
+    if (!a || !y) return fail(XCAPE_EINVAL, "NULL argument");
+    int rc = ensure_device();
+    if (rc) return rc;
+    if (n == 0) return XCAPE_OK;
+    math_check_f64_kernel<<<(unsigned)((n + 255) / 256), 256, 0, (hipStream_t)stream>>>(fn, a, n, y);
+    const hipError_t e = hipGetLastError();
+    return e ? hip_fail(e, "math_check_f64") : XCAPE_OK;
 }

@@ -163,6 +163,60 @@ __device__ __forceinline__ float pow_core(float x, float y)
     return __builtin_ldexpf(__builtin_amdgcn_exp2f(f), (int)n);
 }
 
+// A constant the compiler materialises where it is used (in scalar
+// registers) instead of holding it across a loop.
+__device__ __forceinline__ double xc_sconst(double c)
+{
+    __asm__ volatile("" : "+s"(c));
+    return c;
+}
+
+// ln x in FP64 for the helicity chain's heights, by the reduction of Sun's
+// fdlibm log (1993): x = 2^k m with m in [sqrt(1/2), sqrt(2)), f = m - 1
+// (exact), s = f / (2 + f), ln m = 2 atanh s = f - (f^2/2 - s (f^2/2 + R(s^2)))
+// with R(z) = sum 2 z^j / (2j + 1) (j >= 1) as z P(z), P the degree-6
+// Chebyshev fit of R(z)/z on [0, ((sqrt 2 - 1)/(sqrt 2 + 1))^2] (relative
+// error of the fit in ln m below 5e-18), and k ln 2 as k (LN2_HI + LN2_LO),
+// LN2_HI with 29 significant bits so that k LN2_HI is exact.  s by the
+// hardware reciprocal refined by two Newton steps (s enters only the
+// correction term).  Within an ulp of ln x for positive finite x (the
+// hardware's frexp normalises subnormals); ln 0 = -inf, ln inf = inf, and
+// NaN for negative x or NaN, as the C library's.
+__device__ __forceinline__ double log_core_f64(double x)
+{
+    if (__builtin_expect(!(x > 0.0 && x <= 0x1.fffffffffffffp+1023), 0))
+        return x == 0.0 ? -__builtin_inf() : (x > 0.0 ? x : __builtin_nan(""));
+    constexpr double LN2_HI = 0x1.62e42fe000000p-1;
+    constexpr double LN2_LO = 0x1.f473de6af278fp-30;
+    constexpr double C1 = 0x1.5555555555558p-1, C2 = 0x1.99999999952e2p-2,
+                     C3 = 0x1.2492492df148dp-2, C4 = 0x1.c71c62e5800a1p-3,
+                     C5 = 0x1.7462b4ab2ef6bp-3, C6 = 0x1.39fe606542ddep-3,
+                     C7 = 0x1.2b584aae78a57p-3;
+    double m = __builtin_amdgcn_frexp_mant(x);            // [1/2, 1)
+    int k = __builtin_amdgcn_frexp_exp(x);
+    if (m < 0x1.6a09e667f3bcdp-1) {
+        m = m + m;
+        k = k - 1;
+    }
+    const double f = m - 1.0;
+    const double b = 2.0 + f;
+    double y = __builtin_amdgcn_rcp(b);
+    y = __builtin_fma(__builtin_fma(-b, y, 1.0), y, y);
+    y = __builtin_fma(__builtin_fma(-b, y, 1.0), y, y);
+    const double s = f * y;
+    const double z = s * s;
+    double p = __builtin_fma(z, xc_sconst(C7), xc_sconst(C6));
+    p = __builtin_fma(z, p, xc_sconst(C5));
+    p = __builtin_fma(z, p, xc_sconst(C4));
+    p = __builtin_fma(z, p, xc_sconst(C3));
+    p = __builtin_fma(z, p, xc_sconst(C2));
+    p = __builtin_fma(z, p, xc_sconst(C1));
+    const double r = z * p;
+    const double hfsq = 0.5 * f * f;
+    const double dk = (double)k;
+    return dk * LN2_HI - ((hfsq - (s * (hfsq + r) + dk * LN2_LO)) - f);
+}
+
 // ---- elementary functions per precision and mode ---------------------------
 template <typename T, int MATH> struct Fn;
 

@@ -18,9 +18,14 @@
 //
 // Literal constants of the reference's DOUBLE PRECISION routines are default
 // REAL literals, so their values are those of the nearest float (SPD below).
+// The heights' two logarithms per level are log_core_f64 (phys.h), within an
+// ulp of ln like the C library's; they were half of the chain's instructions
+// with the device library's log.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include "phys.h"
 
 namespace xcape {
 
@@ -41,7 +46,7 @@ __device__ __forceinline__ double std_tv(double T, double Td, double P)
     const double eps = XC_SPD(0.6219800858985514);
     double Tin = T + XC_SPD(273.15);
     double Tdin = Td + XC_SPD(273.15);
-    double E = XC_SPD(6.112) * exp((XC_SPD(53.49) - (6808.0 / Tdin)) - XC_SPD(5.09) * log(Tdin));
+    double E = XC_SPD(6.112) * exp((XC_SPD(53.49) - (6808.0 / Tdin)) - XC_SPD(5.09) * log_core_f64(Tdin));
     double w = eps * (E / (P - E));
     return Tin * ((w + eps) / (eps * (1.0 + w)));
 }
@@ -68,7 +73,7 @@ __device__ __forceinline__ double std_dh(double tv, double tv_prev, double P, do
 #pragma clang fp contract(off)
     const double R = XC_SPD(287.04);
     const double g = XC_SPD(-9.80665);
-    return ((R * ((tv + tv_prev) / 2.0)) / g) * log(P / P_prev);
+    return ((R * ((tv + tv_prev) / 2.0)) / g) * log_core_f64(P / P_prev);
 }
 
 // Column accessor C: P(k), Tc(k), Tdc(k), U(k), V(k) for k = 0..nk-1 with
