@@ -46,7 +46,8 @@ __device__ __forceinline__ double std_tv(double T, double Td, double P)
     const double eps = XC_SPD(0.6219800858985514);
     double Tin = T + XC_SPD(273.15);
     double Tdin = Td + XC_SPD(273.15);
-    double E = XC_SPD(6.112) * exp((XC_SPD(53.49) - (6808.0 / Tdin)) - XC_SPD(5.09) * log_core_f64(Tdin));
+    double E = XC_SPD(6.112) *
+               exp((XC_SPD(53.49) - (6808.0 / Tdin)) - XC_SPD(5.09) * log_core_f64(Tdin));
     double w = eps * (E / (P - E));
     return Tin * ((w + eps) / (eps * (1.0 + w)));
 }

@@ -13,7 +13,10 @@
 // rows are padded to 65 words so that the transposed LDS reads hit distinct
 // banks.
 //
-// HBM traffic: 8 bytes per element (one read, one write).
+// HBM traffic: 8 bytes per element (one read, one write).  The slab kernel's
+// loads and stores are non-temporal: each element passes once, and the
+// level-major copy, larger than the caches, is read back from HBM by the next
+// kernels either way (the helicity-only call ran 0.1 ms faster so).
 #include <hip/hip_runtime.h>
 
 #include "cape_launch.h"
@@ -75,10 +78,11 @@ __global__ void __launch_bounds__(256) transpose_slab_kernel(TransposeArgs a, in
     const float *s = src + c0 * a.nk;
     if constexpr (VEC4) {
         const int total4 = total >> 2;
-        const float4 *s4 = reinterpret_cast<const float4 *>(s);
-        float4 *l4 = reinterpret_cast<float4 *>(slab);
+        typedef float v4f __attribute__((ext_vector_type(4)));
+        const v4f *s4 = reinterpret_cast<const v4f *>(s);
+        v4f *l4 = reinterpret_cast<v4f *>(slab);
 #pragma unroll 4
-        for (int q = threadIdx.x; q < total4; q += 256) l4[q] = s4[q];
+        for (int q = threadIdx.x; q < total4; q += 256) l4[q] = __builtin_nontemporal_load(&s4[q]);
         for (int e = (total4 << 2) + (int)threadIdx.x; e < total; e += 256) slab[e] = s[e];
     } else {
         // element e of the run is level k of column c: walk (c, k) along with e
@@ -100,7 +104,8 @@ __global__ void __launch_bounds__(256) transpose_slab_kernel(TransposeArgs a, in
     if (lane < nc) {
         float *d = dst + c0 + lane;
 #pragma unroll 4
-        for (int kk = w; kk < nk; kk += 4) d[(int64_t)kk * a.ncol] = slab[lane * pitch + kk];
+        for (int kk = w; kk < nk; kk += 4)
+            __builtin_nontemporal_store(slab[lane * pitch + kk], &d[(int64_t)kk * a.ncol]);
     }
 }
 
