@@ -17,10 +17,11 @@ correctly rounded reciprocal, and take exp and pow from short kernels
 * pow stays within the device library's powf accuracy: its error against the
   exactly rounded x^y (from float64 on the host) is at most 1 ulp over the
   Exner function's and theta-e's ranges, and no worse than powf's;
-* the helicity chain's FP64 log (xcape_math_check_f64) is within an ulp of
-  ln x (long double on the host) over the doubles' whole positive range, the
-  heights' arguments (level pressure ratios, dewpoints in K) included, and
-  gives the C library's ln 0, ln inf and NaN.
+* the helicity chain's FP64 log and exp (xcape_math_check_f64) are within an
+  ulp of ln x and e^x (long double on the host) over the doubles' whole
+  range, the heights' arguments (level pressure ratios, dewpoints in K, the
+  saturation vapour pressure's exponent) included, and give the C library's
+  special values.
 """
 import numpy as np
 import pytest
@@ -165,4 +166,39 @@ def test_log_kernel_f64_within_an_ulp(dev):
     got = H.math_check_f64(0, torch.from_numpy(sp).cuda()).cpu().numpy()
     want = np.array([-np.inf, -np.inf, np.inf, np.nan, np.nan, np.nan, 0.0, np.log(2.0)])
     np.testing.assert_array_equal(got, want)
+
+
+def test_exp_kernel_f64_within_an_ulp(dev):
+    torch, H = dev
+    rng = np.random.default_rng(19)
+    n = 1 << 21
+    x = rng.uniform(-708.0, 709.0, n)
+    q = n // 4
+    x[:q] = rng.uniform(-30.0, 0.0, q)            # the vapour pressure's exponent
+    x[q:2 * q] = rng.uniform(-1.0, 1.0, q)
+    # around the reduction's half-integer multiples of ln 2
+    j = rng.integers(-1000, 1000, q)
+    x[2 * q:3 * q] = (j + 0.5) * np.log(2.0) * (1.0 + rng.integers(-8, 9, q) * 2.0 ** -52)
+    tx = torch.from_numpy(x).cuda()
+    mine = H.math_check_f64(2, tx).cpu().numpy()
+    lib = H.math_check_f64(3, tx).cpu().numpy()
+    exact = np.exp(x.astype(np.longdouble))
+    ulp = np.spacing(exact.astype(np.float64)).astype(np.longdouble)
+    e_mine = np.asarray(np.abs(mine.astype(np.longdouble) - exact) / ulp, np.float64)
+    e_lib = np.asarray(np.abs(lib.astype(np.longdouble) - exact) / ulp, np.float64)
+    cr = exact.astype(np.float64)
+    print(f'exp f64 kernel: max {e_mine.max():.3f} ulp, correctly rounded '
+          f'{np.mean(mine == cr):.5f}; library max {e_lib.max():.3f} ulp, correctly rounded '
+          f'{np.mean(lib == cr):.5f}', flush=True)
+    assert e_mine.max() < 1.0
+    sp = np.array([np.nan, np.inf, -np.inf, 710.0, -746.0, 0.0, -0.0, -740.0, 709.7])
+    got = H.math_check_f64(2, torch.from_numpy(sp).cuda()).cpu().numpy()
+    want = H.math_check_f64(3, torch.from_numpy(sp).cuda()).cpu().numpy()
+    np.testing.assert_array_equal(got[:7], want[:7])
+    np.testing.assert_allclose(got[7:], want[7:], rtol=2.3e-16)
+    # subnormal results: within an ulp of the subnormal spacing
+    sub = -np.linspace(708.5, 744.0, 1001)
+    got = H.math_check_f64(2, torch.from_numpy(sub).cuda()).cpu().numpy()
+    ref = np.exp(sub.astype(np.longdouble))
+    assert np.all(np.abs(got.astype(np.longdouble) - ref) <= np.longdouble(2.0 ** -1074))
 

@@ -1,14 +1,16 @@
-"""The helicity chain's FP64 log kernel (log_core_f64 in phys.h), pinned on
-the host from its source text:
+"""The helicity chain's FP64 log and exp kernels (log_core_f64 and
+exp_core_f64 in phys.h), pinned on the host from their source text:
 
-* its constants by what the mathematics fixes: the polynomial P against
-  R(z)/z = sum 2 z^(j-1) / (2j + 1) in 50-digit arithmetic over the reduced
-  range (the fit's error in ln m below 5e-18), LN2_HI + LN2_LO = ln 2 with
-  k LN2_HI exact for every binary exponent, and the split point sqrt(1/2);
-* its steps by running them in IEEE double on the host (Python floats, fma
+* the constants by what the mathematics fixes: the log's polynomial P
+  against R(z)/z = sum 2 z^(j-1) / (2j + 1) and the exp's Q against
+  (e^r - 1 - r) / r^2, in 50-digit arithmetic over the reduced ranges (the
+  fits' errors below 5e-18 in ln m and 4e-19 in e^r), LN2_HI + LN2_LO = ln 2
+  with k LN2_HI exact for every binary exponent, 1/ln 2 and the split point
+  sqrt(1/2) correctly rounded;
+* the steps by running them in IEEE double on the host (Python floats, fma
   exact through rationals, 1/b correctly rounded where the device starts
-  from its reciprocal instruction) against ln x to 50 digits: within an ulp,
-  with the reduction's edges and both signs of ln m included.
+  from its reciprocal instruction) against ln x and e^x to 50 digits: within
+  an ulp, with the reductions' edges included.
 
 The device's own results are checked against the host's long double log in
 the GPU tests (test_gpu_math.py).
@@ -115,3 +117,66 @@ def test_a_wrong_constant_is_caught():
     x = 1.3
     err = abs(mp.mpf(_log_core(x, bad, split)) - mp.log(mp.mpf(x))) / math.ulp(math.log(x))
     assert err > 1.0
+
+
+def _exp_constants():
+    src = open(PHYS).read()
+    body = src[src.index('double exp_core_f64('):]
+    body = body[:body.index('\n}\n')]
+    vals = dict((k, float.fromhex(v)) for k, v in
+                re.findall(r'\b(INV_LN2|LN2_HI|LN2_LO|E10|E[0-9])\s*=\s*(0x[0-9a-fA-Fp.+-]+)', body))
+    assert len(vals) == 14, sorted(vals)
+    return vals
+
+
+def _exp_core(x, K):
+    """exp_core_f64's steps for -745.2 < x < 709.8."""
+    k = float(round(x * K['INV_LN2']))         # Python rounds half to even, as rint
+    hi = _fma(-k, K['LN2_HI'], x)
+    assert Fraction(hi) == Fraction(x) - Fraction(k) * Fraction(K['LN2_HI'])   # exact
+    lo = k * K['LN2_LO']
+    r = hi - lo
+    c = (hi - r) - lo
+    q = _fma(r, K['E10'], K['E9'])
+    for e in ('E8', 'E7', 'E6', 'E5', 'E4', 'E3', 'E2', 'E1', 'E0'):
+        q = _fma(r, q, K[e])
+    p = 1.0 + (r + _fma(r * r, q, c))
+    return math.ldexp(p, int(k))
+
+
+def test_exp_kernel_constants():
+    K = _exp_constants()
+    mp.mp.dps = 50
+    ln2 = mp.log(2)
+    assert K['INV_LN2'] == float(1 / ln2)
+    assert K['LN2_HI'] == _constants()[0]['LN2_HI'] and K['LN2_LO'] == _constants()[0]['LN2_LO']
+    half = ln2 / 2
+    worst = mp.mpf(0)
+    for i in range(-800, 801):
+        r = half * i / 800
+        q = mp.mpf(0)
+        for c in ('E10', 'E9', 'E8', 'E7', 'E6', 'E5', 'E4', 'E3', 'E2', 'E1', 'E0'):
+            q = q * r + mp.mpf(K[c])
+        worst = max(worst, abs(1 + r + r * r * q - mp.exp(r)) / mp.exp(r))
+    assert worst < 4e-19, worst
+
+
+def test_exp_kernel_steps_within_an_ulp():
+    K = _exp_constants()
+    mp.mp.dps = 50
+    rng = np.random.default_rng(29)
+    xs = list(rng.uniform(-708.0, 709.0, 4000)) + list(rng.uniform(-30.0, 0.0, 4000))
+    xs += list(rng.uniform(-1.0, 1.0, 1000))
+    for j in rng.integers(-1000, 1000, 300):
+        for d in (-3, -1, 0, 1, 3):
+            xs.append((float(j) + 0.5) * math.log(2.0) * (1.0 + d * 2.0 ** -52))
+    xs += [0.0, 1.0, -1.0, 709.7, -708.3]
+    worst = 0.0
+    for x in xs:
+        got = _exp_core(x, K)
+        exact = mp.exp(mp.mpf(x))
+        err = float(abs(mp.mpf(got) - exact)) / math.ulp(float(exact))
+        worst = max(worst, err)
+    print(f'exp kernel steps: max {worst:.3f} ulp over {len(xs)} arguments')
+    assert worst < 1.0
+

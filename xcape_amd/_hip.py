@@ -358,8 +358,9 @@ def math_check(fn, a, b=None):
 
 
 def math_check_f64(fn, a):
-    """xcape_math_check_f64 on a device tensor: the helicity chain's FP64
-    log kernel (fn 0) or the device library's log (fn 1), elementwise."""
+    """xcape_math_check_f64 on a device tensor, elementwise: the helicity
+    chain's FP64 log kernel (fn 0), the device library's log (1), the FP64 exp
+    kernel (2), the device library's exp (3)."""
     import torch
     a = a.to(torch.float64).contiguous()
     y = torch.empty_like(a)

@@ -521,8 +521,13 @@ __global__ void warm_reset_kernel(unsigned int *work, uint32_t ncol, uint32_t gr
 // precision (float: the reference's; double: the FP64 re-run).  TI: the
 // input element type (double: float64 profiles, which feed the FP64 heights
 // and helicity unrounded as the reference's f2py wrappers pass them).
+//
+// Held to 5 waves per SIMD (96 VGPRs; it needs 99 unconstrained, 4 waves): in
+// the combined call it shares the CUs with the ascent, and the step ran 1.4 %
+// faster so (the helicity-only call 4 % faster).
 template <typename TB, typename TI>
-__global__ void __launch_bounds__(256) srh_grid_kernel(CapeGridArgs a)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 5)))
+srh_grid_kernel(CapeGridArgs a)
 {
     const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     SigmaCol<uint64_t, TI> c{};

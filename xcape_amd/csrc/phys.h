@@ -217,6 +217,46 @@ __device__ __forceinline__ double log_core_f64(double x)
     return dk * LN2_HI - ((hfsq - (s * (hfsq + r) + dk * LN2_LO)) - f);
 }
 
+// e^x in FP64 for the helicity chain's heights: x = k ln 2 + r with k the
+// nearest integer to x / ln 2 and r = x - k (LN2_HI + LN2_LO) (|r| <= ln 2 /
+// 2; x - k LN2_HI is exact, and c carries the rounding error of r),
+// e^r = 1 + (r + (r^2 Q(r) + c)) with Q the degree-10 Chebyshev fit of
+// (e^r - 1 - r) / r^2 on that interval (relative error of the fit in e^r
+// below 4e-19), scaled by 2^k.  Within an ulp of e^x wherever the result
+// is normal (-708.3 < x < 709.7); subnormal results, overflow to inf,
+// underflow to 0, NaN as the C library's (up to an ulp of the subnormal).
+__device__ __forceinline__ double exp_core_f64(double x)
+{
+    if (__builtin_expect(!(x > -745.2 && x < 709.8), 0))
+        return x != x ? x : (x > 0.0 ? __builtin_inf() : 0.0);
+    constexpr double INV_LN2 = 0x1.71547652b82fep+0;
+    constexpr double LN2_HI = 0x1.62e42fe000000p-1;
+    constexpr double LN2_LO = 0x1.f473de6af278fp-30;
+    constexpr double E0 = 0x1.0000000000000p-1, E1 = 0x1.5555555555557p-3,
+                     E2 = 0x1.5555555555556p-5, E3 = 0x1.11111111100dfp-7,
+                     E4 = 0x1.6c16c16c162d6p-10, E5 = 0x1.a01a01abe63b3p-13,
+                     E6 = 0x1.a01a01a6d7883p-16, E7 = 0x1.71de02372bdd3p-19,
+                     E8 = 0x1.27e4db679bec4p-22, E9 = 0x1.af4ddddcf06d6p-26,
+                     E10 = 0x1.1f72fca58cc92p-29;
+    const double k = __builtin_rint(x * xc_sconst(INV_LN2));
+    const double hi = __builtin_fma(-k, xc_sconst(LN2_HI), x);    // exact
+    const double lo = k * xc_sconst(LN2_LO);
+    const double r = hi - lo;
+    const double c = (hi - r) - lo;             // r's rounding error
+    double q = __builtin_fma(r, xc_sconst(E10), xc_sconst(E9));
+    q = __builtin_fma(r, q, xc_sconst(E8));
+    q = __builtin_fma(r, q, xc_sconst(E7));
+    q = __builtin_fma(r, q, xc_sconst(E6));
+    q = __builtin_fma(r, q, xc_sconst(E5));
+    q = __builtin_fma(r, q, xc_sconst(E4));
+    q = __builtin_fma(r, q, xc_sconst(E3));
+    q = __builtin_fma(r, q, xc_sconst(E2));
+    q = __builtin_fma(r, q, xc_sconst(E1));
+    q = __builtin_fma(r, q, E0);
+    const double p = 1.0 + (r + __builtin_fma(r * r, q, c));
+    return __builtin_ldexp(p, (int)k);
+}
+
 // ---- elementary functions per precision and mode ---------------------------
 template <typename T, int MATH> struct Fn;
 

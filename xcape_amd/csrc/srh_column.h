@@ -18,9 +18,9 @@
 //
 // Literal constants of the reference's DOUBLE PRECISION routines are default
 // REAL literals, so their values are those of the nearest float (SPD below).
-// The heights' two logarithms per level are log_core_f64 (phys.h), within an
-// ulp of ln like the C library's; they were half of the chain's instructions
-// with the device library's log.
+// The heights' two logarithms and one exponential per level are log_core_f64
+// and exp_core_f64 (phys.h), within an ulp like the C library's; the logs
+// were half of the chain's instructions with the device library's log.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -47,7 +47,7 @@ __device__ __forceinline__ double std_tv(double T, double Td, double P)
     double Tin = T + XC_SPD(273.15);
     double Tdin = Td + XC_SPD(273.15);
     double E = XC_SPD(6.112) *
-               exp((XC_SPD(53.49) - (6808.0 / Tdin)) - XC_SPD(5.09) * log_core_f64(Tdin));
+               exp_core_f64((XC_SPD(53.49) - (6808.0 / Tdin)) - XC_SPD(5.09) * log_core_f64(Tdin));
     double w = eps * (E / (P - E));
     return Tin * ((w + eps) / (eps * (1.0 + w)));
 }
