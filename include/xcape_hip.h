@@ -375,9 +375,11 @@ int xcape_math_check(int fn, const float *a, const float *b, int64_t n, float *y
                      void *stream);
 
 /*
- * The helicity chain's FP64 logarithm and exponential, elementwise on device
- * arrays a (n,) -> y (n,): fn 0 the log kernel of the heights, 1 the device
- * library's log, 2 the exp kernel of the heights, 3 the device library's exp.
+ * The helicity chain's FP64 logarithm, exponential and division,
+ * elementwise on device arrays a (n,) -> y (n,): fn 0 the log kernel of the
+ * heights, 1 the device library's log, 2 the exp kernel of the heights, 3 the
+ * device library's exp; fn 4 the raw division kernel and 5 IEEE division
+ * take a (2n,) as the operand pairs (a[2i], a[2i+1]).
  * Asynchronous on `stream`.  No reference counterpart.
  */
 int xcape_math_check_f64(int fn, const double *a, int64_t n, double *y, void *stream);

@@ -21,7 +21,9 @@ correctly rounded reciprocal, and take exp and pow from short kernels
   ulp of ln x and e^x (long double on the host) over the doubles' whole
   range, the heights' arguments (level pressure ratios, dewpoints in K, the
   saturation vapour pressure's exponent) included, and give the C library's
-  special values.
+  special values; its raw FP64 division equals IEEE division bit for bit
+  for operands from 1e-150 to 1e150, and gives NaN for a zero, infinite or
+  NaN operand.
 """
 import numpy as np
 import pytest
@@ -201,4 +203,24 @@ def test_exp_kernel_f64_within_an_ulp(dev):
     got = H.math_check_f64(2, torch.from_numpy(sub).cuda()).cpu().numpy()
     ref = np.exp(sub.astype(np.longdouble))
     assert np.all(np.abs(got.astype(np.longdouble) - ref) <= np.longdouble(2.0 ** -1074))
+
+
+def test_raw_division_f64_is_ieee(dev):
+    torch, H = dev
+    rng = np.random.default_rng(31)
+    n = 1 << 21
+    # quotients within 1e-300 .. 1e300 (normal, far from overflow)
+    ab = rng.choice([-1.0, 1.0], (n, 2)) * 10.0 ** rng.uniform(-150, 150, (n, 2))
+    q = n // 2
+    ab[:q, 0] = rng.uniform(-1e5, 1e5, q)           # the heights' operands
+    ab[:q, 1] = rng.uniform(0.5, 1.2e5, q)
+    raw = H.math_check_f64(4, torch.from_numpy(ab).cuda()).cpu().numpy()
+    ieee = H.math_check_f64(5, torch.from_numpy(ab).cuda()).cpu().numpy()
+    np.testing.assert_array_equal(ieee, ab[:, 0] / ab[:, 1])
+    mism = np.count_nonzero(raw.view(np.int64) != ieee.view(np.int64))
+    print(f'raw FP64 division: {mism} of {n} differ from IEEE', flush=True)
+    assert mism == 0
+    sp = np.array([[1.0, 0.0], [0.0, 0.0], [np.inf, 2.0], [2.0, np.inf], [np.nan, 1.0],
+                   [3.0, np.nan]])
+    assert np.isnan(H.math_check_f64(4, torch.from_numpy(sp).cuda()).cpu().numpy()).all()
 

@@ -360,10 +360,16 @@ def math_check(fn, a, b=None):
 def math_check_f64(fn, a):
     """xcape_math_check_f64 on a device tensor, elementwise: the helicity
     chain's FP64 log kernel (fn 0), the device library's log (1), the FP64 exp
-    kernel (2), the device library's exp (3)."""
+    kernel (2), the device library's exp (3); the raw division kernel (4) and
+    IEEE division (5) of the pairs in the last axis of `a` (..., 2)."""
     import torch
     a = a.to(torch.float64).contiguous()
-    y = torch.empty_like(a)
+    if fn >= 4:
+        if a.shape[-1] != 2:
+            raise ValueError('division takes operand pairs: a.shape[-1] must be 2')
+        y = torch.empty(a.shape[:-1], dtype=torch.float64, device=a.device)
+    else:
+        y = torch.empty_like(a)
     stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
-    check(lib().xcape_math_check_f64(fn, _ptr(a), a.numel(), _ptr(y), stream))
+    check(lib().xcape_math_check_f64(fn, _ptr(a), y.numel(), _ptr(y), stream))
     return y

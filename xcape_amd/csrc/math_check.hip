@@ -6,7 +6,7 @@
 //     2 div_const(a, b, RN(1/b))   3 exp_core(a)   4 expf(a)
 //     5 pow_core(a, b)   6 powf(a, b)
 // xcape_math_check_f64: fn 0 log_core_f64(a)   1 log(a)   2 exp_core_f64(a)
-//     3 exp(a) (FP64)
+//     3 exp(a)   4 ddiv_raw(a0, a1)   5 a0 / a1 (FP64; a holds the pairs)
 #include <hip/hip_runtime.h>
 
 #include "../../include/xcape_hip.h"
@@ -40,6 +40,12 @@ __global__ void math_check_f64_kernel(int fn, const double *a, int64_t n, double
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (fn >= 4) {
+        // binary: the operands are the pairs (a[2i], a[2i + 1])
+        const double u = a[2 * i], v = a[2 * i + 1];
+        y[i] = fn == 4 ? ddiv_raw(u, v) : u / v;
+        return;
+    }
     const double x = a[i];
     y[i] = fn == 0 ? log_core_f64(x) : (fn == 1 ? log(x) : (fn == 2 ? exp_core_f64(x) : exp(x)));
 }
@@ -63,7 +69,7 @@ extern "C" int xcape_math_check(int fn, const float *a, const float *b, int64_t 
 
 extern "C" int xcape_math_check_f64(int fn, const double *a, int64_t n, double *y, void *stream)
 {
-    if (fn < 0 || fn > 3) return fail(XCAPE_EINVAL, "fn=%d not in 0..3", fn);
+    if (fn < 0 || fn > 5) return fail(XCAPE_EINVAL, "fn=%d not in 0..5", fn);
     if (n < 0 || n > ((int64_t)1 << 31)) return fail(XCAPE_EINVAL, "n=%lld out of range", (long long)n);
     if (!a || !y) return fail(XCAPE_EINVAL, "NULL argument");
     int rc = ensure_device();

@@ -163,6 +163,21 @@ __device__ __forceinline__ float pow_core(float x, float y)
     return __builtin_ldexpf(__builtin_amdgcn_exp2f(f), (int)n);
 }
 
+// FP64 division by the steps of the compiler's IEEE sequence (the hardware
+// reciprocal, two Newton steps, one correction by the exact residual) without
+// its operand scaling and special-value fix-up: the same quotient bit for bit
+// wherever that scaling is the identity (a, b and a/b normal, |b| far from
+// overflow); a zero, infinite or NaN operand gives NaN.  Callers re-evaluate
+// with IEEE division when their result is not finite.
+__device__ __forceinline__ double ddiv_raw(double a, double b)
+{
+    double y = __builtin_amdgcn_rcp(b);
+    y = __builtin_fma(y, __builtin_fma(-b, y, 1.0), y);
+    y = __builtin_fma(y, __builtin_fma(-b, y, 1.0), y);
+    const double q = a * y;
+    return __builtin_fma(__builtin_fma(-b, q, a), y, q);
+}
+
 // A constant the compiler materialises where it is used (in scalar
 // registers) instead of holding it across a loop.
 __device__ __forceinline__ double xc_sconst(double c)

@@ -20,7 +20,9 @@
 // REAL literals, so their values are those of the nearest float (SPD below).
 // The heights' two logarithms and one exponential per level are log_core_f64
 // and exp_core_f64 (phys.h), within an ulp like the C library's; the logs
-// were half of the chain's instructions with the device library's log.
+// were half of the chain's instructions with the device library's log.  Their
+// divisions are the raw FP64 kernel (IEEE quotients, re-evaluated with IEEE
+// division where a level's result is not finite).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -39,17 +41,33 @@ template <typename TB> struct SrhOutT {
 };
 using SrhOut = SrhOutT<float>;
 
+// FP64 quotient: the raw division kernel (RAW) or IEEE division
+template <bool RAW> __device__ __forceinline__ double sdiv(double a, double b)
+{
+    if constexpr (RAW) return ddiv_raw(a, b);
+    else return a / b;
+}
+
 // virtual temperature of one level (stdheight_2D_model_lev.f90:102-127), FP64
-__device__ __forceinline__ double std_tv(double T, double Td, double P)
+template <bool RAW> __device__ __forceinline__ double std_tv_ops(double T, double Td, double P)
 {
 #pragma clang fp contract(off)
     const double eps = XC_SPD(0.6219800858985514);
     double Tin = T + XC_SPD(273.15);
     double Tdin = Td + XC_SPD(273.15);
-    double E = XC_SPD(6.112) *
-               exp_core_f64((XC_SPD(53.49) - (6808.0 / Tdin)) - XC_SPD(5.09) * log_core_f64(Tdin));
-    double w = eps * (E / (P - E));
-    return Tin * ((w + eps) / (eps * (1.0 + w)));
+    double E = XC_SPD(6.112) * exp_core_f64((XC_SPD(53.49) - sdiv<RAW>(6808.0, Tdin)) -
+                                            XC_SPD(5.09) * log_core_f64(Tdin));
+    double w = eps * sdiv<RAW>(E, P - E);
+    return Tin * sdiv<RAW>(w + eps, eps * (1.0 + w));
+}
+
+// with the raw divisions, the IEEE ones where the result is not finite (the
+// raw kernel's NaN for a zero or infinite divisor): IEEE results throughout
+__device__ __forceinline__ double std_tv(double T, double Td, double P)
+{
+    const double tv = std_tv_ops<true>(T, Td, P);
+    if (__builtin_expect(__builtin_isfinite(tv), 1)) return tv;
+    return std_tv_ops<false>(T, Td, P);
 }
 
 // std_tv's value to within about 1e-3 relative, in FP32 with the hardware
@@ -69,12 +87,20 @@ __device__ __forceinline__ float std_tv_sign(float T, float Td, float P)
 }
 
 // H(k) = H(k-1) + (R ((Tv(k) + Tv(k-1))/2) / g) ln(P(k)/P(k-1)) (:147, :155)
-__device__ __forceinline__ double std_dh(double tv, double tv_prev, double P, double P_prev)
+template <bool RAW>
+__device__ __forceinline__ double std_dh_ops(double tv, double tv_prev, double P, double P_prev)
 {
 #pragma clang fp contract(off)
     const double R = XC_SPD(287.04);
     const double g = XC_SPD(-9.80665);
-    return ((R * ((tv + tv_prev) / 2.0)) / g) * log_core_f64(P / P_prev);
+    return sdiv<RAW>(R * ((tv + tv_prev) / 2.0), g) * log_core_f64(sdiv<RAW>(P, P_prev));
+}
+
+__device__ __forceinline__ double std_dh(double tv, double tv_prev, double P, double P_prev)
+{
+    const double dh = std_dh_ops<true>(tv, tv_prev, P, P_prev);
+    if (__builtin_expect(__builtin_isfinite(dh), 1)) return dh;
+    return std_dh_ops<false>(tv, tv_prev, P, P_prev);
 }
 
 // Column accessor C: P(k), Tc(k), Tdc(k), U(k), V(k) for k = 0..nk-1 with
