@@ -83,13 +83,19 @@ def test_levels_above_the_outputs(hip, oracle_lib):
     drop below 6 km and climb through the Bunkers levels again, whose last
     brackets then win), a NaN temperature at the top, and the upper levels'
     pressures above the surface's (the last level below the surface: the
-    reference's decreasing orientation).  The untouched columns too."""
+    reference's decreasing orientation).  Below 6 km, where the walk keeps
+    the Bunkers sums as the heights rise: a level's pressure above the one
+    before it (the heights fall there and cross some Bunkers levels again),
+    and a NaN dewpoint (NaN heights from there up).  The untouched columns
+    too."""
     c = synthetic.sigma_columns(1024, 137, 1005, winds=True)
-    p, t = c['p'].copy(), c['t'].copy()
+    p, t, td = c['p'].copy(), c['t'].copy(), c['td'].copy()
     p[0:100, 120] = c['ps'][0:100] * 0.9
     t[100:200, 136] = np.nan
     p[200:300, 100:] = c['ps'][200:300, None] * 20.0
-    args = [p, t] + [c[k] for k in ('td', 'u', 'v', 'ps', 'ts', 'tds', 'us', 'vs')]
+    p[300:400, 30] = c['p'][300:400, 24]
+    td[400:500, 40] = np.nan
+    args = [p, t, td] + [c[k] for k in ('u', 'v', 'ps', 'ts', 'tds', 'us', 'vs')]
     g = hip.srh(*args, depth=3000.)
     o = oracle_lib.srh(*args, 3000.)
     for x, y in zip(g, o):
@@ -101,9 +107,10 @@ def test_levels_above_the_outputs(hip, oracle_lib):
         np.testing.assert_array_equal(x[fin], y[fin])
     # the first group's Bunkers levels were crossed again above the dropped
     # level, and the third group's column turned over: both change the motion
-    ref = oracle_lib.srh(c['p'], c['t'], *args[2:], 3000.)
+    ref = oracle_lib.srh(c['p'], c['t'], c['td'], *args[3:], 3000.)
     assert np.any(o[2][0:100] != ref[2][0:100])
     assert np.any(o[2][200:300] != ref[2][200:300])
+    assert np.any(o[2][300:400] != ref[2][300:400])
     _cmp_srh(g, o, 'levels above the outputs')
 
 

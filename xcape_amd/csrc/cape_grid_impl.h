@@ -522,11 +522,13 @@ __global__ void warm_reset_kernel(unsigned int *work, uint32_t ncol, uint32_t gr
 // input element type (double: float64 profiles, which feed the FP64 heights
 // and helicity unrounded as the reference's f2py wrappers pass them).
 //
-// Held to 5 waves per SIMD (96 VGPRs; it needs 99 unconstrained, 4 waves): in
-// the combined call it shares the CUs with the ascent, and the step ran 1.4 %
-// faster so (the helicity-only call 4 % faster).
+// The float chain is held to 6 waves per SIMD (80 VGPRs, 86 unconstrained):
+// in the combined call it shares the CUs with the ascent, and each wave per
+// SIMD it gained took the step about 1 % faster (the helicity-only call 3 %).
+// The FP64 re-run (TB = double) is left unconstrained.
 template <typename TB, typename TI>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 5)))
+__global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(sizeof(TB) == 4 ? 6 : 1, sizeof(TB) == 4 ? 6 : 8)))
 srh_grid_kernel(CapeGridArgs a)
 {
     const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

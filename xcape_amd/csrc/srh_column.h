@@ -103,6 +103,66 @@ __device__ __forceinline__ double std_dh(double tv, double tv_prev, double P, do
     return std_dh_ops<false>(tv, tv_prev, P, P_prev);
 }
 
+// The Bunkers inputs (Bunkers_model_lev.f90:123-177): the winds at the 12
+// target heights h_i = 500 (i + 1) m (-999999 where no layer brackets h_i),
+// as the sums su = ((0 + u0) + u_0) + ... + u_11 in index order (the
+// reference's accumulation) and the three the shear uses.
+template <typename TB> struct BunkersIn {
+    TB su, sv, u0, v0, u10, v10, u11, v11;
+};
+
+template <typename TB> __device__ __forceinline__ TB bunkers_h(int i)
+{
+    return TB(500 * (i + 1));
+}
+
+// The Bunkers inputs by the reference's own method: the winds interpolated
+// in every bracketing layer of a walk up the column, the last one bottom-up
+// kept (its top-down search stops at the first, :218-231); `decreasing`:
+// heights falling with k (IP = 1), the bracket z(k) <= h < z(k-1).  Walks
+// the heights again; for the columns whose heights do not rise steadily.
+template <typename TB, class C>
+__device__ __noinline__ void bunkers_full(const C &col, int nk, double hs, double tv0, double Ps,
+                                          bool decreasing, BunkersIn<TB> &b)
+{
+#pragma clang fp contract(off)
+    TB iu[12], iv[12];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) { iu[i] = TB(-999999.0); iv[i] = TB(-999999.0); }
+    double tv_prev = tv0, P_prev = Ps, H = hs;
+    TB z_prev = (TB)hs, u_prev = (TB)col.U(0), v_prev = (TB)col.V(0);
+    for (int k = 1; k < nk; ++k) {
+        const double Pk = (double)col.P(k);
+        const double tv = std_tv((double)col.Tc(k), (double)col.Tdc(k), Pk);
+        H = H + std_dh(tv, tv_prev, Pk, P_prev);
+        const TB z = (TB)H;
+        const TB uk = (TB)col.U(k), vk = (TB)col.V(k);
+        // the layer's lower and upper ends in the orientation's order
+        const TB zl = decreasing ? z : z_prev, zh = decreasing ? z_prev : z;
+        const TB ul = decreasing ? uk : u_prev, uh = decreasing ? u_prev : uk;
+        const TB vl = decreasing ? vk : v_prev, vh = decreasing ? v_prev : vk;
+#pragma unroll
+        for (int i = 0; i < 12; ++i) {
+            const TB h = bunkers_h<TB>(i);
+            if (zl <= h && zh > h) {
+                const TB w2 = (h - zl) / (zh - zl);
+                const TB w1 = (TB)(1.0 - (double)w2);
+                iu[i] = w1 * ul + w2 * uh;
+                iv[i] = w1 * vl + w2 * vh;
+            }
+        }
+        z_prev = z; u_prev = uk; v_prev = vk;
+        tv_prev = tv; P_prev = Pk;
+    }
+    b.su = TB(0) + (TB)col.U(0);
+    b.sv = TB(0) + (TB)col.V(0);
+#pragma unroll
+    for (int i = 0; i < 12; ++i) { b.su = b.su + iu[i]; b.sv = b.sv + iv[i]; }
+    b.u0 = iu[0]; b.v0 = iv[0];
+    b.u10 = iu[10]; b.v10 = iv[10];
+    b.u11 = iu[11]; b.v11 = iv[11];
+}
+
 // Column accessor C: P(k), Tc(k), Tdc(k), U(k), V(k) for k = 0..nk-1 with
 // k = 0 the surface (hPa, degC, m/s).  hs: surface height (2 m in calc_srh).
 // TB = float is the reference's arithmetic; TB = double evaluates the storm
@@ -111,17 +171,12 @@ template <typename TB, class C>
 __device__ void srh_column(const C &col, int nk, double depth, double hs, SrhOutT<TB> &out)
 {
 #pragma clang fp contract(off)
-    const TB lv[12] = {TB(500), TB(1000), TB(1500), TB(2000), TB(2500), TB(3000),
-                       TB(3500), TB(4000), TB(4500), TB(5000), TB(5500), TB(6000)};
-    TB iu[12], iv[12];
-#pragma unroll
-    for (int i = 0; i < 12; ++i) { iu[i] = TB(-999999.0); iv[i] = TB(-999999.0); }
-
     // ---- pass 1: heights and Bunkers brackets (increasing orientation),
     // and the helicity sum's storm-motion-free part ----
     const double Ps = (double)col.P(0);
     const double tv0 = std_tv((double)col.Tc(0), (double)col.Tdc(0), Ps);
     const TB zsurf = (TB)hs;
+    const TB u0 = (TB)col.U(0), v0 = (TB)col.V(0);
     // SREH 0..depth (SREH_model_lev.f90:82-114) sums, over the levels up to
     // the first one above `depth` (its wind interpolated to `depth`),
     // (u - cu)(v - v') - (v - cv)(u - u') with c the storm motion, known only
@@ -129,6 +184,12 @@ __device__ void srh_column(const C &col, int nk, double depth, double hs, SrhOut
     // and the telescoping sums of (v - v') and (u - u'), v_top - v_0 and
     // u_top - u_0, so c enters at the end and the heights are computed once.
     // FP64; the split moves the sum by rounding only.
+    //
+    // Bunkers brackets: while the heights do not fall (`mono`), each target
+    // height h_i is bracketed once, in the order of i, so the walk keeps the
+    // index nb of the next one and the reference's in-order sums as it goes;
+    // a column whose heights fall anywhere (or are NaN) gets them from
+    // bunkers_full, as does one whose heights decrease with k.
     //
     // No output depends on a level above both the helicity's top and the
     // last Bunkers level (6 km) but through the orientation test after the
@@ -140,15 +201,25 @@ __device__ void srh_column(const C &col, int nk, double depth, double hs, SrhOut
     // in full, so the outputs are those of the full walk in every case.
     double tv_prev, P_prev, H, s0, z1prev, ut_prev, vt_prev;
     TB z_prev, u_prev, v_prev;
-    bool reached, rising = false;
+    bool reached, rising = false, mono;
+    int nb;
+    BunkersIn<TB> b;
     for (int walk = 0; walk < 2; ++walk) {
-#pragma unroll
-        for (int i = 0; i < 12; ++i) { iu[i] = TB(-999999.0); iv[i] = TB(-999999.0); }
         tv_prev = tv0; P_prev = Ps; H = hs;
-        z_prev = zsurf; u_prev = (TB)col.U(0); v_prev = (TB)col.V(0);
+        z_prev = zsurf; u_prev = u0; v_prev = v0;
         s0 = 0.0; z1prev = hs; ut_prev = (double)col.U(0); vt_prev = (double)col.V(0);
         reached = false;
         rising = false;
+        mono = true;
+        b.su = TB(0) + u0; b.sv = TB(0) + v0;
+        b.u0 = b.v0 = b.u10 = b.v10 = b.u11 = b.v11 = TB(-999999.0);
+        // target heights below the surface are never bracketed by rising heights
+        nb = 0;
+        while (nb < 12 && bunkers_h<TB>(nb) < zsurf) {
+            b.su = b.su + TB(-999999.0);
+            b.sv = b.sv + TB(-999999.0);
+            ++nb;
+        }
         bool failed = false;
         float p_last = 0.0f;
         for (int k = 1; k < nk; ++k) {
@@ -169,13 +240,24 @@ __device__ void srh_column(const C &col, int nk, double depth, double hs, SrhOut
             // FP64 from the inputs as given (both as the reference's wrappers
             // pass them)
             const TB uk = (TB)col.U(k), vk = (TB)col.V(k);
-#pragma unroll
-            for (int i = 0; i < 12; ++i) {
-                if (z_prev <= lv[i] && z > lv[i]) {
-                    const TB w2 = (lv[i] - z_prev) / (z - z_prev);
-                    const TB w1 = (TB)(1.0 - (double)w2);
-                    iu[i] = w1 * u_prev + w2 * uk;
-                    iv[i] = w1 * v_prev + w2 * vk;
+            if (mono) {
+                if (!(z >= z_prev)) {
+                    mono = false;
+                } else {
+                    // the targets h_nb, h_nb+1, ... below z: z_prev <= h_nb holds
+                    while (nb < 12) {
+                        const TB h = bunkers_h<TB>(nb);
+                        if (!(h < z)) break;
+                        const TB w2 = (h - z_prev) / (z - z_prev);
+                        const TB w1 = (TB)(1.0 - (double)w2);
+                        const TB bu = w1 * u_prev + w2 * uk, bv = w1 * v_prev + w2 * vk;
+                        b.su = b.su + bu;
+                        b.sv = b.sv + bv;
+                        if (nb == 0) { b.u0 = bu; b.v0 = bv; }
+                        if (nb == 10) { b.u10 = bu; b.v10 = bv; }
+                        if (nb == 11) { b.u11 = bu; b.v11 = bv; }
+                        ++nb;
+                    }
                 }
             }
             if (!reached) {
@@ -198,56 +280,33 @@ __device__ void srh_column(const C &col, int nk, double depth, double hs, SrhOut
             tv_prev = tv; P_prev = Pk;
             // every output's levels behind and this level above the surface:
             // from here on the first walk checks the rest only
-            if (walk == 0 && reached && z > lv[11] && z > zsurf && tv > 1.0) {
+            if (walk == 0 && reached && z > bunkers_h<TB>(11) && z > zsurf && tv > 1.0) {
                 rising = true;
                 p_last = (float)Pk;
             }
         }
         if (!failed) break;
     }
-    // (a walk that went on rising ends above the surface: the test is false)
-    if (!rising && zsurf > z_prev) {
-        // heights decrease with k (IP = 1, IM = 0 in the reference): the
-        // bracket is z(k) <= h < z(k-1); the top-down search keeps the
-        // highest k, i.e. the last bracket of this bottom-up walk
-#pragma unroll
-        for (int i = 0; i < 12; ++i) { iu[i] = TB(-999999.0); iv[i] = TB(-999999.0); }
-        tv_prev = tv0;
-        P_prev = Ps;
-        H = hs;
-        z_prev = (TB)hs;
-        u_prev = (TB)col.U(0); v_prev = (TB)col.V(0);
-        for (int k = 1; k < nk; ++k) {
-            const double Pk = (double)col.P(k);
-            const double tv = std_tv((double)col.Tc(k), (double)col.Tdc(k), Pk);
-            H = H + std_dh(tv, tv_prev, Pk, P_prev);
-            const TB z = (TB)H;
-            const TB uk = (TB)col.U(k), vk = (TB)col.V(k);
-#pragma unroll
-            for (int i = 0; i < 12; ++i) {
-                if (z <= lv[i] && z_prev > lv[i]) {
-                    const TB w2 = (lv[i] - z) / (z_prev - z);
-                    const TB w1 = (TB)(1.0 - (double)w2);
-                    iu[i] = w1 * uk + w2 * u_prev;
-                    iv[i] = w1 * vk + w2 * v_prev;
-                }
-            }
-            z_prev = z; u_prev = uk; v_prev = vk;
-            tv_prev = tv; P_prev = Pk;
+    // heights decreasing with k (IP = 1, IM = 0 in the reference); a walk that
+    // went on rising ends above the surface, so the test is false for it
+    const bool decreasing = !rising && zsurf > z_prev;
+    if (decreasing || !mono) {
+        bunkers_full<TB>(col, nk, hs, tv0, Ps, decreasing, b);
+    } else {
+        // the targets no layer reached, in index order
+        for (int i = nb; i < 12; ++i) {
+            b.su = b.su + TB(-999999.0);
+            b.sv = b.sv + TB(-999999.0);
         }
     }
 
     // ---- Bunkers (Bunkers_model_lev.f90:123-177), accumulators from zero ----
-    const TB u0 = (TB)col.U(0), v0 = (TB)col.V(0);
-    TB mu = TB(0) + u0, mv = TB(0) + v0;
-#pragma unroll
-    for (int i = 0; i < 12; ++i) { mu = mu + iu[i]; mv = mv + iv[i]; }
-    mu = mu / TB(13);
-    mv = mv / TB(13);
-    const TB ushr_u = ((TB(0) + iu[10]) + iu[11]) / TB(2);
-    const TB vshr_u = ((TB(0) + iv[10]) + iv[11]) / TB(2);
-    const TB ushr_d = ((TB(0) + u0) + iu[0]) / TB(2);
-    const TB vshr_d = ((TB(0) + v0) + iv[0]) / TB(2);
+    const TB mu = b.su / TB(13);
+    const TB mv = b.sv / TB(13);
+    const TB ushr_u = ((TB(0) + b.u10) + b.u11) / TB(2);
+    const TB vshr_u = ((TB(0) + b.v10) + b.v11) / TB(2);
+    const TB ushr_d = ((TB(0) + u0) + b.u0) / TB(2);
+    const TB vshr_d = ((TB(0) + v0) + b.v0) / TB(2);
     const TB ushr = ushr_u - ushr_d;
     const TB vshr = vshr_u - vshr_d;
     TB mag;
