@@ -486,8 +486,22 @@ XC_HD __forceinline__ void layer_update(T th2, T qv2, T ql2, T qi2, T e_thv, T e
 {
     using F = Fn<T, MATH>;
     using K = Phys<T>;
-    const T thv2 = F::div(th2 * (T(1.0) + K::reps * qv2), T(1.0) + qv2 + ql2 + qi2);
-    b2 = F::div(K::g * (thv2 - e_thv), e_thv);
+    T thv2 = T(0.0);
+    bool raw = false;
+#if defined(__HIP_DEVICE_COMPILE__)
+    // the accurate mode's raw division kernel (phys.h): thv2 and b2 have
+    // divisors of at least 1 and the environment's thv, so finite results are
+    // the IEEE quotients; otherwise both again with IEEE division
+    if constexpr (sizeof(T) == 4 && MATH == MATH_ACCURATE) {
+        thv2 = div_raw(th2 * (1.0f + K::reps * qv2), 1.0f + qv2 + ql2 + qi2);
+        b2 = div_raw(K::g * (thv2 - e_thv), e_thv);
+        raw = __builtin_isfinite(thv2) && __builtin_isfinite(b2);
+    }
+#endif
+    if (!raw) {
+        thv2 = F::div(th2 * (T(1.0) + K::reps * qv2), T(1.0) + qv2 + ql2 + qi2);
+        b2 = F::div(K::g * (thv2 - e_thv), e_thv);
+    }
     dz = layer_dz(e_thv, thv_prev, e_pi, pi_prev);
     // the reference's four cases (:509-545) as selects over one division:
     // each case's expressions are evaluated as written there, and the wave
@@ -495,7 +509,16 @@ XC_HD __forceinline__ void layer_update(T th2, T qv2, T ql2, T qi2, T e_thv, T e
     const bool enter = b2 >= T(0.0) && b1 < T(0.0);   // entering positive area
     const bool leave = b2 < T(0.0) && b1 > T(0.0);    // leaving it
     const bool neg = b2 < T(0.0) && !leave;
-    const T frac = F::div(enter ? b2 : b1, enter ? b2 - b1 : b1 - b2);
+    const T fnum = enter ? b2 : b1, fden = enter ? b2 - b1 : b1 - b2;
+    // (used only when entering or leaving, where fden = |b1| + |b2| > 0 and
+    // the buoyancies are finite: the raw quotient is the IEEE one)
+    T frac;
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (sizeof(T) == 4 && MATH == MATH_ACCURATE)
+        frac = raw ? div_raw(fnum, fden) : F::div(fnum, fden);
+    else
+#endif
+        frac = F::div(fnum, fden);
     const T half_dz_sum = T(0.5) * dz * (b1 + b2);
     T parea;
     if (enter) {
