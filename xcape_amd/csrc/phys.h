@@ -134,6 +134,21 @@ __device__ __forceinline__ float exp_core(float x)
     return __builtin_ldexpf(__builtin_amdgcn_exp2f(f), (int)n);
 }
 
+// ln x: the device library's steps -- the hardware log2, scaled by ln 2 as
+// the double-float (0x3f317217, 0x3377d1cf) -- without its subnormal
+// scaling and infinity select: the library's value for normal positive
+// finite x.
+__device__ __forceinline__ float log_core(float x)
+{
+    constexpr float LN2_HI = 0x1.62e42ep-1f;
+    constexpr float LN2_LO = 0x1.efa39ep-25f;
+    const float r = __builtin_amdgcn_logf(x);
+    const float h = r * LN2_HI;
+    float l = __builtin_fmaf(r, LN2_HI, -h);
+    l = __builtin_fmaf(r, LN2_LO, l);
+    return h + l;
+}
+
 // x^y for 0 < x < 2^126, |y| <= 1 (the Exner function and theta-e's power):
 // x = m 2^k with m in [sqrt(1/2), sqrt(2)); log2(m) from the hardware log2
 // (|log2 m| <= 1/2); y (k + log2 m) as a double-float (exact products by
@@ -428,7 +443,8 @@ XC_HD __forceinline__ T thetae_ops(T p, T t, T td, T q)
 // and value are where those give the IEEE quotients and expf's values, and
 // the caller then takes the library's operations.  The test: t and td within
 // (1, 1e4) K and p within (1e-30, 1e30) Pa (so that 1/(td - 56), t/td and
-// 1e5/p are normal quotients of normal operands or NaN), the two divisors
+// 1e5/p are normal quotients of normal operands or NaN, and log_core's
+// argument t/td normal and positive), the two divisors
 // formed from them of magnitude above 1e-30, the exp's argument within +-60
 // and a finite result.
 __device__ __forceinline__ bool thetae_raw(float p, float t, float td, float q, float &r)
@@ -439,7 +455,7 @@ __device__ __forceinline__ bool thetae_raw(float p, float t, float td, float q, 
         tlcl = t;
     } else {
         const float a = div_raw(1.0f, td - 56.0f);
-        const float s = a + 0.00125f * logf(div_raw(t, td));
+        const float s = a + 0.00125f * log_core(div_raw(t, td));
         tlcl = 56.0f + div_raw(1.0f, s);
         ok = ok && __builtin_fabsf(s) > 1e-30f;
     }
