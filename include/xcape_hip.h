@@ -367,9 +367,8 @@ int xcape_surrogate_log_f32(const float *p, const float *t, const float *td,
  * (n,) -> y (n,), for checking its rounding on the device at hand: fn 0 the
  * raw IEEE division kernel, 1 the compiler's IEEE division, 2 division by a
  * per-call constant (its reciprocal rounded once), 3 the exp kernel, 4 the
- * device library's expf, 5 the pow kernel, 6 the device library's powf, 7 the
- * log kernel, 8 the device library's logf.  b is unused (may be NULL) for
- * fn 3, 4, 7 and 8.  Asynchronous on `stream`.
+ * device library's expf, 5 the pow kernel, 6 the device library's powf.
+ * b is unused (may be NULL) for fn 3 and 4.  Asynchronous on `stream`.
  * No reference counterpart.
  */
 int xcape_math_check(int fn, const float *a, const float *b, int64_t n, float *y,

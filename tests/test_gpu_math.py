@@ -13,9 +13,7 @@ correctly rounded reciprocal, and take exp and pow from short kernels
 * division by a constant equals IEEE division for the constants the kernels
   use (the ice ramp's 40 K, pressure increments) -- the CPU check of the
   same sequence over every float is in the notes of phys.h;
-* the exp kernel equals the device library's expf bit for bit on (-80, 80),
-  and the log kernel its logf on every normal positive float tried (1e-37 to
-  1e38, and the sub-steps' pressure ratios);
+* the exp kernel equals the device library's expf bit for bit on (-80, 80);
 * pow stays within the device library's powf accuracy: its error against the
   exactly rounded x^y (from float64 on the host) is at most 1 ulp over the
   Exner function's and theta-e's ranges, and no worse than powf's;
@@ -112,20 +110,6 @@ def test_exp_kernel_equals_library_expf(dev):
     assert mism == 0
     nan = H.math_check(3, torch.tensor([float('nan')], device='cuda')).cpu().numpy()
     assert np.isnan(nan).all()
-
-
-def test_log_kernel_equals_library_logf(dev):
-    torch, H = dev
-    rng = np.random.default_rng(7)
-    x = (10.0 ** rng.uniform(-37, 38, 1 << 22)).astype(np.float32)
-    x[:1 << 20] = rng.uniform(0.85, 1.0, 1 << 20).astype(np.float32)   # p2 / p1
-    x[1 << 20:1 << 21] = rng.uniform(0.5, 2.0, 1 << 20).astype(np.float32)   # t / td
-    tx = torch.from_numpy(x).cuda()
-    lib = H.math_check(8, tx).cpu().numpy()
-    mine = H.math_check(7, tx).cpu().numpy()
-    mism = np.count_nonzero(_bits(mine) != _bits(lib))
-    print(f'log kernel: {mism} of {x.size} differ from logf', flush=True)
-    assert mism == 0
 
 
 @pytest.mark.parametrize('y,lo,hi', [(287.04 / 1005.7, 0.005, 1.2),      # Exner, p/p00

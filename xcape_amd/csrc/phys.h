@@ -134,21 +134,6 @@ __device__ __forceinline__ float exp_core(float x)
     return __builtin_ldexpf(__builtin_amdgcn_exp2f(f), (int)n);
 }
 
-// ln x: the device library's steps -- the hardware log2, scaled by ln 2 as
-// the double-float (0x3f317217, 0x3377d1cf) -- without its subnormal
-// scaling and infinity select: the library's value for normal positive
-// finite x.
-__device__ __forceinline__ float log_core(float x)
-{
-    constexpr float LN2_HI = 0x1.62e42ep-1f;
-    constexpr float LN2_LO = 0x1.efa39ep-25f;
-    const float r = __builtin_amdgcn_logf(x);
-    const float h = r * LN2_HI;
-    float l = __builtin_fmaf(r, LN2_HI, -h);
-    l = __builtin_fmaf(r, LN2_LO, l);
-    return h + l;
-}
-
 // x^y for 0 < x < 2^126, |y| <= 1 (the Exner function and theta-e's power):
 // x = m 2^k with m in [sqrt(1/2), sqrt(2)); log2(m) from the hardware log2
 // (|log2 m| <= 1/2); y (k + log2 m) as a double-float (exact products by
@@ -423,7 +408,7 @@ XC_HD __forceinline__ T qvi(T p, T t)
 
 // getthe: Bolton (1980) theta-e with the LCL temperature (:604-620)
 template <typename T, int MATH>
-XC_HD __forceinline__ T thetae_ops(T p, T t, T td, T q)
+XC_HD __forceinline__ T thetae(T p, T t, T td, T q)
 {
     using F = Fn<T, MATH>;
     T tlcl;
@@ -435,47 +420,6 @@ XC_HD __forceinline__ T thetae_ops(T p, T t, T td, T q)
     }
     return t * F::pow(F::div(T(100000.0), p), T(0.2854) * (T(1.0) - T(0.28) * q)) *
            F::exp((F::div(T(3376.0), tlcl) - T(2.54)) * q * (T(1.0) + T(0.81) * q));
-}
-
-#if defined(__HIP_DEVICE_COMPILE__)
-// thetae_ops<float, MATH_ACCURATE>'s operations in the same order with the
-// raw division kernel and exp_core; false unless every quotient's operands
-// and value are where those give the IEEE quotients and expf's values, and
-// the caller then takes the library's operations.  The test: t and td within
-// (1, 1e4) K and p within (1e-30, 1e30) Pa (so that 1/(td - 56), t/td and
-// 1e5/p are normal quotients of normal operands or NaN, and log_core's
-// argument t/td normal and positive), the two divisors
-// formed from them of magnitude above 1e-30, the exp's argument within +-60
-// and a finite result.
-__device__ __forceinline__ bool thetae_raw(float p, float t, float td, float q, float &r)
-{
-    float tlcl;
-    bool ok = t > 1.0f && t < 1e4f && td > 1.0f && td < 1e4f && p > 1e-30f && p < 1e30f;
-    if ((td - t) >= -0.1f) {
-        tlcl = t;
-    } else {
-        const float a = div_raw(1.0f, td - 56.0f);
-        const float s = a + 0.00125f * log_core(div_raw(t, td));
-        tlcl = 56.0f + div_raw(1.0f, s);
-        ok = ok && __builtin_fabsf(s) > 1e-30f;
-    }
-    const float x = (div_raw(3376.0f, tlcl) - 2.54f) * q * (1.0f + 0.81f * q);
-    r = t * pow_core(div_raw(100000.0f, p), 0.2854f * (1.0f - 0.28f * q)) * exp_core(x);
-    return ok && __builtin_fabsf(tlcl) > 1e-30f && x > -60.0f && x < 60.0f &&
-           __builtin_isfinite(r);
-}
-#endif
-
-template <typename T, int MATH>
-XC_HD __forceinline__ T thetae(T p, T t, T td, T q)
-{
-#if defined(__HIP_DEVICE_COMPILE__)
-    if constexpr (sizeof(T) == 4 && MATH == MATH_ACCURATE) {
-        float r;
-        if (__builtin_expect(thetae_raw(p, t, td, q, r), 1)) return r;
-    }
-#endif
-    return thetae_ops<T, MATH>(p, t, td, q);
 }
 
 }  // namespace xcape
