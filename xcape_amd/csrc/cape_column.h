@@ -88,9 +88,9 @@ XC_HD __forceinline__ Raw load_raw(const C &col, int k)
     return Raw{col.P(k), col.Tc(k), col.Tdc(k)};
 }
 
-// :232-240 for one level, with the operations of math mode MATH
+// :232-240 for one level
 template <typename T, int MATH>
-XC_HD __forceinline__ Env<T> env_from_ops(const Raw &r)
+XC_HD __forceinline__ Env<T> env_from(const Raw &r)
 {
     using F = Fn<T, MATH>;
     using K = Phys<T>;
@@ -103,44 +103,6 @@ XC_HD __forceinline__ Env<T> env_from_ops(const Raw &r)
     e.th = F::div(e.t, e.pi);
     e.thv = F::div(e.th * (T(1.0) + K::reps * e.q), T(1.0) + e.q);
     return e;
-}
-
-#if defined(__HIP_DEVICE_COMPILE__)
-// The accurate mode's environment on the device: env_from_ops<float,
-// MATH_ACCURATE>'s operations in the same order, with the raw IEEE division
-// kernel and exp_core (phys.h).  Returns false unless every quotient's
-// operands and value are in the range where those give the IEEE quotients
-// and expf's values bit for bit (the vapour pressure's exponent above -60 and
-// below 60, p - es of normal magnitude, every result finite); the caller then
-// evaluates the level with the library's operations.
-__device__ __forceinline__ bool env_from_raw(const Raw &r, Env<float> &e)
-{
-    using K = Phys<float>;
-    e.p = 100.0f * r.p;
-    e.t = 273.15f + r.t;
-    e.td = 273.15f + r.td;
-    e.pi = pow_core(e.p * K::rp00, K::rddcp);
-    const float x = div_raw(17.67f * (e.td - 273.15f), e.td - 29.65f);
-    const float es = 611.2f * exp_core(x);
-    const float pe = e.p - es;
-    e.q = div_raw(K::eps * es, pe);
-    e.th = div_raw(e.t, e.pi);
-    e.thv = div_raw(e.th * (1.0f + K::reps * e.q), 1.0f + e.q);
-    return x > -60.0f && x < 60.0f && !(__builtin_fabsf(pe) < 0x1p-126f) &&
-           __builtin_isfinite(e.q) && __builtin_isfinite(e.thv);
-}
-#endif
-
-template <typename T, int MATH>
-XC_HD __forceinline__ Env<T> env_from(const Raw &r)
-{
-#if defined(__HIP_DEVICE_COMPILE__)
-    if constexpr (sizeof(T) == 4 && MATH == MATH_ACCURATE) {
-        Env<float> e;
-        if (__builtin_expect(env_from_raw(r, e), 1)) return e;
-    }
-#endif
-    return env_from_ops<T, MATH>(r);
 }
 
 template <typename T, int MATH, class C>
@@ -544,17 +506,6 @@ template <typename T, int MATH>
 XC_HD __forceinline__ int substep_count(T &dp, T pinc)
 {
     if (dp < pinc) return 1;
-#if defined(__HIP_DEVICE_COMPILE__)
-    // the raw IEEE division kernel for finite dp >= pinc > 0 (normal operands,
-    // normal quotients: the IEEE quotients)
-    if constexpr (sizeof(T) == 4 && MATH == MATH_ACCURATE) {
-        if (__builtin_expect(dp < 1e30f && pinc > 1e-30f, 1)) {
-            const int n = 1 + ftrunc(div_raw(dp, pinc));
-            dp = div_raw(dp, (float)n);
-            return n;
-        }
-    }
-#endif
     const int nloop = 1 + ftrunc(Fn<T, MATH>::div(dp, pinc));
     dp = Fn<T, MATH>::div(dp, (T)nloop);
     return nloop;
