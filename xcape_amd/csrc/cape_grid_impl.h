@@ -214,9 +214,6 @@ __global__ void __launch_bounds__(256) cape_grid_kernel(CapeGridArgs a)
 // the mask of the few lanes that reach a new level in a given trip.
 // ---------------------------------------------------------------------------
 
-typedef float xc_v4f __attribute__((ext_vector_type(4)));
-__device__ __forceinline__ xc_v4f xc_v4(const float4 r) { return xc_v4f{r.x, r.y, r.z, r.w}; }
-
 // Level records as float4 loads and stores (the layout of LevelRec).
 __device__ __forceinline__ float4 level_rec(const Level<float> &lv)
 {
@@ -316,11 +313,7 @@ warm_setup_kernel(CapeGridArgs a)
             if (k < k0) continue;
             const Env<T> e = env_from<T, XC_MATH>(load_raw(c, k));
             const float4 r = level_rec(level_from<T, XC_MATH>(e, p_below, pinc));
-            // non-temporal: the ascent reads the records back from HBM (1 % off
-            // the call); the source scan's stores above stay ordinary, as the
-            // record of kmax + 1 may be read back below
-            __builtin_nontemporal_store(xc_v4(r),
-                                        reinterpret_cast<xc_v4f *>(env) + ((uint32_t)k * ls + j));
+            env[(uint32_t)k * ls + j] = r;
             if (envp) envp[(uint32_t)k * ls + j] = e.p;
             if (k == pc.kmax + 1) r1 = r;
             p_below = e.p;
