@@ -9,9 +9,12 @@
 //  * pass 1 walks the levels bottom-up, building H(k) and, for each of the 12
 //    Bunkers target heights, the interpolant of the *last* bracketing layer
 //    seen.  The reference searches top-down and stops at the first bracket
-//    (Bunkers_model_lev.f90:218-231), which is the last one bottom-up.
-//    Heights fall with k only for a column whose top is below its surface;
-//    that orientation (IP = 1 in the reference) is handled by a second walk.
+//    (Bunkers_model_lev.f90:218-231), which is the last one bottom-up.  While
+//    the heights rise each target is bracketed once, in order, so the walk
+//    keeps only the next target and the reference's in-order sums; a column
+//    whose heights fall somewhere, or whose heights fall with k throughout
+//    (its top below its surface: IP = 1 in the reference), is walked again
+//    with the reference's arrays (bunkers_full).
 //  * the same walk sums the helicity terms up to the first level above
 //    `depth` with the storm motion factored out (it is known only once the
 //    walk has ended); it enters the sum at the end.
