@@ -112,6 +112,22 @@ def test_levels_above_the_outputs(hip, oracle_lib):
     assert np.any(o[2][200:300] != ref[2][200:300])
     assert np.any(o[2][300:400] != ref[2][300:400])
     _cmp_srh(g, o, 'levels above the outputs')
+    # the FP64 re-run takes the same paths (its storm motion in double): the
+    # untouched columns within the usual bar; on the constructed ones the
+    # helicity and the NaN pattern, as their Bunkers motion differs from the
+    # reference's FP32 one by that one's rounding at large magnitudes (NaN
+    # heights leave targets unbracketed, and the mean wind then sums their
+    # -999999 fill values: |mean| ~ 5e5 m/s, float spacing ~0.06 m/s)
+    g64 = hip.srh(*args, depth=3000., fp64=True)
+    for x, y in zip(g64, o):
+        np.testing.assert_array_equal(np.isnan(x), np.isnan(y))
+    _cmp_srh([x[500:] for x in g64], [y[500:] for y in o],
+             'levels above the outputs, FP64 chain, untouched columns')
+    d_bk = max(np.nanmax(np.abs(g64[j][:500] - o[j][:500])) for j in (2, 3, 4))
+    print(f'levels above the outputs, FP64 chain, constructed columns: '
+          f'max|dBunkers|={d_bk:.3g}', flush=True)
+    for j in (0, 1):
+        assert np.nanmax(np.abs(g64[j][:500] - o[j][:500])) <= SRH_ABS
 
 
 def test_pressure_synthetic_srh(hip, oracle_lib):
