@@ -187,10 +187,22 @@ int cape_block()
 // straight to the kernels; column-contiguous ones are staged level-major
 // first through the LDS-tiled transpose.  run_grid works in chunks of
 // columns whose scratch fits in about 16 GiB: per chunk, one transpose launch
-// for every profile array the call needs (shared by the CAPE and helicity
-// launches; none for level-major input), the warm-started ascent's scratch
-// (level environments and parcels), then the launches.
-hipError_t launch_grid(const CapeGridArgs &a, bool cape, bool srh, int math, hipStream_t s)
+// for the profile arrays the CAPE launches read (shared with the helicity
+// launch; none for level-major input), the warm-started ascent's scratch
+// (level environments and parcels), then the launches.  A combined call
+// copies the winds on the helicity's stream, where they run beside the CAPE
+// launches instead of ahead of them.
+// Profile copies only the helicity launch reads (the winds of a combined
+// call), made on whichever stream runs that launch.
+struct HelicityStage {
+    const float *src[2];
+    float *dst[2];
+    int narr;
+    int64_t n, nk;
+};
+
+hipError_t launch_grid(const CapeGridArgs &a, bool cape, bool srh, int math, hipStream_t s,
+                       const HelicityStage *hs = nullptr)
 {
     hipError_t e = hipSuccess;
     if (cape && srh) {
@@ -209,7 +221,8 @@ hipError_t launch_grid(const CapeGridArgs &a, bool cape, bool srh, int math, hip
             if ((e = hipEventRecord(fork, s)) != hipSuccess ||
                 (e = hipStreamWaitEvent(s2, fork, 0)) != hipSuccess)
                 return e;
-            e = launch_cape_grid_accurate(a, false, GRID_SRH, 256, s2);
+            if (hs) e = launch_transpose_levels(hs->src, hs->dst, hs->narr, hs->n, hs->nk, s2);
+            if (e == hipSuccess) e = launch_cape_grid_accurate(a, false, GRID_SRH, 256, s2);
             const hipError_t er = hipEventRecord(join, s2);
             if (e == hipSuccess) e = er;
             if (e == hipSuccess) e = launch_grid(a, true, false, math, s);
@@ -218,6 +231,9 @@ hipError_t launch_grid(const CapeGridArgs &a, bool cape, bool srh, int math, hip
             return e != hipSuccess ? e : ej;
         }
     }
+    if (hs && (e = launch_transpose_levels(hs->src, hs->dst, hs->narr, hs->n, hs->nk, s)) !=
+                  hipSuccess)
+        return e;
     if (cape) {
         const int block = cape_block();
         if (math == 2)
@@ -299,6 +315,7 @@ hipError_t run_grid(const CapeGridArgs &a0, bool cape, bool srh, int math, hipSt
         const int64_t n = chunk < a0.ncol - c0 ? chunk : a0.ncol - c0;
         CapeGridArgs a = a0;
         a.ncol = n;
+        HelicityStage hs_, *hs = nullptr;
         if (stage) {
             const int64_t o = c0 * nk;
             const float *src[5];
@@ -307,7 +324,14 @@ hipError_t run_grid(const CapeGridArgs &a0, bool cape, bool srh, int math, hipSt
                 src[i] = src0[i] + o;
                 dst[i] = scratch + (size_t)i * (size_t)nk * (size_t)n;
             }
-            if ((e = launch_transpose_levels(src, dst, narr, n, nk, s)) != hipSuccess) return e;
+            // in a combined call the winds are copied on the helicity's stream,
+            // beside the CAPE launches
+            const int nfirst = cape && srh ? narr - 2 : narr;
+            if ((e = launch_transpose_levels(src, dst, nfirst, n, nk, s)) != hipSuccess) return e;
+            if (nfirst < narr) {
+                hs_ = HelicityStage{{src[iu], src[iv]}, {dst[iu], dst[iv]}, 2, n, nk};
+                hs = &hs_;
+            }
             a.ls = n;
             a.t = dst[it];
             a.td = dst[itd];
@@ -350,7 +374,7 @@ hipError_t run_grid(const CapeGridArgs &a0, bool cape, bool srh, int math, hipSt
                 a.mean6 = static_cast<char *>(a.mean6) + b;
             }
         }
-        if ((e = launch_grid(a, cape, srh, math, s)) != hipSuccess) return e;
+        if ((e = launch_grid(a, cape, srh, math, s, hs)) != hipSuccess) return e;
     }
     return hipSuccess;
 }
