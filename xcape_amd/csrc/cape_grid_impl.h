@@ -624,6 +624,9 @@ hipError_t launch_warm_s(const CapeGridArgs &a, int block, hipStream_t s)
     warm_setup_kernel<ICE, PSEUDO, SUR, STATS><<<sgrid, 256, lds, s>>>(a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    if (a.setup_done &&
+        (e = hipEventRecord(static_cast<hipEvent_t>(a.setup_done), s)) != hipSuccess)
+        return e;
     const int grid = ascent_grid<ICE, PSEUDO, PREDICT, STATS, LEVELS>(block, a.ncol);
     // the counters start at their parts' first columns, for the grid's parts
     warm_reset_kernel<<<1, 64, 0, s>>>(a.work, (uint32_t)a.ncol, (uint32_t)grid);

@@ -53,6 +53,9 @@ struct CapeGridArgs {
     float *env_p;                // 1 float per level (LFC/EL calls), or nullptr
     float *par;                  // 16 floats per column
     unsigned int *work;          // 8 counters
+    // host side: an event (hipEvent_t) the warm-started launch records on its
+    // stream after the setup pass, or nullptr
+    void *setup_done;
 };
 
 // floats of warm-start scratch per column (env, env_p when `levels`, par)

@@ -191,7 +191,8 @@ int cape_block()
 // launch; none for level-major input), the warm-started ascent's scratch
 // (level environments and parcels), then the launches.  A combined call
 // copies the winds on the helicity's stream, where they run beside the CAPE
-// launches instead of ahead of them.
+// launches (beside the warm-started ascent, after its setup pass) instead of
+// ahead of them.
 // Profile copies only the helicity launch reads (the winds of a combined
 // call), made on whichever stream runs that launch.
 struct HelicityStage {
@@ -212,11 +213,30 @@ hipError_t launch_grid(const CapeGridArgs &a, bool cape, bool srh, int math, hip
         hipStream_t s2 = aux_stream();
         // the thread's own fork/join events, created once (recording an
         // event again only moves it to the new point of its stream)
-        static thread_local hipEvent_t fork = nullptr, join = nullptr;
+        static thread_local hipEvent_t fork = nullptr, join = nullptr, setup = nullptr;
         if (!fork && hipEventCreateWithFlags(&fork, hipEventDisableTiming) != hipSuccess)
             fork = nullptr;
         if (!join && hipEventCreateWithFlags(&join, hipEventDisableTiming) != hipSuccess)
             join = nullptr;
+        if (!setup && hipEventCreateWithFlags(&setup, hipEventDisableTiming) != hipSuccess)
+            setup = nullptr;
+        if (s2 && s2 != s && fork && join && hs && setup && a.warm != 0 && math != 2) {
+            // warm start with the winds to copy: the second stream starts after
+            // the setup pass, so that the copies run beside the ascent (bound by
+            // VALU issue) rather than beside the setup pass (which moves about
+            // as many bytes per microsecond as they do)
+            CapeGridArgs ac = a;
+            ac.setup_done = setup;
+            if ((e = launch_grid(ac, true, false, math, s)) != hipSuccess) return e;
+            if ((e = hipStreamWaitEvent(s2, setup, 0)) != hipSuccess) return e;
+            e = launch_transpose_levels(hs->src, hs->dst, hs->narr, hs->n, hs->nk, s2);
+            if (e == hipSuccess) e = launch_cape_grid_accurate(a, false, GRID_SRH, 256, s2);
+            const hipError_t er = hipEventRecord(join, s2);
+            if (e == hipSuccess) e = er;
+            // join even when a launch failed: s must not run ahead of s2
+            const hipError_t ej = hipStreamWaitEvent(s, join, 0);
+            return e != hipSuccess ? e : ej;
+        }
         if (s2 && s2 != s && fork && join) {
             if ((e = hipEventRecord(fork, s)) != hipSuccess ||
                 (e = hipStreamWaitEvent(s2, fork, 0)) != hipSuccess)
