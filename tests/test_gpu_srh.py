@@ -171,6 +171,31 @@ def test_fp64_storm_motion_rerun(hip, oracle_lib, p_shared):
     _cmp_srh(g64, o, f'fp64 chain vs oracle (p_shared={p_shared})')
 
 
+def test_srh_in_parts_matches_one_launch(hip, monkeypatch):
+    """A helicity-only call on 2^18 column-contiguous columns or more is
+    staged in four parts, each part's helicity launch on a second stream
+    beside the next part's copies; forced here down to 128 columns (3000
+    columns: parts of 768, the last of 696).  The outputs equal the one-launch
+    call's: sigma and shared pressure levels, FP32 and the FP64 re-run."""
+    c = synthetic.sigma_columns(3000, 137, 31, winds=True)
+    pc = synthetic.pressure_columns(3000, 32, winds=True)
+    keys = ('p', 't', 'td', 'u', 'v', 'ps', 'ts', 'tds', 'us', 'vs')
+
+    def run():
+        out = [hip.srh(*(c[k] for k in keys), depth=d, fp64=f)
+               for d in (1000., 3000.) for f in (False, True)]
+        out.append(hip.srh(*(pc[k] for k in keys), depth=3000., p_shared=True))
+        return out
+
+    monkeypatch.delenv('XCAPE_STAGE_COLUMNS', raising=False)
+    monkeypatch.delenv('XCAPE_SRH_PIPE_COLUMNS', raising=False)
+    ref = run()
+    monkeypatch.setenv('XCAPE_SRH_PIPE_COLUMNS', '128')
+    for x, r in zip(run(), ref):
+        for a, b in zip(x, r):
+            np.testing.assert_array_equal(a, b)
+
+
 @pytest.mark.parametrize('fast', [False, True])
 def test_fused_cape_srh_matches_separate_calls(hip, fast):
     c = synthetic.sigma_columns(2048, 137, 1004, winds=True)

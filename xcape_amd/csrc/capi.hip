@@ -80,6 +80,19 @@ hipStream_t aux_stream()
     return s;
 }
 
+// The thread's events that order its two streams, created once (recording an
+// event again only moves it to the new point of its stream; a wait already
+// queued keeps the point it was queued for).
+enum { EV_FORK = 0, EV_JOIN, EV_SETUP, EV_COUNT };
+
+hipEvent_t thread_event(int i)
+{
+    static thread_local hipEvent_t ev[EV_COUNT] = {};
+    if (!ev[i] && hipEventCreateWithFlags(&ev[i], hipEventDisableTiming) != hipSuccess)
+        ev[i] = nullptr;
+    return ev[i];
+}
+
 __global__ void zero_u64(unsigned long long *p, int n)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -211,15 +224,8 @@ hipError_t launch_grid(const CapeGridArgs &a, bool cape, bool srh, int math, hip
         // (FP32-bound): forked onto the thread's second stream after the
         // work already queued on s, joined back before anything later on s
         hipStream_t s2 = aux_stream();
-        // the thread's own fork/join events, created once (recording an
-        // event again only moves it to the new point of its stream)
-        static thread_local hipEvent_t fork = nullptr, join = nullptr, setup = nullptr;
-        if (!fork && hipEventCreateWithFlags(&fork, hipEventDisableTiming) != hipSuccess)
-            fork = nullptr;
-        if (!join && hipEventCreateWithFlags(&join, hipEventDisableTiming) != hipSuccess)
-            join = nullptr;
-        if (!setup && hipEventCreateWithFlags(&setup, hipEventDisableTiming) != hipSuccess)
-            setup = nullptr;
+        const hipEvent_t fork = thread_event(EV_FORK), join = thread_event(EV_JOIN),
+                         setup = thread_event(EV_SETUP);
         if (s2 && s2 != s && fork && join && hs && setup && a.warm != 0 && math != 2) {
             // warm start with the winds to copy: the second stream starts after
             // the setup pass, so that the copies run beside the ascent (bound by
@@ -281,6 +287,21 @@ int64_t grid_scratch_floats(const CapeGridArgs &a0, bool cape, bool srh, int mat
            (warm ? warm_scratch_floats(a0.nk_in, a0.p_lfc != nullptr) : 0);
 }
 
+// A helicity-only call on at least this many column-contiguous columns
+// (XCAPE_SRH_PIPE_COLUMNS overrides it, read on every call; for tests) is
+// staged and computed in SRH_PARTS parts: each part's helicity launch runs on
+// the thread's second stream while the next part's profiles are copied.
+constexpr int SRH_PARTS = 4;
+
+int64_t srh_pipe_min_columns()
+{
+    if (const char *env = getenv("XCAPE_SRH_PIPE_COLUMNS")) {
+        const long long v = atoll(env);
+        if (v > 0) return v;
+    }
+    return (int64_t)1 << 18;
+}
+
 // ext/ext_work: caller-owned scratch (grid_scratch_floats per column of the
 // whole call, which holds any chunk's, and 8 counters) used instead of
 // allocating; NULL to allocate.
@@ -318,13 +339,28 @@ hipError_t run_grid(const CapeGridArgs &a0, bool cape, bool srh, int math, hipSt
         if (chunk > cap) chunk = cap;
     }
     if (chunk > a0.ncol) chunk = a0.ncol;
+    // the helicity-only call in parts (each part's copies in a region of its
+    // own, so that one part's helicity launch can still read them while the
+    // next part is copied), when the whole call fits one chunk's scratch
+    hipStream_t s2 = nullptr;
+    int64_t parts = 1;
+    if (!cape && srh && stage && !ext && chunk >= a0.ncol &&
+        a0.ncol >= srh_pipe_min_columns() && a0.ncol >= 2 * 64 &&
+        (s2 = aux_stream()) != nullptr && s2 != s && thread_event(EV_FORK) &&
+        thread_event(EV_JOIN)) {
+        int64_t c = (a0.ncol + SRH_PARTS - 1) / SRH_PARTS;
+        c = (c + 63) & ~(int64_t)63;   // parts of whole waves
+        chunk = c;
+        parts = (a0.ncol + c - 1) / c;
+    }
     DevBuf<float> scratch_;
     DevBuf<unsigned int> work_;
     float *scratch = ext;
     unsigned int *work = ext_work;
     hipError_t e = hipSuccess;
     if (!ext) {
-        if ((e = scratch_.alloc((size_t)(narr * nk + wfl) * (size_t)chunk, s)) != hipSuccess)
+        if ((e = scratch_.alloc((size_t)(narr * nk + wfl) * (size_t)chunk * (size_t)parts,
+                                s)) != hipSuccess)
             return e;
         if (warm && (e = work_.alloc(8, s)) != hipSuccess) return e;   // WORK_PARTS counters
         scratch = scratch_.ptr;
@@ -340,9 +376,11 @@ hipError_t run_grid(const CapeGridArgs &a0, bool cape, bool srh, int math, hipSt
             const int64_t o = c0 * nk;
             const float *src[5];
             float *dst[5];
+            // (parts > 1: the part's own region)
+            float *cs = scratch + (parts > 1 ? (size_t)narr * (size_t)nk * (size_t)c0 : 0);
             for (int i = 0; i < narr; ++i) {
                 src[i] = src0[i] + o;
-                dst[i] = scratch + (size_t)i * (size_t)nk * (size_t)n;
+                dst[i] = cs + (size_t)i * (size_t)nk * (size_t)n;
             }
             // in a combined call the winds are copied on the helicity's stream,
             // beside the CAPE launches
@@ -393,6 +431,23 @@ hipError_t run_grid(const CapeGridArgs &a0, bool cape, bool srh, int math, hipSt
                 a.lm = static_cast<char *>(a.lm) + b;
                 a.mean6 = static_cast<char *>(a.mean6) + b;
             }
+        }
+        if (parts > 1) {
+            // this part's helicity on s2, once its copies are done; the next
+            // part's copies go on s meanwhile
+            const hipEvent_t fork = thread_event(EV_FORK);
+            if ((e = hipEventRecord(fork, s)) == hipSuccess &&
+                (e = hipStreamWaitEvent(s2, fork, 0)) == hipSuccess)
+                e = launch_cape_grid_accurate(a, math == 2, GRID_SRH, 256, s2);
+            if (e != hipSuccess || c0 + n >= a0.ncol) {
+                // after the last part, or a failed one: s waits for s2
+                const hipEvent_t join = thread_event(EV_JOIN);
+                const hipError_t er = hipEventRecord(join, s2);
+                const hipError_t ew = hipStreamWaitEvent(s, join, 0);
+                if (e == hipSuccess) e = er != hipSuccess ? er : ew;
+            }
+            if (e != hipSuccess) return e;
+            continue;
         }
         if ((e = launch_grid(a, cape, srh, math, s, hs)) != hipSuccess) return e;
     }
