@@ -224,7 +224,9 @@ int xcape_cape_levels_f64(const float *p, int p_shared, const int32_t *start_lev
  * m/s; with XCAPE_INPUT_F64 the ten input pointers point at double arrays.
  * Outputs: srh_rm, srh_lm (ncol,) FP64; rm, lm, mean6 (ncol, 2) as
  * (u, v) pairs (the reference's (2, n2) Fortran arrays), all three NULL to
- * skip them.
+ * skip them.  Grids of 2^18 column-contiguous columns or more are staged in
+ * four parts, each part's launch on a second stream of the calling thread
+ * while the next part is copied (joined back as above).
  */
 int xcape_srh_f32(const float *p, int p_shared, const int32_t *start_lev,
                   const float *t, const float *td, const float *u, const float *v,
@@ -248,7 +250,8 @@ int xcape_srh_f64(const float *p, int p_shared, const int32_t *start_lev,
  * CAPE/CIN (as xcape_cape_f32) and the helicity chain (as xcape_srh_f32) of
  * the same columns in one call: column-contiguous profiles are staged
  * level-major once for both, and the helicity launch runs on a second stream
- * beside the ascent (forked after the work queued on `stream`, joined back
+ * beside the ascent (forked after the work queued on `stream`, with the warm
+ * start after the setup pass, with the copies of the winds; joined back
  * before the call returns or anything later on `stream` runs).  No single
  * reference counterpart: it serves callers that need both calls above.
  */
