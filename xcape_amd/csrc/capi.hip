@@ -192,6 +192,15 @@ int cape_block()
     return 256;
 }
 
+// Profile copies only the helicity launch reads (the winds of a combined
+// call), made on whichever stream runs that launch.
+struct HelicityStage {
+    const float *src[2];
+    float *dst[2];
+    int narr;
+    int64_t n, nk;
+};
+
 // The grid kernels read level-major profiles: with a column per lane, 64
 // lanes reading 64 different columns of column-contiguous arrays fetch 64
 // cache lines per level each, and on large grids the L2 cannot hold them
@@ -206,15 +215,6 @@ int cape_block()
 // copies the winds on the helicity's stream, where they run beside the CAPE
 // launches (beside the warm-started ascent, after its setup pass) instead of
 // ahead of them.
-// Profile copies only the helicity launch reads (the winds of a combined
-// call), made on whichever stream runs that launch.
-struct HelicityStage {
-    const float *src[2];
-    float *dst[2];
-    int narr;
-    int64_t n, nk;
-};
-
 hipError_t launch_grid(const CapeGridArgs &a, bool cape, bool srh, int math, hipStream_t s,
                        const HelicityStage *hs = nullptr)
 {
@@ -367,6 +367,19 @@ hipError_t run_grid(const CapeGridArgs &a0, bool cape, bool srh, int math, hipSt
         work = work_.ptr;
     }
     const int64_t esz = math == 2 ? 8 : 4;
+    // once a part's helicity launch is on s2, every return joins s2 back into
+    // s first: the scratch is freed on s, and s2 may still read it
+    bool forked = false;
+    const auto joined = [&](hipError_t err) {
+        if (forked) {
+            const hipEvent_t join = thread_event(EV_JOIN);
+            const hipError_t er = hipEventRecord(join, s2);
+            const hipError_t ew = hipStreamWaitEvent(s, join, 0);
+            forked = false;
+            if (err == hipSuccess) err = er != hipSuccess ? er : ew;
+        }
+        return err;
+    };
     for (int64_t c0 = 0; c0 < a0.ncol; c0 += chunk) {
         const int64_t n = chunk < a0.ncol - c0 ? chunk : a0.ncol - c0;
         CapeGridArgs a = a0;
@@ -385,7 +398,8 @@ hipError_t run_grid(const CapeGridArgs &a0, bool cape, bool srh, int math, hipSt
             // in a combined call the winds are copied on the helicity's stream,
             // beside the CAPE launches
             const int nfirst = cape && srh ? narr - 2 : narr;
-            if ((e = launch_transpose_levels(src, dst, nfirst, n, nk, s)) != hipSuccess) return e;
+            if ((e = launch_transpose_levels(src, dst, nfirst, n, nk, s)) != hipSuccess)
+                return joined(e);
             if (nfirst < narr) {
                 hs_ = HelicityStage{{src[iu], src[iv]}, {dst[iu], dst[iv]}, 2, n, nk};
                 hs = &hs_;
@@ -437,15 +451,12 @@ hipError_t run_grid(const CapeGridArgs &a0, bool cape, bool srh, int math, hipSt
             // part's copies go on s meanwhile
             const hipEvent_t fork = thread_event(EV_FORK);
             if ((e = hipEventRecord(fork, s)) == hipSuccess &&
-                (e = hipStreamWaitEvent(s2, fork, 0)) == hipSuccess)
+                (e = hipStreamWaitEvent(s2, fork, 0)) == hipSuccess) {
+                forked = true;
                 e = launch_cape_grid_accurate(a, math == 2, GRID_SRH, 256, s2);
-            if (e != hipSuccess || c0 + n >= a0.ncol) {
-                // after the last part, or a failed one: s waits for s2
-                const hipEvent_t join = thread_event(EV_JOIN);
-                const hipError_t er = hipEventRecord(join, s2);
-                const hipError_t ew = hipStreamWaitEvent(s, join, 0);
-                if (e == hipSuccess) e = er != hipSuccess ? er : ew;
             }
+            // after the last part, or a failed one: s waits for s2
+            if (e != hipSuccess || c0 + n >= a0.ncol) e = joined(e);
             if (e != hipSuccess) return e;
             continue;
         }
