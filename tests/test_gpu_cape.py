@@ -564,3 +564,50 @@ def test_staging_of_unaligned_and_even_level_inputs(hip):
         got = hip.cape(*shifted, *[torch.from_numpy(a).cuda() for a in args[3:]], **kw)
         for x, y in zip(got, ref):
             assert torch.equal(x.cpu(), y), nk
+
+
+def test_warm_parts_match_one_launch(hip, monkeypatch):
+    """Warm-started grids of 2^20 column-contiguous columns or more are staged
+    in four parts, each part's setup pass beside the next part's copies, then
+    one ascent over every column; forced here down to 128 columns (3000
+    columns: parts of 768, the last of 696).  Every output (LFC/EL too), the
+    per-column kernel counts and the fused call's helicity equal the one-launch
+    call's: the extrapolation, the surrogate's first iterates, the cold loop,
+    fast math, sigma and shared pressure levels.  Last in this file: it loads
+    the surrogate."""
+    from xcape_amd import surrogate
+    surrogate.Surrogate.load().activate()
+    c = synthetic.sigma_columns(3000, 137, 37, winds=True)
+    pc = synthetic.pressure_columns(3000, 38)
+    cols = ('p', 't', 'td', 'ps', 'ts', 'tds')
+    wcols = ('p', 't', 'td', 'u', 'v', 'ps', 'ts', 'tds', 'us', 'vs')
+    kw0 = dict(pinc=500., source=2, adiabat=3)
+    cases = [dict(warm_start='extrapolation', stats=True), dict(warm_start=True, stats=True),
+             dict(warm_start='cold', stats=True), dict(warm_start='extrapolation', levels=True),
+             dict(warm_start=True, fast_math=True, stats=True)]
+
+    def run():
+        out = [hip.cape(*(c[k] for k in cols), **kw0, **kw) for kw in cases]
+        out.append(hip.cape(*(pc[k] for k in cols), pinc=500., source=1, adiabat=1,
+                            p_shared=True, warm_start=True, stats=True))
+        for ws in ('extrapolation', True):
+            out.append(hip.cape_srh(*(c[k] for k in wcols), **kw0, depth=3000., warm_start=ws,
+                                    stats=True))
+        return out
+
+    monkeypatch.delenv('XCAPE_STAGE_COLUMNS', raising=False)
+    monkeypatch.delenv('XCAPE_WARM_PIPE_COLUMNS', raising=False)
+    ref = run()
+    monkeypatch.setenv('XCAPE_WARM_PIPE_COLUMNS', '128')
+    got = run()
+    for x, r in zip(got, ref):
+        assert len(x) == len(r)
+        for a, b in zip(x, r):
+            if isinstance(b, dict):
+                # the per-column counts (the wave-level ones follow the columns'
+                # packing into waves)
+                per_col = [k for k in b if k not in ('wave_trips', 'blocks')]
+                assert {k: a[k] for k in per_col} == {k: b[k] for k in per_col}
+            else:
+                np.testing.assert_array_equal(a, b)
+    assert got[0][-1]['columns'] == 3000

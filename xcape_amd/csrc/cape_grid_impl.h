@@ -286,7 +286,10 @@ warm_setup_kernel(CapeGridArgs a)
     const bool ran = in_grid && c.ts > 0.0f;
     Parcel<T> pc;
     bool ok = false;
-    const uint32_t ls = (uint32_t)a.ncol, j = (uint32_t)col;
+    // records: row length ls, this column's place j in it (a setup pass over
+    // part of the records' columns: rec_ls, rec_col0)
+    const uint32_t ls = a.rec_ls ? (uint32_t)a.rec_ls : (uint32_t)a.ncol;
+    const uint32_t j = (uint32_t)(col + a.rec_col0);
     float4 *env = reinterpret_cast<float4 *>(a.env);
     float *envp = a.env_p;
     const T pinc = T(a.pinc);
@@ -320,7 +323,7 @@ warm_setup_kernel(CapeGridArgs a)
         }
         // (the scan stored level kmax + 1 already)
         if (cover > pc.kmax + 1) r1 = env[(uint32_t)(pc.kmax + 1) * ls + j];
-        float4 *r4 = reinterpret_cast<float4 *>(reinterpret_cast<ParcelRec *>(a.par) + col);
+        float4 *r4 = reinterpret_cast<float4 *>(reinterpret_cast<ParcelRec *>(a.par) + j);
         r4[0] = make_float4(__int_as_float(pc.kmax), __int_as_float(nk), pc.th2, pc.t2);
         r4[2] = make_float4(pc.e.p, pc.e.pi, pc.e.thv, 0.0f);
         r4[3] = r1;
@@ -343,7 +346,7 @@ warm_setup_kernel(CapeGridArgs a)
     } else if (in_grid) {
         // ts <= 0 degC: zeros; no level above the parcel (or an unknown
         // source): CAPE = CIN = 0 and the height's initial -999999
-        reinterpret_cast<ParcelRec *>(a.par)[col].kmax = -1;
+        reinterpret_cast<ParcelRec *>(a.par)[j].kmax = -1;
         T *cape = static_cast<T *>(a.cape), *cin = static_cast<T *>(a.cin),
           *zout = static_cast<T *>(a.zout);
         const int64_t oc = out_col(a, col);
@@ -364,7 +367,7 @@ warm_setup_kernel(CapeGridArgs a)
                                             f1, f2,
                                             dyn_lds + (threadIdx.x >> 6) * MLP_LDS_WORDS_PER_WAVE);
         if (go) {
-            float4 *r4 = reinterpret_cast<float4 *>(reinterpret_cast<ParcelRec *>(a.par) + col);
+            float4 *r4 = reinterpret_cast<float4 *>(reinterpret_cast<ParcelRec *>(a.par) + j);
             r4[1] = make_float4(pc.qv2, pc.b2, pc.zk, lp1 != T(0.0) ? pc.th2 + r * lp1 : T(NAN));
         }
     }
@@ -621,12 +624,15 @@ hipError_t launch_warm_s(const CapeGridArgs &a, int block, hipStream_t s)
 {
     const dim3 sgrid((unsigned)((a.ncol + 255) / 256));
     const size_t lds = SUR ? sizeof(uint32_t) * MLP_LDS_WORDS_PER_WAVE * 4 : 0;
-    warm_setup_kernel<ICE, PSEUDO, SUR, STATS><<<sgrid, 256, lds, s>>>(a);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    if (a.setup_done &&
-        (e = hipEventRecord(static_cast<hipEvent_t>(a.setup_done), s)) != hipSuccess)
-        return e;
+    hipError_t e = hipSuccess;
+    if (a.warm_phase != 2) {
+        warm_setup_kernel<ICE, PSEUDO, SUR, STATS><<<sgrid, 256, lds, s>>>(a);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if (a.setup_done &&
+            (e = hipEventRecord(static_cast<hipEvent_t>(a.setup_done), s)) != hipSuccess)
+            return e;
+        if (a.warm_phase == 1) return hipSuccess;
+    }
     const int grid = ascent_grid<ICE, PSEUDO, PREDICT, STATS, LEVELS>(block, a.ncol);
     // the counters start at their parts' first columns, for the grid's parts
     warm_reset_kernel<<<1, 64, 0, s>>>(a.work, (uint32_t)a.ncol, (uint32_t)grid);

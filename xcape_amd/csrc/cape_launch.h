@@ -56,6 +56,12 @@ struct CapeGridArgs {
     // host side: an event (hipEvent_t) the warm-started launch records on its
     // stream after the setup pass, or nullptr
     void *setup_done;
+    // host side: the warm-started launch's phases, 0 both, 1 the setup pass
+    // only, 2 the counter reset and ascent only
+    int warm_phase;
+    // a setup pass over part of the records' columns: the records' row length
+    // (0: ncol) and the part's first column in it (parcel records too)
+    int64_t rec_ls, rec_col0;
 };
 
 // floats of warm-start scratch per column (env, env_p when `levels`, par)
@@ -71,7 +77,7 @@ enum { GRID_CAPE = 1, GRID_SRH = 2, GRID_CAPE_SRH = 3 };
 // dst[j][i] = src[i][j] for up to 5 arrays, LDS-tiled so that both the
 // reads and the writes are coalesced.
 hipError_t launch_transpose_levels(const float *const *src, float *const *dst, int narr,
-                                   int64_t ncol, int64_t nk, hipStream_t s);
+                                   int64_t ncol, int64_t nk, hipStream_t s, int64_t ld = -1);
 
 hipError_t launch_cape_grid_accurate(const CapeGridArgs &a, bool fp64, int kind, int block,
                                      hipStream_t s);

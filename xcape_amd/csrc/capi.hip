@@ -83,7 +83,8 @@ hipStream_t aux_stream()
 // The thread's events that order its two streams, created once (recording an
 // event again only moves it to the new point of its stream; a wait already
 // queued keeps the point it was queued for).
-enum { EV_FORK = 0, EV_JOIN, EV_SETUP, EV_COUNT };
+constexpr int WARM_PARTS = 4;   // parts of a large warm-started grid's staging
+enum { EV_FORK = 0, EV_JOIN, EV_SETUP, EV_PART, EV_COUNT = EV_PART + WARM_PARTS };
 
 hipEvent_t thread_event(int i)
 {
@@ -302,6 +303,117 @@ int64_t srh_pipe_min_columns()
     return (int64_t)1 << 18;
 }
 
+// Warm-started grids of at least this many column-contiguous columns
+// (XCAPE_WARM_PIPE_COLUMNS overrides it, read on every call; for tests) are
+// staged in WARM_PARTS parts.
+int64_t warm_pipe_min_columns()
+{
+    if (const char *env = getenv("XCAPE_WARM_PIPE_COLUMNS")) {
+        const long long v = atoll(env);
+        if (v > 0) return v;
+    }
+    return (int64_t)1 << 20;
+}
+
+hipError_t launch_cape_only(const CapeGridArgs &a, int math, hipStream_t s)
+{
+    const int block = cape_block();
+    if (math == 1) return launch_cape_grid_fast(a, GRID_CAPE, block, s);
+    return launch_cape_grid_accurate(a, math == 2, GRID_CAPE, block, s);
+}
+
+// A large warm-started grid of column-contiguous profiles, staged in parts:
+// part i's setup pass runs on s while part i + 1's profiles are copied on the
+// thread's second stream s2, then one counter reset and ascent cover every
+// column.  The copies fill the whole grid's level-major arrays dst (rows of
+// a.ncol; a's profile pointers are those arrays), and each setup pass its
+// part's columns of the records.  In a combined call the winds (hs) are then
+// copied and the helicity launched on s2, after the last setup pass, beside
+// the ascent.  Columns are independent, so the outputs are those of the
+// one-launch call.
+hipError_t launch_warm_parts(const CapeGridArgs &a, bool srh, int math, hipStream_t s,
+                             hipStream_t s2, const float *const *src, float *const *dst,
+                             int narr, const HelicityStage *hs)
+{
+    const int64_t n = a.ncol, nk = a.nk_in, esz = 4;
+    int64_t c = (n + WARM_PARTS - 1) / WARM_PARTS;
+    c = (c + 63) & ~(int64_t)63;   // parts of whole waves
+    const int parts = (int)((n + c - 1) / c);
+    // once anything is queued on s2, every return joins s2 back into s
+    bool forked = false;
+    const auto joined = [&](hipError_t err) {
+        if (forked) {
+            const hipEvent_t join = thread_event(EV_JOIN);
+            const hipError_t er = hipEventRecord(join, s2);
+            const hipError_t ew = hipStreamWaitEvent(s, join, 0);
+            if (err == hipSuccess) err = er != hipSuccess ? er : ew;
+        }
+        return err;
+    };
+    const auto copies = [&](int64_t c0, hipStream_t st) {
+        const int64_t m = c < n - c0 ? c : n - c0;
+        const float *ps[5];
+        float *pd[5];
+        for (int i = 0; i < narr; ++i) {
+            ps[i] = src[i] + c0 * nk;
+            pd[i] = dst[i] + c0;
+        }
+        return launch_transpose_levels(ps, pd, narr, m, nk, st, n);
+    };
+    hipError_t e = copies(0, s);
+    if (e != hipSuccess) return e;
+    const hipEvent_t fork = thread_event(EV_FORK);
+    if ((e = hipEventRecord(fork, s)) != hipSuccess ||
+        (e = hipStreamWaitEvent(s2, fork, 0)) != hipSuccess)
+        return e;
+    forked = true;
+    for (int i = 1; i < parts; ++i) {
+        if ((e = copies(i * c, s2)) != hipSuccess ||
+            (e = hipEventRecord(thread_event(EV_PART + i), s2)) != hipSuccess)
+            return joined(e);
+    }
+    for (int i = 0; i < parts; ++i) {
+        const int64_t c0 = i * c, m = c < n - c0 ? c : n - c0;
+        if (i > 0 && (e = hipStreamWaitEvent(s, thread_event(EV_PART + i), 0)) != hipSuccess)
+            return joined(e);
+        CapeGridArgs b = a;
+        b.ncol = m;
+        // level k of column c at [k * ls + c]: the part's columns start c0 into each row
+        b.t += c0;
+        b.td += c0;
+        if (b.p) b.p += c0;
+        b.ps += c0; b.ts += c0; b.tds += c0;
+        if (b.start) b.start += c0;
+        b.cape = static_cast<char *>(b.cape) + c0 * esz;
+        b.cin = static_cast<char *>(b.cin) + c0 * esz;
+        b.zout = static_cast<char *>(b.zout) + c0 * esz;
+        b.mulvl += c0;
+        if (b.p_lfc) {
+            b.p_lfc = static_cast<char *>(b.p_lfc) + c0 * esz;
+            b.p_el = static_cast<char *>(b.p_el) + c0 * esz;
+        }
+        b.rec_ls = n;
+        b.rec_col0 = c0;
+        b.warm_phase = 1;
+        b.setup_done = nullptr;
+        if ((e = launch_cape_only(b, math, s)) != hipSuccess) return joined(e);
+    }
+    if (srh) {
+        const hipEvent_t setup = thread_event(EV_SETUP);
+        if ((e = hipEventRecord(setup, s)) != hipSuccess ||
+            (e = hipStreamWaitEvent(s2, setup, 0)) != hipSuccess)
+            return joined(e);
+        if (hs && (e = launch_transpose_levels(hs->src, hs->dst, hs->narr, hs->n, hs->nk,
+                                               s2)) != hipSuccess)
+            return joined(e);
+        if ((e = launch_cape_grid_accurate(a, false, GRID_SRH, 256, s2)) != hipSuccess)
+            return joined(e);
+    }
+    CapeGridArgs z = a;
+    z.warm_phase = 2;
+    return joined(launch_cape_only(z, math, s));
+}
+
 // ext/ext_work: caller-owned scratch (grid_scratch_floats per column of the
 // whole call, which holds any chunk's, and 8 counters) used instead of
 // allocating; NULL to allocate.
@@ -353,6 +465,15 @@ hipError_t run_grid(const CapeGridArgs &a0, bool cape, bool srh, int math, hipSt
         chunk = c;
         parts = (a0.ncol + c - 1) / c;
     }
+    // a large warm-started grid: staged in parts, each beside a setup pass
+    bool wparts = false;
+    if (warm && stage && !ext && chunk >= a0.ncol && a0.ncol >= warm_pipe_min_columns() &&
+        a0.ncol >= 2 * 64 && !a0.nk_col && !a0.src_col && !a0.out_index) {
+        s2 = aux_stream();
+        wparts = s2 && s2 != s && thread_event(EV_FORK) && thread_event(EV_JOIN) &&
+                 thread_event(EV_SETUP) && thread_event(EV_PART + WARM_PARTS - 1);
+        for (int i = 1; wparts && i < WARM_PARTS; ++i) wparts = thread_event(EV_PART + i);
+    }
     DevBuf<float> scratch_;
     DevBuf<unsigned int> work_;
     float *scratch = ext;
@@ -385,10 +506,11 @@ hipError_t run_grid(const CapeGridArgs &a0, bool cape, bool srh, int math, hipSt
         CapeGridArgs a = a0;
         a.ncol = n;
         HelicityStage hs_, *hs = nullptr;
+        const float *src[5];
+        float *dst[5];
+        int nfirst = 0;
         if (stage) {
             const int64_t o = c0 * nk;
-            const float *src[5];
-            float *dst[5];
             // (parts > 1: the part's own region)
             float *cs = scratch + (parts > 1 ? (size_t)narr * (size_t)nk * (size_t)c0 : 0);
             for (int i = 0; i < narr; ++i) {
@@ -396,9 +518,10 @@ hipError_t run_grid(const CapeGridArgs &a0, bool cape, bool srh, int math, hipSt
                 dst[i] = cs + (size_t)i * (size_t)nk * (size_t)n;
             }
             // in a combined call the winds are copied on the helicity's stream,
-            // beside the CAPE launches
-            const int nfirst = cape && srh ? narr - 2 : narr;
-            if ((e = launch_transpose_levels(src, dst, nfirst, n, nk, s)) != hipSuccess)
+            // beside the CAPE launches; a grid in parts is copied part by part
+            nfirst = cape && srh ? narr - 2 : narr;
+            if (!wparts && (e = launch_transpose_levels(src, dst, nfirst, n, nk, s)) !=
+                               hipSuccess)
                 return joined(e);
             if (nfirst < narr) {
                 hs_ = HelicityStage{{src[iu], src[iv]}, {dst[iu], dst[iv]}, 2, n, nk};
@@ -460,7 +583,9 @@ hipError_t run_grid(const CapeGridArgs &a0, bool cape, bool srh, int math, hipSt
             if (e != hipSuccess) return e;
             continue;
         }
-        if ((e = launch_grid(a, cape, srh, math, s, hs)) != hipSuccess) return e;
+        if (wparts) e = launch_warm_parts(a, srh, math, s, s2, src, dst, nfirst, hs);
+        else e = launch_grid(a, cape, srh, math, s, hs);
+        if (e != hipSuccess) return e;
     }
     return hipSuccess;
 }

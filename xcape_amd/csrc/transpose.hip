@@ -31,6 +31,7 @@ struct TransposeArgs {
     const float *src[MAX_ARR];
     float *dst[MAX_ARR];
     int64_t ncol, nk;
+    int64_t ld;   // the destination's row length (>= ncol)
 };
 
 // blockIdx.x: column tile, blockIdx.y: level tile, blockIdx.z: array
@@ -53,7 +54,7 @@ __global__ void __launch_bounds__(256) transpose_levels_kernel(TransposeArgs a)
 #pragma unroll 4
     for (int r = ty; r < TILE; r += 4) {
         const int64_t kk = k0 + r;
-        if (c < a.ncol && kk < a.nk) dst[kk * a.ncol + c] = tile[tx][r];
+        if (c < a.ncol && kk < a.nk) dst[kk * a.ld + c] = tile[tx][r];
     }
 }
 
@@ -105,16 +106,17 @@ __global__ void __launch_bounds__(256) transpose_slab_kernel(TransposeArgs a, in
         float *d = dst + c0 + lane;
 #pragma unroll 4
         for (int kk = w; kk < nk; kk += 4)
-            __builtin_nontemporal_store(slab[lane * pitch + kk], &d[(int64_t)kk * a.ncol]);
+            __builtin_nontemporal_store(slab[lane * pitch + kk], &d[(int64_t)kk * a.ld]);
     }
 }
 
 }  // namespace
 
 hipError_t launch_transpose_levels(const float *const *src, float *const *dst, int narr,
-                                   int64_t ncol, int64_t nk, hipStream_t s)
+                                   int64_t ncol, int64_t nk, hipStream_t s, int64_t ld)
 {
-    if (narr < 1 || narr > MAX_ARR || ncol < 0 || nk < 0) return hipErrorInvalidValue;
+    if (ld < 0) ld = ncol;
+    if (narr < 1 || narr > MAX_ARR || ncol < 0 || nk < 0 || ld < ncol) return hipErrorInvalidValue;
     if (ncol == 0 || nk == 0) return hipSuccess;
     const int64_t gx = (ncol + TILE - 1) / TILE, gy = (nk + TILE - 1) / TILE;
     if (gx > 0x7fffffff || gy > 65535) return hipErrorInvalidValue;
@@ -125,6 +127,7 @@ hipError_t launch_transpose_levels(const float *const *src, float *const *dst, i
     }
     a.ncol = ncol;
     a.nk = nk;
+    a.ld = ld;
     const int64_t pitch = nk | 1;   // odd: the column-wise LDS reads hit distinct banks
     if (pitch <= SLAB_MAX_PITCH) {
         const int64_t sx = (ncol + SLAB - 1) / SLAB;
