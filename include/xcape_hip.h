@@ -16,7 +16,11 @@
  * `stream` (a hipStream_t; NULL is HIP's default stream) unless `stats` is
  * non-NULL (the counters are then read back synchronously).  Host-pointer
  * calls are synchronous; with stream NULL they run on a stream of the calling
- * thread's own.
+ * thread's own.  Some calls also queue work on a second stream of the calling
+ * thread (the helicity launches of the combined call, and the staging copies
+ * of a warm-started or helicity-only grid of many column-contiguous columns),
+ * forked after the work already queued on `stream` and joined back into
+ * `stream` before the call returns, so a call stays ordered as one unit.
  */
 #ifndef XCAPE_HIP_H
 #define XCAPE_HIP_H
